@@ -1,0 +1,136 @@
+"""Benchmark of the device Monte-Carlo sweep on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--paths P] [--workload NAME]
+
+A step is one device sweep of the workload's book over `paths` Sobol paths
+(the books are prepared and resident in HBM before the timed region: the LSV
+leverage calibration, grid and packing are setup).  Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from thoth_amd import abi, configs, mcl  # noqa: E402
+from thoth_amd.book import BuiltRequest  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (256 CU x 128 lanes x 2 x 2.4 GHz)
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic FLOPs per path (FMA = 2; add/mul/min/max/transcendental = 1;
+# integer Sobol work not counted): per LSV asset-step 2 normals x 35 + QE 22 +
+# Andersen 21 + leverage lookup 9 + Cholesky share 6 = 128; worst-of payoff 16;
+# Heston + discrete barrier step 105; one-step BS path 35 (FP64).
+FLOPS = {
+    "lsv_worst_of_basket": lambda steps, assets: steps * assets * 128 + 16,
+    "heston_down_out_call": lambda steps, assets: steps * 105,
+    "black_scholes_call": lambda steps, assets: 35,
+}
+
+
+def workload(name, paths):
+    if name == "lsv_worst_of_basket":
+        return configs.lsv_worst_of_basket(paths=paths, precision=abi.FP32)
+    if name == "heston_down_out_call":
+        return configs.heston_down_out_call(paths=paths, precision=abi.FP32)
+    if name == "black_scholes_call":
+        return configs.black_scholes_call(paths=paths)
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(name, sample_paths):
+    """The CPU FP64 oracle (a restatement of the reference engine, which is
+    single-threaded) timed on this host on a bounded sample of the workload."""
+    try:
+        sys.path.insert(0, str(ROOT))
+        from oracle import pyoracle
+        pyoracle.lib()
+    except Exception as e:  # the oracle is built on the CPU side; report if absent
+        return {"value": None, "unit": "paths/s", "cores": 1, "kind": "port",
+                "sample": "oracle unavailable: %s" % e}
+    spec = workload(name, sample_paths)
+    spec.precision = abi.FP64
+    sec, _ = pyoracle.sweep_seconds(BuiltRequest(spec))
+    return {"value": sample_paths / sec, "unit": "paths/s", "cores": 1, "kind": "port",
+            "sample": "%d paths of the same book, path sweep only (grid, LSV calibration excluded), "
+                      "%.1f s on %s" % (sample_paths, sec, platform.processor() or platform.machine())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--paths", type=int, default=1 << 20)
+    ap.add_argument("--workload", default="lsv_worst_of_basket")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build is single-GPU (one MI355X); --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    spec = workload(args.workload, args.paths)
+    built = BuiltRequest(spec)
+    t0 = time.perf_counter()
+    plan = mcl.Plan(built)
+    setup_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        plan.run()
+    wall, dev = [], []
+    res = None
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        res = plan.run()
+        wall.append(time.perf_counter() - t)
+        dev.append(res.sweep_ms)
+    total = sum(wall)
+    ms_step = 1e3 * total / args.steps
+    value = args.paths * args.steps / total
+    kernel_ms = sum(dev) / len(dev)
+    _, t, _ = mcl.grid(built)
+    steps = len(t) - 1
+    flops = args.paths * FLOPS[args.workload](steps, len(spec.equities))
+    fp32 = spec.precision == abi.FP32
+    peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    line = {
+        "metric": "paths/s & time-to-1bp CI: LSV basket, learned-CV FP32 vs plain MC, 1 MI355X",
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if fp32 else "f64",
+        "data": "synthetic (seeded books, Joe-Kuo Sobol paths)",
+        "config": {"workload": args.workload, "paths": args.paths, "steps": steps,
+                   "assets": len(spec.equities), "precision": "fp32 stepping, fp64 accumulation" if fp32 else "fp64"},
+        "kernel_ms": kernel_ms,
+        "setup_s": setup_s,
+        "premium": res.premium,
+        "premium_trust": res.premium_trust,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None},
+        "device": mcl.device_info(),
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(args.workload, args.cpu_sample)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

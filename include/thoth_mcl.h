@@ -1,0 +1,177 @@
+/*
+ * thoth_mcl.h -- C-ABI of the book-level Monte-Carlo engine (host side).
+ *
+ * A request describes one `!mcl_pricer` book in the reference's own units:
+ * the `!mcl_configuration` fields, the equities with their bs / heston / lsv
+ * volatility (vols and rates quoted in percent, as the YAML quotes them), the
+ * spot/spot correlation matrix, the yield curve, the rainbow underlyings and the
+ * vanilla / barrier contracts.  `thoth_mcl_price` runs the whole pricing
+ * (diffusion grid, Cholesky, LSV leverage calibration, device path sweep,
+ * bump-and-revalue Greeks) and fills caller-owned results.
+ *
+ * It replaces, for GPU-eligible books, the body of PricerMCL::PriceBook +
+ * PricerMCL::ComputeGreeks (reference: pricer/src/tasks/pricer_mcl.cpp:197-240,
+ * pricer/src/tasks/pricer_mcl_greeks.cpp:114-299, pricer/src/tasks/pricer.cpp:273-391).
+ * Field semantics follow the reference objects cited on each struct.
+ *
+ * Errors: every entry point returns a THOTH_* status; thoth_last_error() holds
+ * the message of the last failure on the calling thread.  No CPU fallback is
+ * taken: without a usable device the call returns THOTH_ERR_UNAVAILABLE and the
+ * caller keeps its own CPU engine (the reference gate's contract,
+ * pricer/src/tasks/pricer_mcl_gpu_gate.cpp:82-105).
+ */
+#ifndef THOTH_MCL_H
+#define THOTH_MCL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "thoth_mcl_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THOTH_MAX_PILLARS 16
+
+/* volatility kinds (reference: vol_correl/{bs,heston,lsv}_volatility.hpp) */
+enum { THOTH_VOL_BS = 0, THOTH_VOL_HESTON = 1, THOTH_VOL_LSV = 2 };
+/* contract kinds (reference: contracts/{vanilla,barrier}.hpp) */
+enum { THOTH_VANILLA = 0, THOTH_BARRIER = 1 };
+/* barrier types (reference: helpers/constants.hpp BARRIER_TYPE_*) */
+enum { THOTH_UP_OUT = 0, THOTH_UP_IN = 1, THOTH_DOWN_OUT = 2, THOTH_DOWN_IN = 3 };
+/* Greek request bits (reference: Pricer::_request_*) */
+enum { THOTH_GREEK_DELTA = 1, THOTH_GREEK_GAMMA = 2, THOTH_GREEK_VEGA = 4, THOTH_GREEK_RHO = 8 };
+/* arithmetic of the device path sweep */
+enum { THOTH_FP64 = 0, THOTH_FP32 = 1 };
+
+/* !mcl_configuration (reference: tasks/mcl_configuration.hpp:11-42, .cpp:15-47) */
+typedef struct thoth_mcl_configuration {
+    int64_t paths;         /* > 1 and <= 2e8 */
+    int32_t max_day_step;  /* > 0 */
+    int32_t min_day_step;
+    double vol_year_step;  /* caps the step for stochastic-vol books (pricer_mcl.cpp:71-117) */
+    int32_t use_sobol;     /* the device engine supports the Sobol + bridge mode only */
+    int32_t seed;
+    int64_t sobol_skip;    /* leading Sobol points discarded (disjoint blocks) */
+    int32_t allow_gpu;
+} thoth_mcl_configuration;
+
+/* !sabr_volatility (reference: vol_correl/sabr_volatility.cpp:27-35) */
+typedef struct thoth_sabr_surface {
+    int32_t n; /* pillars */
+    double maturities[THOTH_MAX_PILLARS];
+    double alpha[THOTH_MAX_PILLARS];
+    double beta[THOTH_MAX_PILLARS];
+    double rho[THOTH_MAX_PILLARS];
+    double nu[THOTH_MAX_PILLARS];
+} thoth_sabr_surface;
+
+/* !equity + its volatility object (reference: marketdata/equity.cpp, vol_correl/ sources) */
+typedef struct thoth_asset {
+    double spot;
+    int32_t vol_kind;       /* THOTH_VOL_* */
+    double volatility;      /* bs: percent (bs_volatility.cpp:24) */
+    double init_vol;        /* heston/lsv: percent, v0 = (init_vol/100)^2 (heston_volatility.cpp:38) */
+    double long_vol;        /* heston/lsv: percent, theta = (long_vol/100)^2 */
+    double kappa;
+    double vol_of_vol;
+    double rho_spot_var;    /* correlation(<name>, <name>_var) (pricer.cpp:678-687) */
+    double dividend_yield;  /* flat continuous dividend curve, percent (0 = none) */
+    thoth_sabr_surface surface; /* lsv target implied surface */
+} thoth_asset;
+
+/* !yield_curve: pillars in days from today, zero rates in percent (curve.cpp:45-73) */
+typedef struct thoth_curve {
+    int32_t n;
+    int32_t days[THOTH_MAX_PILLARS];
+    double values[THOTH_MAX_PILLARS];
+} thoth_curve;
+
+/* !rainbow over equities (reference: underlyings/rainbow.cpp:46-61, basket.cpp:31-38);
+ * the rebasing reference spots are the members' spots at load. */
+typedef struct thoth_rainbow {
+    int32_t best; /* 1 best_of (max), 0 worst_of (min) */
+    int32_t n;
+    int32_t members[THOTH_MCL_MAX_ASSETS];
+} thoth_rainbow;
+
+/* !vanilla / !barrier (reference: contracts/vanilla.cpp, contracts/barrier.cpp) */
+typedef struct thoth_contract {
+    int32_t kind;                 /* THOTH_VANILLA / THOTH_BARRIER */
+    int32_t is_call;
+    int32_t underlying;           /* equity index >= 0, or -(1 + rainbow index) */
+    int32_t maturity_days;        /* days from today */
+    double strike;
+    int32_t is_absolute_strike;   /* 0: strike is a percent of the underlying spot */
+    int32_t barrier_type;         /* THOTH_UP_OUT .. THOTH_DOWN_IN */
+    int32_t discrete_monitoring;  /* 1 discrete_monitoring, 0 continuous_monitoring */
+    int32_t monitoring_period_days;
+    double barrier_up_level;
+    double barrier_down_level;
+} thoth_contract;
+
+typedef struct thoth_mcl_request {
+    thoth_mcl_configuration mcl;
+    int32_t n_assets;              /* equities in name order (the reference's factor order) */
+    const thoth_asset* assets;
+    const double* correlation;     /* n_assets x n_assets spot/spot matrix, row-major */
+    thoth_curve rate;
+    int32_t n_rainbows;
+    const thoth_rainbow* rainbows;
+    int32_t n_contracts;           /* in the book's contract order */
+    const thoth_contract* contracts;
+    int32_t request_greeks;        /* THOTH_GREEK_* bits */
+    /* engine extensions (no reference counterpart) */
+    int32_t precision;             /* THOTH_FP64 (oracle-matched) or THOTH_FP32 (production) */
+    int32_t correlate_stochastic;  /* 1: Cholesky-correlate Heston/LSV spots too (the reference
+                                      leaves them uncorrelated, pricer_mcl.cpp:280-284) */
+    int32_t uniform_steps;         /* > 0: replace the calendar grid by uniform steps t_i = i*T/n */
+} thoth_mcl_request;
+
+typedef struct thoth_mcl_result {
+    double premium, premium_trust;
+    double delta, gamma, vega, rho;
+    /* per-contract arrays of n_contracts entries, caller-owned; any may be NULL */
+    double* contract_premium;
+    double* contract_trust;
+    double* contract_delta;
+    double* contract_gamma;
+    double* contract_vega;
+    double* contract_rho;
+    int32_t n_dates;     /* diffusion dates incl. today */
+    int32_t n_scenarios; /* books swept (base + bumps) */
+    double sweep_ms;     /* device time of the path sweeps */
+    double calib_ms;     /* leverage calibration time */
+} thoth_mcl_result;
+
+/* Price one book.  Returns THOTH_OK or an error status. */
+int thoth_mcl_price(const thoth_mcl_request* req, thoth_mcl_result* res);
+
+/* Split form of thoth_mcl_price for repeated sweeps of one book: prepare
+ * resolves the grid, calibrates, packs the base and bump-scenario books and
+ * uploads them once (device-resident); run sweeps them and fills res (same
+ * fields as thoth_mcl_price); release frees the plan.  A plan is tied to the
+ * calling process and its device. */
+typedef struct thoth_mcl_plan_s* thoth_mcl_plan;
+int thoth_mcl_prepare(const thoth_mcl_request* req, thoth_mcl_plan* plan);
+int thoth_mcl_run(thoth_mcl_plan plan, thoth_mcl_result* res);
+void thoth_mcl_release(thoth_mcl_plan plan);
+
+/* Diffusion grid of a request (pricer_mcl.cpp:71-117): writes up to cap dates
+ * (days from today, year fractions t and step dt) and returns the date count,
+ * or a negative THOTH_* status. */
+int thoth_mcl_grid(const thoth_mcl_request* req, int32_t cap, double* days, double* t, double* dt);
+
+/* FP64 LSV leverage calibration of equity `asset` on the request's grid, with
+ * the equity spot replaced by `spot` (reference: Single::CalibrateLeverage,
+ * marketdata/single.cpp:400-473).  Fills cap_dates layers of 201 points:
+ * ln_step[k], offset[k], levels[k*201 + i].  Returns the layer count or a
+ * negative THOTH_* status. */
+int thoth_mcl_calibrate_leverage(const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_dates,
+                                 double* ln_step, int64_t* offset, double* levels);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,271 @@
+// oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY: C-ABI of the CPU oracle for the
+// Python test-suite (ctypes).  See mcl_oracle.hpp.
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "mcl_oracle.hpp"
+
+using namespace oracle;
+
+namespace
+{
+thread_local std::string g_err;
+int Fail( const std::exception& e )
+{
+    g_err = e.what();
+    return THOTH_ERR_INVALID;
+}
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error( void ) { return g_err.c_str(); }
+
+int oracle_load_joe_kuo( const char* path ) { return LoadJoeKuo( path ) ? 0 : -1; }
+
+int oracle_sobol_max_dimension( void ) { return (int)SobolMaxDimension(); }
+
+// direction numbers V[d][32] of the first dim dimensions
+int oracle_sobol_directions( int dim, uint32_t* out )
+{
+    Sobol s( (unsigned)dim );
+    for ( unsigned d = 0; d < s.Dimension(); d++ )
+        for ( int k = 0; k < 32; k++ )
+            out[d * 32 + k] = s.V()[d][k];
+    return (int)s.Dimension();
+}
+
+// integer coordinates of points skip+1 .. skip+n (sequential Gray-code recurrence)
+int oracle_sobol_int( int dim, int64_t skip, int64_t n, uint32_t* out )
+{
+    Sobol s( (unsigned)dim );
+    s.Skip( (uint64_t)skip );
+    std::vector<uint32_t> x;
+    for ( int64_t i = 0; i < n; i++ )
+    {
+        s.NextInt( x );
+        std::memcpy( out + i * s.Dimension(), x.data(), sizeof( uint32_t ) * s.Dimension() );
+    }
+    return (int)s.Dimension();
+}
+
+void oracle_normal_cdf_inv( int64_t n, const double* p, double* z )
+{
+    for ( int64_t i = 0; i < n; i++ )
+        z[i] = NormalCdfInv( p[i] );
+}
+
+void oracle_xoshiro( uint64_t seed, int64_t n, uint64_t* out )
+{
+    Rng r( seed );
+    for ( int64_t i = 0; i < n; i++ )
+        out[i] = r();
+}
+
+// raw-state xoshiro256++ (published test vectors start from a chosen state)
+void oracle_xoshiro_raw( const uint64_t* state, int64_t n, uint64_t* out )
+{
+    uint64_t s[4] = { state[0], state[1], state[2], state[3] };
+    auto rotl = []( uint64_t x, int k ) { return ( x << k ) | ( x >> ( 64 - k ) ); };
+    for ( int64_t i = 0; i < n; i++ )
+    {
+        // identical update to Rng::operator() (rng.hpp:54-65)
+        const uint64_t result = rotl( s[0] + s[3], 23 ) + s[0];
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl( s[3], 45 );
+        out[i] = result;
+    }
+}
+
+// the 256-bit state splitmix64 expands a seed into (rng.hpp:40-51)
+void oracle_xoshiro_seed_state( uint64_t seed, uint64_t* state )
+{
+    uint64_t z = seed;
+    for ( int i = 0; i < 4; i++ )
+    {
+        z += 0x9e3779b97f4a7c15ULL;
+        uint64_t x = z;
+        x = ( x ^ ( x >> 30 ) ) * 0xbf58476d1ce4e5b9ULL;
+        x = ( x ^ ( x >> 27 ) ) * 0x94d049bb133111ebULL;
+        state[i] = x ^ ( x >> 31 );
+    }
+}
+
+void oracle_gaussians( uint64_t seed, int64_t n, double* out )
+{
+    Rng r( seed );
+    for ( int64_t i = 0; i < n; i++ )
+        out[i] = r.Gaussian();
+}
+
+int oracle_bridge_schedule( int n_times, const double* t, int* bidx, int* lidx, int* ridx, double* lw,
+                            double* rw, double* sd )
+{
+    std::vector<double> tt( t, t + n_times );
+    BridgeSchedule s = BuildBridgeSchedule( tt );
+    for ( int i = 0; i + 1 < n_times; i++ )
+    {
+        bidx[i] = s.bridge_index[i];
+        lidx[i] = s.left_index[i];
+        ridx[i] = s.right_index[i];
+        lw[i] = s.left_weight[i];
+        rw[i] = s.right_weight[i];
+        sd[i] = s.std_dev[i];
+    }
+    return n_times - 1;
+}
+
+// normalized increments of `paths` paths: out[p][f][i], i = 0..n_times-1 (i=0 unused)
+int oracle_path_noise( int n_times, const double* t, int factors, int64_t skip, int64_t paths, double* out )
+{
+    std::vector<double> tt( t, t + n_times );
+    Rng rng( 0 );
+    PathGenerator pg( tt, factors, &rng, (uint64_t)skip );
+    for ( int64_t p = 0; p < paths; p++ )
+    {
+        pg.NextPath();
+        for ( int f = 0; f < factors; f++ )
+            std::memcpy( out + ( p * factors + f ) * n_times, pg.Buffer( f ).data(), sizeof( double ) * n_times );
+    }
+    return 0;
+}
+
+double oracle_sabr_vol( const thoth_sabr_surface* s, double K, double F, double day )
+{
+    Sabr sb;
+    for ( int i = 0; i < s->n; i++ )
+    {
+        sb.maturities.push_back( s->maturities[i] );
+        sb.alpha.push_back( s->alpha[i] );
+        sb.beta.push_back( s->beta[i] );
+        sb.rho.push_back( s->rho[i] );
+        sb.nu.push_back( s->nu[i] );
+    }
+    return sb.ImplicitVol( K, F, day );
+}
+
+double oracle_local_vol( const thoth_mcl_request* req, int asset, double K, double day )
+{
+    Market mk = BuildMarket( *req );
+    return mk.LocalVol( asset, K, day );
+}
+
+int oracle_cholesky( int n, double* a ) { std::vector<double> v( a, a + n * n ); bool ok = CholeskyDecomposeLower( v, n ); std::memcpy( a, v.data(), sizeof( double ) * n * n ); return ok ? 0 : -1; }
+
+// diffusion grid of a request: returns the number of dates (<= cap)
+int oracle_grid( const thoth_mcl_request* req, int cap, double* days, double* t, double* dt )
+{
+    try
+    {
+        Market mk = BuildMarket( *req );
+        Grid g = BuildGrid( *req, mk );
+        int n = (int)g.t.size();
+        for ( int i = 0; i < n && i < cap; i++ )
+        {
+            days[i] = g.days[i];
+            t[i] = g.t[i];
+            dt[i] = g.dt[i];
+        }
+        return n;
+    }
+    catch ( const std::exception& e )
+    {
+        return -Fail( e );
+    }
+}
+
+// leverage layers of an LSV asset on the request's grid (base market + optional spot / vol shift)
+int oracle_calibrate( const thoth_mcl_request* req, int asset, double spot, double vol_shift, double rate_shift, int cap,
+                      double* t, double* ln_step, int64_t* offset, double* levels )
+{
+    try
+    {
+        Market mk = BuildMarket( *req );
+        mk.eq[asset].spot = spot;
+        for ( Equity& e : mk.eq )
+            e.vol_shift = vol_shift;
+        mk.rate.shift = rate_shift;
+        Grid g = BuildGrid( *req, mk );
+        LeverageLayers L = CalibrateLeverage( mk, asset, g.days );
+        int n = (int)L.t.size();
+        for ( int k = 0; k < n && k < cap; k++ )
+        {
+            t[k] = L.t[k];
+            ln_step[k] = L.ln_step[k];
+            offset[k] = L.offset[k];
+            std::memcpy( levels + (size_t)k * L.levels[k].size(), L.levels[k].data(), sizeof( double ) * L.levels[k].size() );
+        }
+        return n;
+    }
+    catch ( const std::exception& e )
+    {
+        return -Fail( e );
+    }
+}
+
+// base-book sums (book + contracts) on the request's own market
+int oracle_sweep( const thoth_mcl_request* req, double* book_sum, double* book_sum2, double* sum, double* sum2,
+                  int64_t* count )
+{
+    try
+    {
+        Market mk = BuildMarket( *req );
+        Grid g = BuildGrid( *req, mk );
+        BookSums s = SweepBook( ResolveBook( *req, mk, g ), req->mcl.paths, req->mcl.sobol_skip, req->mcl.seed );
+        *book_sum = s.book_sum;
+        *book_sum2 = s.book_sum2;
+        for ( size_t c = 0; c < s.sum.size(); c++ )
+        {
+            sum[c] = s.sum[c];
+            sum2[c] = s.sum2[c];
+        }
+        *count = s.count;
+        return 0;
+    }
+    catch ( const std::exception& e )
+    {
+        return Fail( e );
+    }
+}
+
+// seconds of the path sweep alone (grid, calibration and book resolution excluded)
+double oracle_sweep_seconds( const thoth_mcl_request* req, double* book_sum )
+{
+    try
+    {
+        Market mk = BuildMarket( *req );
+        Grid g = BuildGrid( *req, mk );
+        ResolvedBook rb = ResolveBook( *req, mk, g );
+        const auto t0 = std::chrono::steady_clock::now();
+        BookSums s = SweepBook( rb, req->mcl.paths, req->mcl.sobol_skip, req->mcl.seed );
+        const double sec = std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 ).count();
+        if ( book_sum )
+            *book_sum = s.book_sum;
+        return sec;
+    }
+    catch ( const std::exception& e )
+    {
+        Fail( e );
+        return -1.0;
+    }
+}
+
+int oracle_price( const thoth_mcl_request* req, thoth_mcl_result* res )
+{
+    try
+    {
+        return Price( *req, res );
+    }
+    catch ( const std::exception& e )
+    {
+        return Fail( e );
+    }
+}
+}

@@ -1,0 +1,62 @@
+"""The device library loads (no GPU needed) and exports every C-ABI entry point
+declared in include/*.h; without a device the gate reports 'unavailable'
+instead of falling back to a CPU path."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(thoth_[a-z_0-9]+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_headers_declare_the_boundary():
+    names = declared_functions()
+    assert {"thoth_gpu_available", "thoth_gpu_device_info", "thoth_gpu_price_european_gbm",
+            "thoth_mcl_sweep", "thoth_mcl_price", "thoth_last_error", "thoth_gpu_sobol_points"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    from thoth_amd import mcl
+    lib = mcl.lib()
+    exported = subprocess.run(["nm", "-D", "--defined-only", str(mcl.LIB)], capture_output=True, text=True).stdout
+    for name in declared_functions():
+        assert getattr(lib, name) is not None
+        assert re.search(r"\bT %s\b" % name, exported), name
+
+
+def test_structs_match_header_sizes():
+    # compile a probe of the header struct sizes and compare with the ctypes mirror
+    from thoth_amd import abi
+    src = ROOT / "_work" / "abi_probe.c"
+    src.parent.mkdir(exist_ok=True)
+    src.write_text('#include <stdio.h>\n#include "thoth_mcl.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu\\n",'
+                   'sizeof(thoth_mcl_request),sizeof(thoth_mcl_result),sizeof(thoth_asset),sizeof(thoth_contract),'
+                   'sizeof(thoth_mcl_book),sizeof(thoth_mcl_sweep_cfg),sizeof(thoth_mcl_book_out));}\n')
+    exe = src.with_suffix("")
+    subprocess.run(["gcc", "-I%s" % (ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(t) for t in (abi.Request, abi.Result, abi.Asset, abi.Contract, abi.Book, abi.SweepCfg, abi.BookOut)]
+    assert got == want
+
+
+def test_no_device_means_unavailable_not_fallback():
+    from thoth_amd import mcl
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    assert not mcl.available()
+    from thoth_amd import configs
+    from thoth_amd.book import BuiltRequest
+    with pytest.raises(mcl.Unavailable):
+        mcl.price(BuiltRequest(configs.black_scholes_call(paths=1024)))

@@ -1,0 +1,439 @@
+// mcl_books.cpp -- thoth_mcl_sweep: validates thoth_mcl_book inputs, packs the
+// device books (bridge program, FP32 step constants, payoff masks), launches
+// the fused sweep kernel once for the whole batch and folds the per-block
+// FP64 partial sums with Kahan compensation (monte_carlo_node.cpp:73-96).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mcl_device.hpp"
+#include "mcl_internal.hpp"
+#include "mcl_launch.hpp"
+
+namespace thoth
+{
+
+namespace
+{
+
+// host image of the device arena: every array is appended 16-byte aligned and
+// addressed by offset until the single upload.
+struct Arena
+{
+    std::vector<char> data;
+    template <class T>
+    size_t put( const T* p, size_t n )
+    {
+        size_t off = ( data.size() + 15 ) & ~size_t( 15 );
+        data.resize( off + n * sizeof( T ) );
+        if ( n )
+            std::memcpy( data.data() + off, p, n * sizeof( T ) );
+        return off;
+    }
+    template <class T>
+    size_t put( const std::vector<T>& v )
+    {
+        return put( v.data(), v.size() );
+    }
+};
+
+struct Offsets
+{
+    size_t t, dt, sqrt_dt, bsq, binv, step_ops, op_b, op_left, op_w, op_wf, drift, sc, lev, levf, lnstep, offset,
+        invstep, mon, flow;
+};
+
+std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
+
+int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena& ar, int& need_depth )
+{
+    const int nd = in.n_dates;
+    const int na = in.n_assets;
+    if ( nd < 2 || !in.t || !in.dt )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "need at least 2 dates and t/dt arrays" ) );
+    if ( na < 1 || na > kMaxAssets )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "1..8 equities supported" ) );
+    if ( in.n_contracts < 1 || in.n_contracts > kMaxContracts )
+        return SetError( THOTH_ERR_UNSUPPORTED, Why( bi, "1..32 contracts per book" ) );
+    if ( in.n_rainbows < 0 || in.n_rainbows > kMaxRainbows )
+        return SetError( THOTH_ERR_UNSUPPORTED, Why( bi, "at most 4 rainbows per book" ) );
+    if ( in.t[0] != 0.0 )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "t[0] must be 0 (today)" ) );
+    for ( int i = 1; i < nd; i++ )
+        if ( !( in.t[i] >= in.t[i - 1] ) || !( in.dt[i] >= 0 ) )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "dates must be non-decreasing" ) );
+    if ( !( in.t[nd - 1] > 0 ) )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "last date must be after today" ) );
+    std::memset( &db, 0, sizeof( db ) );
+    db.n_dates = nd;
+    db.m = nd - 1;
+    db.n_assets = na;
+    db.n_contracts = in.n_contracts;
+    db.n_rainbows = in.n_rainbows;
+    db.corr_stoch = in.correlate_stochastic ? 1 : 0;
+    // factors: spot noises in equity order, then the variance noises of the
+    // stochastic equities (pricer_mcl.cpp:404-432)
+    int f = na;
+    bool any_lsv = false;
+    for ( int a = 0; a < na; a++ )
+    {
+        DevAsset& A = db.asset[a];
+        A.model = in.model[a];
+        if ( A.model < THOTH_MODEL_BS || A.model > THOTH_MODEL_LSV )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "unknown model" ) );
+        A.spot = in.spot[a];
+        if ( !( A.spot > 0 ) )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "spot must be positive" ) );
+        A.vol = in.vol ? in.vol[a] : 0.0;
+        A.v0 = in.v0 ? in.v0[a] : 0.0;
+        A.kappa = in.kappa ? in.kappa[a] : 0.0;
+        A.theta = in.theta ? in.theta[a] : 0.0;
+        A.xi = in.xi ? in.xi[a] : 0.0;
+        A.rho = in.rho ? in.rho[a] : 0.0;
+        A.var_factor = -1;
+        if ( A.model != THOTH_MODEL_BS )
+        {
+            if ( !( A.kappa > 0 ) || !( A.v0 >= 0 ) || !( A.theta >= 0 ) || !( A.xi >= 0 ) )
+                return SetError( THOTH_ERR_INVALID, Why( bi, "Heston parameters out of range" ) );
+            A.var_factor = f++;
+        }
+        any_lsv |= A.model == THOTH_MODEL_LSV;
+    }
+    db.factors = f;
+    if ( (long)( nd - 1 ) * f > kSobolDims )
+        return SetError( THOTH_ERR_UNSUPPORTED,
+                         Why( bi, "steps x factors exceeds the 4097 Sobol dimensions (the reference draws the "
+                                  "tail pseudo-randomly)" ) );
+    if ( na > 1 && !in.chol )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "multi-equity book needs a Cholesky factor" ) );
+    for ( int a = 0; a < na; a++ )
+        for ( int v = 0; v <= a; v++ )
+        {
+            db.chol[a * kMaxAssets + v] = na > 1 ? in.chol[a * na + v] : 1.0;
+            db.cholf[a * kMaxAssets + v] = (float)db.chol[a * kMaxAssets + v];
+        }
+    // grid derived arrays
+    std::vector<double> sqrt_dt( nd, 0.0 ), bsq( nd, 0.0 );
+    std::vector<float> binv( nd, 0.0f );
+    for ( int i = 1; i < nd; i++ )
+    {
+        sqrt_dt[i] = std::sqrt( in.dt[i] );
+        const double dtb = in.t[i] - in.t[i - 1];
+        bsq[i] = dtb > 0 ? std::sqrt( dtb ) : 0.0;
+        binv[i] = dtb > 0 ? (float)( 1.0 / bsq[i] ) : 0.0f;
+    }
+    BridgeProgram bp = BuildBridgeProgram( in.t, nd );
+    need_depth = std::max( need_depth, std::max( 1, bp.max_depth - 1 ) );
+    if ( bp.max_depth - 1 > kMaxStackDepth )
+        return SetError( THOTH_ERR_UNSUPPORTED, Why( bi, "too many steps for the bridge stack" ) );
+    db.sd0 = bp.sd0;
+    db.sd0f = (float)bp.sd0;
+    db.n_ops = (int)bp.op_b.size();
+    std::vector<float> op_wf( bp.op_w.begin(), bp.op_w.end() );
+    of.t = ar.put( in.t, nd );
+    of.dt = ar.put( in.dt, nd );
+    of.sqrt_dt = ar.put( sqrt_dt );
+    of.bsq = ar.put( bsq );
+    of.binv = ar.put( binv );
+    of.step_ops = ar.put( bp.step_ops );
+    of.op_b = ar.put( bp.op_b );
+    of.op_left = ar.put( bp.op_left );
+    of.op_w = ar.put( bp.op_w );
+    of.op_wf = ar.put( op_wf );
+    // drift and FP32 step constants
+    if ( !in.drift )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "drift array required" ) );
+    of.drift = ar.put( in.drift, (size_t)na * nd );
+    std::vector<float> sc( (size_t)na * nd * SC_COUNT, 0.0f );
+    for ( int a = 0; a < na; a++ )
+    {
+        const DevAsset& A = db.asset[a];
+        for ( int i = 1; i < nd; i++ )
+        {
+            float* c = &sc[( (size_t)a * nd + i ) * SC_COUNT];
+            const double dt = in.dt[i];
+            c[SC_CARRY] = (float)( in.drift[(size_t)a * nd + i] * in.t[i] - in.drift[(size_t)a * nd + i - 1] * in.t[i - 1] );
+            c[SC_DT] = (float)dt;
+            c[SC_SQDT] = (float)std::sqrt( dt );
+            c[SC_VOL] = (float)A.vol;
+            c[SC_HALF_V2DT] = (float)( A.vol * A.vol / 2 * dt );
+            if ( A.model != THOTH_MODEL_BS )
+            {
+                const double e = std::exp( -A.kappa * dt );
+                c[SC_E] = (float)e;
+                c[SC_M0] = (float)( A.theta * ( 1 - e ) );
+                c[SC_S1] = (float)( A.xi * A.xi * e / A.kappa * ( 1 - e ) );
+                c[SC_S0] = (float)( A.theta * A.xi * A.xi / ( 2 * A.kappa ) * ( 1 - e ) * ( 1 - e ) );
+                if ( A.xi > 1e-12 )
+                {
+                    c[SC_K0] = (float)( -A.rho * A.kappa * A.theta / A.xi * dt );
+                    c[SC_A1] = (float)( 0.5 * dt * A.kappa * A.rho / A.xi );
+                    c[SC_A2] = (float)( 0.5 * dt * 0.5 );
+                    c[SC_RX] = (float)( A.rho / A.xi );
+                    c[SC_K3] = (float)( 0.5 * dt * ( 1.0 - A.rho * A.rho ) );
+                }
+            }
+        }
+    }
+    of.sc = ar.put( sc );
+    // leverage layers
+    db.lev_points = 0;
+    of.lev = of.levf = of.lnstep = of.offset = of.invstep = 0;
+    if ( any_lsv )
+    {
+        const int P = in.lev_points;
+        if ( P < 2 || !in.lev_values || !in.lev_ln_step || !in.lev_offset )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "LSV equity needs leverage layers" ) );
+        db.lev_points = P;
+        const size_t nl = (size_t)na * nd;
+        std::vector<float> levf( in.lev_values, in.lev_values + nl * P );
+        std::vector<double> off( nl );
+        std::vector<float> inv( nl );
+        for ( size_t k = 0; k < nl; k++ )
+        {
+            off[k] = (double)in.lev_offset[k];
+            inv[k] = in.lev_ln_step[k] > 0 ? (float)( 1.0 / in.lev_ln_step[k] ) : 0.0f;
+        }
+        of.lev = ar.put( in.lev_values, nl * P );
+        of.levf = ar.put( levf );
+        of.lnstep = ar.put( in.lev_ln_step, nl );
+        of.offset = ar.put( off );
+        of.invstep = ar.put( inv );
+    }
+    // rainbows (rainbow_node.cpp:14-27)
+    for ( int r = 0; r < in.n_rainbows; r++ )
+    {
+        db.rb_best[r] = in.rb_best[r];
+        db.rb_n[r] = in.rb_n[r];
+        if ( db.rb_n[r] < 1 || db.rb_n[r] > kMaxAssets )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "rainbow size" ) );
+        for ( int j = 0; j < db.rb_n[r]; j++ )
+        {
+            db.rb_member[r][j] = in.rb_member[r * THOTH_MCL_MAX_ASSETS + j];
+            db.rb_weight[r][j] = in.rb_weight[r * THOTH_MCL_MAX_ASSETS + j];
+            if ( db.rb_member[r][j] < 0 || db.rb_member[r][j] >= na )
+                return SetError( THOTH_ERR_INVALID, Why( bi, "rainbow member index" ) );
+        }
+    }
+    // contracts: flows and monitoring masks per date
+    std::vector<uint32_t> mon( nd, 0u ), flow( nd, 0u );
+    for ( int c = 0; c < in.n_contracts; c++ )
+    {
+        db.c_kind[c] = in.c_kind[c];
+        db.c_call[c] = in.c_is_call[c] ? 1 : 0;
+        db.c_udl[c] = in.c_underlying[c];
+        db.c_strike[c] = in.c_strike[c];
+        const int u = db.c_udl[c];
+        if ( u >= na || u < -in.n_rainbows )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "contract underlying index" ) );
+        const int fi = in.c_flow_index[c];
+        if ( fi < 0 || fi >= nd )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "contract flow index" ) );
+        flow[fi] |= 1u << c;
+        // contract_node.cpp:33-47: discount the flow date back to today
+        const double dfac = fi != 0 ? std::exp( in.t[0] * in.disc_rate[0] - in.t[fi] * in.disc_rate[fi] ) : 1.0;
+        db.c_df[c] = dfac;
+        if ( db.c_kind[c] == THOTH_PAYOFF_BARRIER )
+        {
+            db.c_up[c] = in.c_is_up[c] ? 1 : 0;
+            db.c_in[c] = in.c_is_in[c] ? 1 : 0;
+            db.c_level[c] = in.c_level[c];
+            for ( int k = in.c_mon_offset[c]; k < in.c_mon_offset[c + 1]; k++ )
+            {
+                const int mi = in.c_mon_index[k];
+                if ( mi < 0 || mi > fi )
+                    return SetError( THOTH_ERR_INVALID, Why( bi, "monitoring date after the flow" ) );
+                mon[mi] |= 1u << c;
+            }
+        }
+        else if ( db.c_kind[c] != THOTH_PAYOFF_VANILLA )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "unknown payoff kind" ) );
+    }
+    of.mon = ar.put( mon );
+    of.flow = ar.put( flow );
+    return THOTH_OK;
+}
+
+template <class T>
+T* At( char* base, size_t off )
+{
+    return reinterpret_cast<T*>( base + off );
+}
+
+}  // namespace
+
+DeviceBatch::~DeviceBatch()
+{
+    if ( dbase )
+        (void)hipFree( dbase );
+}
+
+int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B )
+{
+    int rc = EnsureDevice();
+    if ( rc != THOTH_OK )
+        return rc;
+    if ( n_books < 1 || !books || !cfg )
+        return SetError( THOTH_ERR_INVALID, "null argument" );
+    if ( cfg->paths < 1 || cfg->sobol_skip < 0 )
+        return SetError( THOTH_ERR_INVALID, "paths must be >= 1 and sobol_skip >= 0" );
+    if ( cfg->precision != 0 && cfg->precision != 1 )
+        return SetError( THOTH_ERR_INVALID, "precision must be 0 (FP64) or 1 (FP32)" );
+    const uint64_t first = (uint64_t)cfg->sobol_skip + 1;
+    const uint64_t last = (uint64_t)cfg->sobol_skip + (uint64_t)cfg->paths;
+    if ( last >= 0xffffffffull )
+        return SetError( THOTH_ERR_UNSUPPORTED, "Sobol index beyond 2^32 - 2" );
+    B.first = (uint32_t)first;
+    B.last = (uint32_t)last;
+    B.q0 = (uint32_t)( first / 64 );
+    B.nwaves = (uint32_t)( last / 64 ) - B.q0 + 1;
+    B.nblocks = ( B.nwaves + kWavesPerBlock - 1 ) / kWavesPerBlock;
+    B.paths = cfg->paths;
+    B.precision = cfg->precision;
+    B.n_books = n_books;
+
+    Arena ar;
+    B.db.assign( n_books, DevBook{} );
+    std::vector<Offsets> of( n_books );
+    B.depth = 1;
+    B.na_max = 1;
+    for ( int b = 0; b < n_books; b++ )
+    {
+        rc = PackBook( b, books[b], B.db[b], of[b], ar, B.depth );
+        if ( rc != THOTH_OK )
+            return rc;
+        B.na_max = std::max( B.na_max, B.db[b].n_assets );
+    }
+    B.part_off.assign( n_books, 0 );
+    B.part_total = 0;
+    for ( int b = 0; b < n_books; b++ )
+    {
+        B.part_off[b] = B.part_total;
+        B.part_total += (size_t)B.nblocks * ( 1 + B.db[b].n_contracts ) * 2;
+    }
+    const size_t arena_bytes = ( ar.data.size() + 255 ) & ~size_t( 255 );
+    const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
+    const size_t total = arena_bytes + books_bytes + B.part_total * sizeof( double );
+    hipError_t e = hipMalloc( &B.dbase, total );
+    if ( e != hipSuccess )
+        return SetError( THOTH_ERR_DEVICE, std::string( "hipMalloc: " ) + hipGetErrorString( e ) );
+    char* dbase = B.dbase;
+    B.dpart = reinterpret_cast<double*>( dbase + arena_bytes + books_bytes );
+    for ( int b = 0; b < n_books; b++ )
+    {
+        DevBook& d = B.db[b];
+        const Offsets& o = of[b];
+        d.t = At<double>( dbase, o.t );
+        d.dt = At<double>( dbase, o.dt );
+        d.sqrt_dt = At<double>( dbase, o.sqrt_dt );
+        d.bsq = At<double>( dbase, o.bsq );
+        d.binv = At<float>( dbase, o.binv );
+        d.step_ops = At<int>( dbase, o.step_ops );
+        d.op_b = At<int>( dbase, o.op_b );
+        d.op_left = At<int>( dbase, o.op_left );
+        d.op_w = At<double>( dbase, o.op_w );
+        d.op_wf = At<float>( dbase, o.op_wf );
+        d.drift = At<double>( dbase, o.drift );
+        d.sc = At<float>( dbase, o.sc );
+        if ( d.lev_points )
+        {
+            d.lev = At<double>( dbase, o.lev );
+            d.levf = At<float>( dbase, o.levf );
+            d.lev_ln_step = At<double>( dbase, o.lnstep );
+            d.lev_offset = At<double>( dbase, o.offset );
+            d.lev_inv_step = At<float>( dbase, o.invstep );
+        }
+        d.mon_mask = At<uint32_t>( dbase, o.mon );
+        d.flow_mask = At<uint32_t>( dbase, o.flow );
+        d.partial = B.dpart + B.part_off[b];
+    }
+    B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
+    hipStream_t s = Stream();
+#define HIP_OK( expr )                                                                                      \
+    do                                                                                                      \
+    {                                                                                                       \
+        hipError_t e_ = ( expr );                                                                           \
+        if ( e_ != hipSuccess )                                                                             \
+            return SetError( THOTH_ERR_DEVICE, std::string( #expr " failed: " ) + hipGetErrorString( e_ ) ); \
+    } while ( 0 )
+    HIP_OK( hipMemcpyAsync( dbase, ar.data.data(), ar.data.size(), hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipMemcpyAsync( dbase + arena_bytes, B.db.data(), n_books * sizeof( DevBook ), hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipStreamSynchronize( s ) );
+    return THOTH_OK;
+}
+
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
+{
+    hipStream_t s = Stream();
+    hipEvent_t e0, e1;
+    HIP_OK( hipEventCreate( &e0 ) );
+    HIP_OK( hipEventCreate( &e1 ) );
+    HIP_OK( hipEventRecord( e0, s ) );
+    const dim3 grid( B.nblocks, B.n_books );
+    bool ok = B.precision == 0
+                  ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.dbooks, DeviceDirections(), B.q0, B.nwaves, B.first,
+                                      B.last )
+                  : launch_sweep_f32( B.na_max, B.depth, grid, s, B.dbooks, DeviceDirections(), B.q0, B.nwaves, B.first,
+                                      B.last );
+    if ( !ok )
+    {
+        (void)hipEventDestroy( e0 );
+        (void)hipEventDestroy( e1 );
+        return SetError( THOTH_ERR_UNSUPPORTED, "no kernel instantiation for this book shape" );
+    }
+    HIP_OK( hipGetLastError() );
+    HIP_OK( hipEventRecord( e1, s ) );
+    std::vector<double> part( B.part_total );
+    HIP_OK( hipMemcpyAsync( part.data(), B.dpart, B.part_total * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+    HIP_OK( hipStreamSynchronize( s ) );
+    float ms = 0;
+    HIP_OK( hipEventElapsedTime( &ms, e0, e1 ) );
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    for ( int b = 0; b < B.n_books; b++ )
+    {
+        const int nc = B.db[b].n_contracts;
+        const int ne = 1 + nc;
+        std::vector<double> sum( ne * 2, 0.0 ), comp( ne * 2, 0.0 );
+        const double* pb = part.data() + B.part_off[b];
+        for ( uint32_t k = 0; k < B.nblocks; k++ )
+            for ( int j = 0; j < ne * 2; j++ )
+            {
+                // Kahan over blocks in block order (deterministic)
+                const double y = pb[(size_t)k * ne * 2 + j] - comp[j];
+                const double t = sum[j] + y;
+                comp[j] = ( t - sum[j] ) - y;
+                sum[j] = t;
+            }
+        thoth_mcl_book_out& o = out[b];
+        std::memset( &o, 0, sizeof( o ) );
+        o.book_sum = sum[0];
+        o.book_sum2 = sum[1];
+        for ( int c = 0; c < nc; c++ )
+        {
+            o.contract_sum[c] = sum[2 * ( 1 + c )];
+            o.contract_sum2[c] = sum[2 * ( 1 + c ) + 1];
+        }
+        o.count = B.paths;
+        o.device_ms = ms;
+    }
+    return THOTH_OK;
+}
+
+int SweepBooks( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg,
+                thoth_mcl_book_out* out )
+{
+    if ( !out )
+        return SetError( THOTH_ERR_INVALID, "null argument" );
+    DeviceBatch B;
+    int rc = PrepareBatch( n_books, books, cfg, B );
+    if ( rc != THOTH_OK )
+        return rc;
+    return RunBatch( B, out );
+}
+
+}  // namespace thoth

@@ -1,0 +1,84 @@
+// mcl_device.hpp -- device-side book layout shared by the host launcher
+// (mcl_gpu.cpp) and the path-sweep kernels (mcl_sweep.hpp).
+#pragma once
+#include <cstdint>
+
+namespace thoth
+{
+
+constexpr int kMaxAssets = 8;
+constexpr int kMaxContracts = 32;
+constexpr int kMaxRainbows = 4;
+constexpr int kWavesPerBlock = 4;          // 256-thread blocks
+constexpr int kBlock = 64 * kWavesPerBlock;
+
+// FP32 per-(equity, date) step constants, precomputed on the host in FP64
+// from the reference's QE / Andersen formulas (heston_variance_node.cpp:29-33,
+// heston_spot_node.cpp:60-75).
+enum StepConst
+{
+    SC_CARRY = 0,  // carry drift D_i t_i - D_{i-1} t_{i-1}
+    SC_E,          // exp(-kappa dt)
+    SC_M0,         // theta (1 - e)               : m  = SC_M0 + v SC_E
+    SC_S1,         // xi^2 e (1 - e) / kappa       : s2 = SC_S0 + v SC_S1
+    SC_S0,         // theta xi^2 (1 - e)^2 / (2 kappa)
+    SC_K0,         // -rho kappa theta / xi dt     : K0 = L SC_K0
+    SC_A1,         // g dt kappa rho / xi          : a  = L SC_A1 - L^2 SC_A2
+    SC_A2,         // g dt / 2
+    SC_RX,         // rho / xi                     : K1,2 = a -+ L SC_RX
+    SC_K3,         // g dt (1 - rho^2)             : K3 = L^2 SC_K3
+    SC_DT,         // dt
+    SC_SQDT,       // sqrt(dt)
+    SC_HALF_V2DT,  // BS: v^2 / 2 dt
+    SC_VOL,        // BS: v
+    SC_COUNT = 16
+};
+
+struct DevAsset
+{
+    int model;       // THOTH_MODEL_*
+    int var_factor;  // factor of the variance noise (-1 for BS)
+    double spot, vol, v0, kappa, theta, xi, rho;
+};
+
+struct DevBook
+{
+    // grid
+    int n_dates, m, n_assets, factors;
+    int n_contracts, n_rainbows, corr_stoch, lev_points;
+    int n_ops;
+    double sd0;           // bridge endpoint std dev sqrt(t_m)
+    float sd0f;
+    const double* t;      // [n_dates]
+    const double* dt;     // [n_dates] YearFraction(d_{i-1}, d_i)
+    const double* sqrt_dt;
+    const double* bsq;    // [n_dates] sqrt(t_i - t_{i-1}) (0 when the step is empty)
+    const float* binv;    // [n_dates] 1 / bsq (0 when empty)
+    const int* step_ops;  // [m + 1] prefix offsets into the op arrays
+    const int* op_b;      // [n_ops] bridge build index of the op (Sobol dim = b * factors + f)
+    const int* op_left;   // [n_ops] has a left anchor
+    const double* op_w;   // [n_ops * 3] left weight, right weight, std dev
+    const float* op_wf;   // FP32 copy
+    // equities
+    DevAsset asset[kMaxAssets];
+    double chol[kMaxAssets * kMaxAssets];
+    float cholf[kMaxAssets * kMaxAssets];
+    const double* drift;  // [n_assets * n_dates] drift-node value
+    const float* sc;      // [n_assets * n_dates * SC_COUNT]
+    const double* lev;    // [n_assets * n_dates * lev_points]
+    const float* levf;
+    const double* lev_ln_step;  // [n_assets * n_dates]
+    const double* lev_offset;   // [n_assets * n_dates] (integral values)
+    const float* lev_inv_step;  // [n_assets * n_dates] 1 / ln_step
+    // payoffs
+    int c_kind[kMaxContracts], c_call[kMaxContracts], c_udl[kMaxContracts];
+    int c_up[kMaxContracts], c_in[kMaxContracts];
+    double c_strike[kMaxContracts], c_level[kMaxContracts], c_df[kMaxContracts];
+    const uint32_t* mon_mask;   // [n_dates] contracts monitored at the date
+    const uint32_t* flow_mask;  // [n_dates] contracts paying at the date
+    int rb_best[kMaxRainbows], rb_n[kMaxRainbows], rb_member[kMaxRainbows][kMaxAssets];
+    double rb_weight[kMaxRainbows][kMaxAssets];
+    double* partial;            // [n_blocks][1 + n_contracts][2] per-block sums
+};
+
+}  // namespace thoth

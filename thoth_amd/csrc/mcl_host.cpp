@@ -1,0 +1,651 @@
+// mcl_host.cpp -- C++23 host engine behind thoth_mcl_price (include/thoth_mcl.h).
+//
+// It plays the role of PricerMCL::PriceBook / ComputeGreeks for GPU-eligible
+// books: builds the diffusion grid (pricer_mcl.cpp:71-117), resolves the market
+// into per-date drift / discount arrays, the Cholesky factor (maths.cpp:354-381),
+// the LSV leverage layers (FP64, single.cpp:400-473), the payoffs (vanilla.cpp,
+// barrier.cpp:122-173, rainbow.cpp:46-61), then sweeps the base book and every
+// bump scenario of the reference's Greeks (pricer.cpp:273-391) in ONE device
+// launch on common Sobol points, and finite-differences the results exactly as
+// the reference does.
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <cmath>
+#include <cstring>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mcl_hostlib.hpp"
+#include "mcl_internal.hpp"
+#include "thoth_mcl.h"
+
+namespace thoth::host
+{
+
+struct Invalid : std::runtime_error
+{
+    int code;
+    Invalid( int c, const std::string& s ) : std::runtime_error( s ), code( c ) {}
+};
+
+namespace
+{
+constexpr double kSpotBump = 0.01;    // helpers/constants.hpp:36
+constexpr double kGammaBump = 0.10;   // tasks/pricer.hpp:20
+constexpr double kVolBump = 0.01;     // tasks/pricer.hpp:25
+constexpr double kRateBump = 0.0001;  // tasks/pricer.hpp:26
+
+
+// barrier.cpp:56-69
+std::set<double> MonitoringDates( const thoth_contract& c )
+{
+    std::set<double> s;
+    if ( c.monitoring_period_days > 0 )
+        for ( double d = c.monitoring_period_days; d < c.maturity_days; d += c.monitoring_period_days )
+            s.insert( d );
+    s.insert( c.maturity_days );
+    return s;
+}
+
+// the arrays one thoth_mcl_book points to
+struct BookArrays
+{
+    std::vector<int32_t> model;
+    std::vector<double> spot, vol, v0, kappa, theta, xi, rho, drift, chol, disc;
+    std::vector<double> lev_ln_step, lev_values;
+    std::vector<int64_t> lev_offset;
+    std::vector<int32_t> rb_best, rb_n, rb_member;
+    std::vector<double> rb_weight;
+    std::vector<int32_t> c_kind, c_call, c_udl, c_flow, c_up, c_in, c_mon_off, c_mon;
+    std::vector<double> c_strike, c_level;
+    thoth_mcl_book book{};
+};
+
+}  // namespace
+
+// pricer_mcl.cpp:71-117 (+ vanilla/barrier GetFixingDates); uniform-steps extension
+Grid BuildGrid( const thoth_mcl_request& req, const Market& mk )
+{
+    Grid g;
+    std::set<double> fixing;
+    for ( int c = 0; c < req.n_contracts; c++ )
+    {
+        const thoth_contract& k = req.contracts[c];
+        if ( k.maturity_days <= 0 )
+            throw Invalid( THOTH_ERR_INVALID, "contract maturity must be after today" );
+        fixing.insert( k.maturity_days );
+        if ( k.kind == THOTH_BARRIER && k.discrete_monitoring )
+        {
+            const std::set<double> m = MonitoringDates( k );
+            fixing.insert( m.begin(), m.end() );
+        }
+    }
+    const double last = *fixing.rbegin();
+    if ( req.uniform_steps > 0 )
+    {
+        if ( fixing.size() != 1 && !( req.n_contracts > 0 ) )
+            throw Invalid( THOTH_ERR_INVALID, "uniform grid needs a common maturity" );
+        for ( int c = 0; c < req.n_contracts; c++ )
+            if ( req.contracts[c].maturity_days != last )
+                throw Invalid( THOTH_ERR_UNSUPPORTED, "uniform grid needs every contract at the last date" );
+        g.uniform = true;
+        const double T = last / kDaysPerYear;
+        for ( int i = 0; i <= req.uniform_steps; i++ )
+        {
+            const double ti = ( i == req.uniform_steps ) ? T : T * i / req.uniform_steps;
+            g.t.push_back( ti );
+            g.days.push_back( i == req.uniform_steps ? last : ti * kDaysPerYear );
+            g.dt.push_back( i == 0 ? 0.0 : ti - g.t[i - 1] );
+        }
+        return g;
+    }
+    if ( req.mcl.max_day_step <= 0 )
+        throw Invalid( THOTH_ERR_INVALID, "max_day_step must be > 0" );
+    long step = req.mcl.max_day_step;
+    if ( req.mcl.vol_year_step > 0 )
+    {
+        bool stoch = false;
+        for ( const Equity& e : mk.eq )
+            stoch |= e.kind != THOTH_VOL_BS;
+        if ( stoch )
+            step = std::min( step, std::max<long>( 1, (long)std::floor( req.mcl.vol_year_step * kDaysPerYear ) ) );
+    }
+    std::set<double> dates;
+    for ( double d = 0; d < last; d += step )
+        dates.insert( d );
+    dates.insert( fixing.begin(), fixing.end() );
+    g.days.assign( dates.begin(), dates.end() );
+    for ( size_t i = 0; i < g.days.size(); i++ )
+    {
+        g.t.push_back( YearFraction( g.days[0], g.days[i] ) );
+        g.dt.push_back( i > 0 ? YearFraction( g.days[i - 1], g.days[i] ) : 0.0 );
+    }
+    return g;
+}
+
+namespace
+{
+
+void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, const std::vector<double>& chol,
+              const std::vector<LeverageGrid>& lev, BookArrays& A )
+{
+    const int n = req.n_assets;
+    const int nd = (int)g.t.size();
+    for ( int a = 0; a < n; a++ )
+    {
+        const Equity& e = mk.eq[a];
+        A.model.push_back( e.kind == THOTH_VOL_BS ? THOTH_MODEL_BS
+                                                  : e.kind == THOTH_VOL_HESTON ? THOTH_MODEL_HESTON : THOTH_MODEL_LSV );
+        A.spot.push_back( e.spot - 0.0 );
+        A.vol.push_back( ( e.bs_vol + e.vol_shift ) * 1.0 );  // bs_volatility.cpp:46-47
+        const StochParams h = e.kind == THOTH_VOL_BS ? StochParams{} : mk.Stoch( a );
+        A.v0.push_back( h.v0 );
+        A.kappa.push_back( h.kappa );
+        A.theta.push_back( h.theta );
+        A.xi.push_back( h.xi );
+        A.rho.push_back( h.rho );
+        for ( int i = 0; i < nd; i++ )
+            A.drift.push_back( mk.Drift( a, g.days[i] ) );
+    }
+    A.chol = chol;
+    for ( int i = 0; i < nd; i++ )
+        A.disc.push_back( mk.rate.Value( g.days[i] ) );
+    bool any_lsv = false;
+    for ( int a = 0; a < n; a++ )
+        any_lsv |= mk.eq[a].kind == THOTH_VOL_LSV;
+    int P = 0;
+    if ( any_lsv )
+    {
+        for ( const LeverageGrid& L : lev )
+            P = std::max( P, L.points );
+        A.lev_ln_step.assign( (size_t)n * nd, 1.0 );
+        A.lev_offset.assign( (size_t)n * nd, 0 );
+        A.lev_values.assign( (size_t)n * nd * P, 1.0 );
+        for ( int a = 0; a < n; a++ )
+        {
+            if ( mk.eq[a].kind != THOTH_VOL_LSV )
+                continue;
+            const LeverageGrid& L = lev[a];
+            for ( int i = 0; i < nd; i++ )
+            {
+                A.lev_ln_step[(size_t)a * nd + i] = L.ln_step[i];
+                A.lev_offset[(size_t)a * nd + i] = L.offset[i];
+                std::copy( L.values.begin() + (size_t)i * L.points, L.values.begin() + (size_t)( i + 1 ) * L.points,
+                           A.lev_values.begin() + ( (size_t)a * nd + i ) * P );
+            }
+        }
+    }
+    for ( int r = 0; r < req.n_rainbows; r++ )
+    {
+        const thoth_rainbow& rb = req.rainbows[r];
+        if ( rb.n < 1 || rb.n > THOTH_MCL_MAX_ASSETS )
+            throw Invalid( THOTH_ERR_INVALID, "rainbow size" );
+        A.rb_best.push_back( rb.best ? 1 : 0 );
+        A.rb_n.push_back( rb.n );
+        for ( int j = 0; j < THOTH_MCL_MAX_ASSETS; j++ )
+        {
+            const int m = j < rb.n ? rb.members[j] : 0;
+            if ( m < 0 || m >= n )
+                throw Invalid( THOTH_ERR_INVALID, "rainbow member index" );
+            A.rb_member.push_back( m );
+            // rainbow.cpp:56-57: 100 / reference spot captured at load (the unbumped spot)
+            A.rb_weight.push_back( j < rb.n ? 100.0 / req.assets[m].spot : 0.0 );
+        }
+    }
+    auto index_of = [&]( double day )
+    {
+        for ( int i = 0; i < nd; i++ )
+            if ( g.days[i] == day )
+                return i;
+        throw Invalid( THOTH_ERR_INVALID, "date not on the diffusion grid" );
+    };
+    A.c_mon_off.push_back( 0 );
+    for ( int c = 0; c < req.n_contracts; c++ )
+    {
+        const thoth_contract& k = req.contracts[c];
+        if ( k.underlying >= n || k.underlying < -req.n_rainbows )
+            throw Invalid( THOTH_ERR_INVALID, "contract underlying index" );
+        const int fi = index_of( k.maturity_days );
+        // vanilla.cpp:29-34 / barrier.cpp:36-41: a relative strike resolves against the base spot
+        const double base_spot = k.underlying >= 0 ? req.assets[k.underlying].spot : 100.0;
+        const double K = k.is_absolute_strike ? k.strike : k.strike / 100 * base_spot;
+        A.c_kind.push_back( k.kind == THOTH_BARRIER ? THOTH_PAYOFF_BARRIER : THOTH_PAYOFF_VANILLA );
+        A.c_call.push_back( k.is_call ? 1 : 0 );
+        A.c_udl.push_back( k.underlying );
+        A.c_flow.push_back( fi );
+        A.c_strike.push_back( K );
+        double H = 0;
+        bool up = false, in = false;
+        if ( k.kind == THOTH_BARRIER )
+        {
+            up = k.barrier_type == THOTH_UP_OUT || k.barrier_type == THOTH_UP_IN;
+            in = k.barrier_type == THOTH_UP_IN || k.barrier_type == THOTH_DOWN_IN;
+            H = up ? k.barrier_up_level : k.barrier_down_level;
+            if ( k.discrete_monitoring && !g.uniform )
+            {
+                for ( double d : MonitoringDates( k ) )
+                    A.c_mon.push_back( index_of( d ) );
+            }
+            else if ( k.discrete_monitoring )
+            {
+                for ( int i = 1; i <= fi; i++ )
+                    A.c_mon.push_back( i );
+            }
+            else
+            {
+                // barrier.cpp:146-161: every diffusion date up to maturity + BGK level shift
+                size_t cnt = 0;
+                for ( int i = 0; i < nd; i++ )
+                    if ( g.days[i] <= k.maturity_days )
+                    {
+                        A.c_mon.push_back( i );
+                        cnt++;
+                    }
+                if ( k.underlying < 0 )
+                    throw Invalid( THOTH_ERR_UNSUPPORTED, "continuous barrier on a rainbow has no implied vol" );
+                const double t = YearFraction( 0, k.maturity_days );
+                const size_t steps = cnt > 1 ? cnt - 1 : 1;
+                const double dt = t / (double)steps;
+                const double vol = mk.ImplicitVol( k.underlying, K, k.maturity_days );
+                const double beta = 0.5826;
+                H *= std::exp( ( up ? -1.0 : 1.0 ) * beta * vol * std::sqrt( dt ) );
+            }
+        }
+        A.c_up.push_back( up ? 1 : 0 );
+        A.c_in.push_back( in ? 1 : 0 );
+        A.c_level.push_back( H );
+        A.c_mon_off.push_back( (int32_t)A.c_mon.size() );
+    }
+    thoth_mcl_book& b = A.book;
+    b.n_dates = nd;
+    b.t = g.t.data();
+    b.dt = g.dt.data();
+    b.n_assets = n;
+    b.model = A.model.data();
+    b.spot = A.spot.data();
+    b.vol = A.vol.data();
+    b.v0 = A.v0.data();
+    b.kappa = A.kappa.data();
+    b.theta = A.theta.data();
+    b.xi = A.xi.data();
+    b.rho = A.rho.data();
+    b.drift = A.drift.data();
+    b.chol = A.chol.empty() ? nullptr : A.chol.data();
+    b.correlate_stochastic = req.correlate_stochastic;
+    b.lev_points = P;
+    b.lev_ln_step = A.lev_ln_step.empty() ? nullptr : A.lev_ln_step.data();
+    b.lev_offset = A.lev_offset.empty() ? nullptr : A.lev_offset.data();
+    b.lev_values = A.lev_values.empty() ? nullptr : A.lev_values.data();
+    b.disc_rate = A.disc.data();
+    b.n_rainbows = req.n_rainbows;
+    b.rb_best = A.rb_best.data();
+    b.rb_n = A.rb_n.data();
+    b.rb_member = A.rb_member.data();
+    b.rb_weight = A.rb_weight.data();
+    b.n_contracts = req.n_contracts;
+    b.c_kind = A.c_kind.data();
+    b.c_is_call = A.c_call.data();
+    b.c_underlying = A.c_udl.data();
+    b.c_flow_index = A.c_flow.data();
+    b.c_strike = A.c_strike.data();
+    b.c_is_up = A.c_up.data();
+    b.c_is_in = A.c_in.data();
+    b.c_level = A.c_level.data();
+    b.c_mon_offset = A.c_mon_off.data();
+    b.c_mon_index = A.c_mon.data();
+}
+
+}  // namespace
+
+// A request resolved into device-resident books (base + bump scenarios).
+struct Plan
+{
+    DeviceBatch batch;
+    std::vector<int> kind;
+    std::vector<double> h;
+    int n_contracts = 0, n_dates = 0;
+    double calib_ms = 0;
+};
+
+namespace
+{
+
+struct Scenario
+{
+    Market mk;
+    int kind;  // 0 base, 1 delta, 2 gamma up, 3 gamma down, 4 vega, 5 rho
+    int asset;
+    double h;
+};
+
+double Mean( const thoth_mcl_book_out& o, int c ) { return ( c < 0 ? o.book_sum : o.contract_sum[c] ) / (double)o.count; }
+
+double Trust( const thoth_mcl_book_out& o, int c )
+{
+    const double n = (double)o.count;
+    const double s = c < 0 ? o.book_sum : o.contract_sum[c];
+    const double s2 = c < 0 ? o.book_sum2 : o.contract_sum2[c];
+    return std::sqrt( ( s2 - s * s / n ) / ( n - 1 ) / n );  // monte_carlo_node.cpp:35-40
+}
+
+}  // namespace
+
+int PreparePlan( const thoth_mcl_request& req, Plan& plan )
+{
+    // mcl_configuration.cpp:30-46 guards
+    if ( req.mcl.paths <= 1 || req.mcl.paths > 200000000 )
+        throw Invalid( THOTH_ERR_INVALID, "paths must be in (1, 2e8]" );
+    if ( !req.mcl.use_sobol )
+        throw Invalid( THOTH_ERR_UNSUPPORTED,
+                       "the device engine runs the Sobol + Brownian-bridge mode only (use_sobol: true)" );
+    if ( req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS || !req.assets )
+        throw Invalid( THOTH_ERR_INVALID, "1..8 equities" );
+    if ( req.n_contracts < 1 || req.n_contracts > THOTH_MCL_MAX_CONTRACTS || !req.contracts )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "1..32 contracts per book" );
+    if ( req.precision != THOTH_FP64 && req.precision != THOTH_FP32 )
+        throw Invalid( THOTH_ERR_INVALID, "precision" );
+    const Market base = BuildMarket( req );
+    const Grid g = BuildGrid( req, base );
+    // Cholesky of the spot/spot sub-matrix (pricer_mcl.cpp:462-491, correlation.cpp:794-859)
+    std::vector<double> chol;
+    const int n = req.n_assets;
+    if ( n > 1 )
+    {
+        if ( !req.correlation )
+            throw Invalid( THOTH_ERR_INVALID, "book pricing requires a correlation matrix" );
+        chol.assign( req.correlation, req.correlation + n * n );
+        if ( !CholeskyLower( chol, n ) )
+            throw Invalid( THOTH_ERR_INVALID, "correlation is not SDP" );
+        for ( int i = 0; i < n; i++ )
+            for ( int j = i + 1; j < n; j++ )
+                chol[i * n + j] = 0;
+    }
+    // scenarios of the bump-and-revalue Greeks (pricer.cpp:287-380)
+    std::vector<Scenario> sc;
+    sc.push_back( { base, 0, -1, 0 } );
+    for ( int a = 0; a < n; a++ )
+    {
+        const double spot = base.eq[a].spot;
+        if ( req.request_greeks & THOTH_GREEK_DELTA )
+        {
+            Scenario s{ base, 1, a, kSpotBump * spot };
+            s.mk.eq[a].spot = spot + s.h;
+            sc.push_back( s );
+        }
+        if ( req.request_greeks & THOTH_GREEK_GAMMA )
+        {
+            Scenario u{ base, 2, a, kGammaBump * spot }, d{ base, 3, a, kGammaBump * spot };
+            u.mk.eq[a].spot = spot + u.h;
+            d.mk.eq[a].spot = spot - d.h;
+            sc.push_back( u );
+            sc.push_back( d );
+        }
+    }
+    if ( req.request_greeks & THOTH_GREEK_VEGA )
+    {
+        Scenario s{ base, 4, -1, kVolBump };
+        for ( Equity& e : s.mk.eq )
+            e.vol_shift = kVolBump;
+        sc.push_back( s );
+    }
+    if ( req.request_greeks & THOTH_GREEK_RHO )
+    {
+        Scenario s{ base, 5, -1, kRateBump };
+        s.mk.rate.shift = kRateBump;
+        sc.push_back( s );
+    }
+    // leverage calibration per scenario (each scenario rebuilds the leverage node, single.cpp:479-500)
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::vector<LeverageGrid>> levs( sc.size(), std::vector<LeverageGrid>( n ) );
+    for ( size_t k = 0; k < sc.size(); k++ )
+        for ( int a = 0; a < n; a++ )
+            if ( sc[k].mk.eq[a].kind == THOTH_VOL_LSV )
+            {
+                int rc = CalibrateLeverage( sc[k].mk, a, g, levs[k][a] );
+                if ( rc != THOTH_OK )
+                    return rc;
+            }
+    const double calib_ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - t0 ).count();
+    std::vector<BookArrays> arrays( sc.size() );
+    std::vector<thoth_mcl_book> books;
+    for ( size_t k = 0; k < sc.size(); k++ )
+    {
+        Resolve( req, sc[k].mk, g, chol, levs[k], arrays[k] );
+        books.push_back( arrays[k].book );
+    }
+    thoth_mcl_sweep_cfg cfg{};
+    cfg.paths = req.mcl.paths;
+    cfg.sobol_skip = req.mcl.sobol_skip;
+    cfg.precision = req.precision;
+    int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, plan.batch );
+    if ( rc != THOTH_OK )
+        return rc;
+    plan.kind.clear();
+    plan.h.clear();
+    for ( const Scenario& s : sc )
+    {
+        plan.kind.push_back( s.kind );
+        plan.h.push_back( s.h );
+    }
+    plan.n_contracts = req.n_contracts;
+    plan.n_dates = (int)g.t.size();
+    plan.calib_ms = calib_ms;
+    return THOTH_OK;
+}
+
+int RunPlan( const Plan& plan, thoth_mcl_result* res )
+{
+    std::vector<thoth_mcl_book_out> out( plan.kind.size() );
+    int rc = RunBatch( plan.batch, out.data() );
+    if ( rc != THOTH_OK )
+        return rc;
+    // Tree_Read (pricer_mcl.cpp:677-709) + ComputeGreeks finite differences
+    const int nc = plan.n_contracts;
+    const thoth_mcl_book_out& o0 = out[0];
+    const double p0 = Mean( o0, -1 );
+    res->premium = p0;
+    res->premium_trust = Trust( o0, -1 );
+    res->delta = res->gamma = res->vega = res->rho = 0;
+    std::vector<double> cd( nc, 0 ), cg( nc, 0 ), cv( nc, 0 ), cr( nc, 0 );
+    for ( size_t k = 1; k < plan.kind.size(); k++ )
+    {
+        const thoth_mcl_book_out& o = out[k];
+        const double h = plan.h[k];
+        switch ( plan.kind[k] )
+        {
+        case 1:
+            res->delta += ( Mean( o, -1 ) - p0 ) / h;
+            for ( int c = 0; c < nc; c++ )
+                cd[c] += ( Mean( o, c ) - Mean( o0, c ) ) / h;
+            break;
+        case 2:
+        {
+            const thoth_mcl_book_out& od = out[k + 1];
+            res->gamma += ( Mean( o, -1 ) - 2 * p0 + Mean( od, -1 ) ) / ( h * h );
+            for ( int c = 0; c < nc; c++ )
+                cg[c] += ( Mean( o, c ) - 2 * Mean( o0, c ) + Mean( od, c ) ) / ( h * h );
+            break;
+        }
+        case 4:
+            res->vega = ( Mean( o, -1 ) - p0 ) / kVolBump * 0.01;
+            for ( int c = 0; c < nc; c++ )
+                cv[c] = ( Mean( o, c ) - Mean( o0, c ) ) / kVolBump * 0.01;
+            break;
+        case 5:
+            res->rho = ( Mean( o, -1 ) - p0 ) / kRateBump * 0.01;
+            for ( int c = 0; c < nc; c++ )
+                cr[c] = ( Mean( o, c ) - Mean( o0, c ) ) / kRateBump * 0.01;
+            break;
+        default:
+            break;
+        }
+    }
+    for ( int c = 0; c < nc; c++ )
+    {
+        if ( res->contract_premium )
+            res->contract_premium[c] = Mean( o0, c );
+        if ( res->contract_trust )
+            res->contract_trust[c] = Trust( o0, c );
+        if ( res->contract_delta )
+            res->contract_delta[c] = cd[c];
+        if ( res->contract_gamma )
+            res->contract_gamma[c] = cg[c];
+        if ( res->contract_vega )
+            res->contract_vega[c] = cv[c];
+        if ( res->contract_rho )
+            res->contract_rho[c] = cr[c];
+    }
+    res->n_dates = plan.n_dates;
+    res->n_scenarios = (int32_t)plan.kind.size();
+    res->sweep_ms = out[0].device_ms;
+    res->calib_ms = plan.calib_ms;
+    return THOTH_OK;
+}
+
+
+int PriceImpl( const thoth_mcl_request& req, thoth_mcl_result* res )
+{
+    Plan plan;
+    int rc = PreparePlan( req, plan );
+    if ( rc != THOTH_OK )
+        return rc;
+    return RunPlan( plan, res );
+}
+
+}  // namespace thoth::host
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price( const thoth_mcl_request* req,
+                                                                         thoth_mcl_result* res )
+{
+    if ( !req || !res )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    try
+    {
+        int rc = thoth::EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        return thoth::host::PriceImpl( *req, res );
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_prepare( const thoth_mcl_request* req,
+                                                                           thoth_mcl_plan* plan )
+{
+    if ( !req || !plan )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    *plan = nullptr;
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    try
+    {
+        int rc = thoth::EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        auto* p = new thoth::host::Plan();
+        rc = thoth::host::PreparePlan( *req, *p );
+        if ( rc != THOTH_OK )
+        {
+            delete p;
+            return rc;
+        }
+        *plan = reinterpret_cast<thoth_mcl_plan>( p );
+        return THOTH_OK;
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_run( thoth_mcl_plan plan, thoth_mcl_result* res )
+{
+    if ( !plan || !res )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    return thoth::host::RunPlan( *reinterpret_cast<thoth::host::Plan*>( plan ), res );
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) void thoth_mcl_release( thoth_mcl_plan plan )
+{
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    delete reinterpret_cast<thoth::host::Plan*>( plan );
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_grid( const thoth_mcl_request* req, int32_t cap,
+                                                                        double* days, double* t, double* dt )
+{
+    if ( !req || cap < 0 )
+        return -thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    try
+    {
+        const thoth::host::Market mk = thoth::host::BuildMarket( *req );
+        const thoth::host::Grid g = thoth::host::BuildGrid( *req, mk );
+        for ( int i = 0; i < (int)g.t.size() && i < cap; i++ )
+        {
+            days[i] = g.days[i];
+            t[i] = g.t[i];
+            dt[i] = g.dt[i];
+        }
+        return (int)g.t.size();
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return -thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return -thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_calibrate_leverage(
+    const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_dates, double* ln_step, int64_t* offset,
+    double* levels )
+{
+    if ( !req || asset < 0 || asset >= req->n_assets )
+        return -thoth::SetError( THOTH_ERR_INVALID, "asset index" );
+    try
+    {
+        thoth::host::Market mk = thoth::host::BuildMarket( *req );
+        if ( mk.eq[asset].kind != THOTH_VOL_LSV )
+            return -thoth::SetError( THOTH_ERR_INVALID, "equity is not LSV" );
+        mk.eq[asset].spot = spot;
+        const thoth::host::Grid g = thoth::host::BuildGrid( *req, mk );
+        thoth::host::LeverageGrid L;
+        int rc = thoth::host::CalibrateLeverage( mk, asset, g, L );
+        if ( rc != THOTH_OK )
+            return -rc;
+        const int n = (int)L.t.size();
+        for ( int k = 0; k < n && k < cap_dates; k++ )
+        {
+            ln_step[k] = L.ln_step[k];
+            offset[k] = L.offset[k];
+            std::copy( L.values.begin() + (size_t)k * L.points, L.values.begin() + (size_t)( k + 1 ) * L.points,
+                       levels + (size_t)k * L.points );
+        }
+        return n;
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return -thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return -thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}

@@ -1,0 +1,85 @@
+// mcl_hostlib.hpp -- market model of the host engine: curves, equities with
+// their bs / heston / lsv volatility, the SABR target surface and its Dupire
+// local volatility, the diffusion grid, the Cholesky factor and the LSV
+// leverage calibration.  All FP64 (the north-star keeps these steps in FP64).
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "thoth_mcl.h"
+
+namespace thoth::host
+{
+
+constexpr double kDaysPerYear = 365.0;  // helpers/constants.hpp:7 (ACT/365)
+
+inline double YearFraction( double from, double to ) { return ( to - from ) / kDaysPerYear; }  // misc.hpp:54-57
+
+// zero curve, linear in rate between pillars, flat outside (curve.cpp:45-73)
+struct Curve
+{
+    std::vector<double> days, values;  // decimal
+    double shift = 0;                  // rho bump
+    double Value( double day ) const;
+};
+
+// Hagan lognormal SABR with BDK power-law wings (sabr_volatility.cpp:40-239)
+struct Sabr
+{
+    std::vector<double> mats, alpha, beta, rho, nu;
+    double Interp( const std::vector<double>& v, double t ) const;
+    double Hagan( double k, double f, double t ) const;
+    double Wing( double k, double f, double t, double cutoff, bool right ) const;
+    double Implied( double strike, double forward, double day ) const;
+};
+
+struct StochParams
+{
+    double v0 = 0, kappa = 0, theta = 0, xi = 0, rho = 0;
+};
+
+struct Equity
+{
+    double spot = 0;
+    int kind = THOTH_VOL_BS;
+    double bs_vol = 0, v0 = 0, theta = 0, kappa = 0, xi = 0, rho = 0;
+    bool has_div = false;
+    double div = 0;
+    Sabr surface;
+    double vol_shift = 0;  // vega bump (Volatility::_vol_shift)
+};
+
+struct Market
+{
+    std::vector<Equity> eq;
+    Curve rate;
+    StochParams Stoch( int a ) const;                       // StochasticParams()
+    double DivRepo( int a, double day ) const;              // equity.cpp:149-154
+    double Drift( int a, double day ) const;                // drift_node.cpp:20-27
+    double Forward( int a, double day ) const;              // equity.cpp:160-191
+    double ImplicitVol( int a, double strike, double day ) const;  // single.cpp:56-62
+    double LocalVol( int a, double strike, double day ) const;     // equity.cpp:196-211 + volatility.cpp:55-128
+};
+
+struct Grid
+{
+    std::vector<double> days, t, dt;
+    bool uniform = false;
+};
+
+// leverage layers of one equity on the diffusion grid (leverage_surface.hpp)
+struct LeverageGrid
+{
+    int points = 0;
+    std::vector<double> t, ln_step;
+    std::vector<int64_t> offset;
+    std::vector<double> values;  // [dates * points]
+};
+
+Market BuildMarket( const thoth_mcl_request& req );
+Grid BuildGrid( const thoth_mcl_request& req, const Market& mk );
+bool CholeskyLower( std::vector<double>& a, int n );  // maths.cpp:354-381
+int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& out );
+
+}  // namespace thoth::host

@@ -1,0 +1,58 @@
+// mcl_internal.hpp -- host-side declarations shared by the device library's
+// translation units (not part of the public C-ABI).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mcl_device.hpp"
+#include "thoth_mcl_gpu.h"
+
+namespace thoth
+{
+
+constexpr int kSobolDims = 4097;  // implicit first dimension + 4096 Joe-Kuo rows
+
+struct BridgeProgram
+{
+    double sd0 = 0;
+    std::vector<int> step_ops;  // [m + 1]
+    std::vector<int> op_b, op_left;
+    std::vector<double> op_w;   // [n_ops * 3]
+    int max_depth = 1;
+};
+
+const std::vector<uint32_t>& SobolDirections();
+BridgeProgram BuildBridgeProgram( const double* t, int n_dates );
+int EnsureDevice();
+std::mutex& DeviceMutex();  // serialises every public entry point
+hipStream_t Stream();
+const uint32_t* DeviceDirections();
+int SetError( int code, const std::string& msg );
+struct DevBook;
+// A packed, uploaded batch of books ready to sweep (device-resident inputs).
+struct DeviceBatch
+{
+    char* dbase = nullptr;
+    double* dpart = nullptr;
+    const DevBook* dbooks = nullptr;
+    std::vector<DevBook> db;
+    std::vector<size_t> part_off;
+    size_t part_total = 0;
+    uint32_t q0 = 0, nwaves = 0, nblocks = 0, first = 0, last = 0;
+    int n_books = 0, depth = 1, na_max = 1, precision = 0;
+    int64_t paths = 0;
+    DeviceBatch() = default;
+    DeviceBatch( const DeviceBatch& ) = delete;
+    DeviceBatch& operator=( const DeviceBatch& ) = delete;
+    ~DeviceBatch();
+};
+int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B );
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out );
+int SweepBooks( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg,
+                thoth_mcl_book_out* out );
+
+}  // namespace thoth

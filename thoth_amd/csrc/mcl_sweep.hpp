@@ -1,0 +1,528 @@
+// mcl_sweep.hpp -- the fused Sobol -> inverse-normal -> Brownian-bridge ->
+// QE/Andersen/LSV -> payoff -> reduction path-sweep kernel (gfx950, wave64).
+//
+// One lane = one path.  A wave owns 64 consecutive Sobol indices aligned on 64,
+// so gray(64q + j) = gray(64q) ^ gray(j): the wave-uniform part of every Sobol
+// coordinate comes from scalar loads of the direction numbers, the per-lane part
+// from the 6 low bits of the lane id.  The Brownian bridge is evaluated in time
+// order (an in-order walk of the reference's bridge tree) with a per-factor
+// stack of pending right anchors held in VGPRs and indexed by a wave-uniform
+// stack pointer.  Included by mcl_sweep_f64.hip (compiled with
+// -ffp-contract=off, oracle-matched FP64) and mcl_sweep_f32.hip (FP32 stepping).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "mcl_device.hpp"
+
+namespace thoth
+{
+
+__device__ __forceinline__ uint32_t gray32( uint32_t n ) { return n ^ ( n >> 1 ); }
+
+// Integer Sobol coordinate of dimension d at index (wave block, lane).
+// Linear in the Gray-code bits over GF(2): reference sequential recurrence
+// (sobol_generator.cpp:80-100) gives x = XOR_{k in gray(n)} V[d][k].
+__device__ __forceinline__ uint32_t sobol_x( const uint32_t* __restrict__ V, int d, uint32_t G, uint32_t g )
+{
+    const uint32_t* v = V + (size_t)d * 32;
+    uint32_t x = 0;
+    uint32_t gg = G;
+    while ( gg )
+    {
+        const int k = __builtin_ctz( gg );
+        x ^= v[k];
+        gg &= gg - 1;
+    }
+#pragma unroll
+    for ( int k = 0; k < 6; k++ )
+        x ^= ( ( g >> k ) & 1u ) ? v[k] : 0u;
+    return x;
+}
+
+// Acklam inverse normal with the reference clamp (path_generator.cpp:148-150,
+// distributions.cpp:37-81), FP64.
+__device__ __forceinline__ double normal_f64( uint32_t x )
+{
+    const double u0 = x * ( 1.0 / 4294967296.0 );
+    const double P = fmin( 1.0 - 1e-12, fmax( 1e-12, u0 ) );
+    const double a0 = -3.969683028665376e+01, a1 = 2.209460984245205e+02, a2 = -2.759285104469687e+02,
+                 a3 = 1.383577518672690e+02, a4 = -3.066479806614716e+01, a5 = 2.506628277459239e+00;
+    const double b0 = -5.447609879822406e+01, b1 = 1.615858368580409e+02, b2 = -1.556989798598866e+02,
+                 b3 = 6.680131188771972e+01, b4 = -1.328068155288572e+01;
+    const double c0 = -7.784894002430293e-03, c1 = -3.223964580411365e-01, c2 = -2.400758277161838e+00,
+                 c3 = -2.549732539343734e+00, c4 = 4.374664141464968e+00, c5 = 2.938163982698783e+00;
+    const double d0 = 7.784695709041462e-03, d1 = 3.224671290700398e-01, d2 = 2.445134137142996e+00,
+                 d3 = 3.754408661907416e+00;
+    const double p_low = 0.02425;
+    const double p_high = 1.0 - p_low;
+    if ( P < p_low )
+    {
+        double q = sqrt( -2.0 * log( P ) );
+        return ( ( ( ( ( c0 * q + c1 ) * q + c2 ) * q + c3 ) * q + c4 ) * q + c5 ) /
+               ( ( ( ( d0 * q + d1 ) * q + d2 ) * q + d3 ) * q + 1.0 );
+    }
+    if ( P <= p_high )
+    {
+        double q = P - 0.5;
+        double r = q * q;
+        return ( ( ( ( ( a0 * r + a1 ) * r + a2 ) * r + a3 ) * r + a4 ) * r + a5 ) * q /
+               ( ( ( ( ( b0 * r + b1 ) * r + b2 ) * r + b3 ) * r + b4 ) * r + 1.0 );
+    }
+    double q = sqrt( -2.0 * log( 1.0 - P ) );
+    return -( ( ( ( ( c0 * q + c1 ) * q + c2 ) * q + c3 ) * q + c4 ) * q + c5 ) /
+           ( ( ( ( d0 * q + d1 ) * q + d2 ) * q + d3 ) * q + 1.0 );
+}
+
+// Acklam's rational approximation evaluated in FP32 from the integer
+// coordinate: the tail probability min(u, 1-u) is formed exactly in integers
+// (so u near 1 keeps its precision), the centred variable u - 1/2 exactly
+// up to the int->float rounding.  The clamp of the reference never binds for
+// Sobol points (the smallest nonzero coordinate is 2^-32 > 1e-12).
+__device__ __forceinline__ float normal_f32( uint32_t x )
+{
+    const uint32_t xm = ( x < 0x80000000u ) ? x : ( 0u - x );
+    if ( xm < 104152957u )  // min(u,1-u) < 0.02425
+    {
+        const float p = fmaxf( (float)xm, 1.0f ) * 2.3283064365386963e-10f;
+        const float q = sqrtf( -2.0f * __logf( p ) );
+        const float num = ( ( ( ( -7.784894002430293e-03f * q - 3.223964580411365e-01f ) * q - 2.400758277161838e+00f ) * q -
+                              2.549732539343734e+00f ) * q + 4.374664141464968e+00f ) * q + 2.938163982698783e+00f;
+        const float den = ( ( ( 7.784695709041462e-03f * q + 3.224671290700398e-01f ) * q + 2.445134137142996e+00f ) * q +
+                            3.754408661907416e+00f ) * q + 1.0f;
+        const float z = num / den;
+        return ( x < 0x80000000u ) ? z : -z;
+    }
+    const float q = (float)(int32_t)( x - 0x80000000u ) * 2.3283064365386963e-10f;
+    const float r = q * q;
+    const float num = ( ( ( ( -3.969683028665376e+01f * r + 2.209460984245205e+02f ) * r - 2.759285104469687e+02f ) * r +
+                          1.383577518672690e+02f ) * r - 3.066479806614716e+01f ) * r + 2.506628277459239e+00f;
+    const float den = ( ( ( ( -5.447609879822406e+01f * r + 1.615858368580409e+02f ) * r - 1.556989798598866e+02f ) * r +
+                          6.680131188771972e+01f ) * r - 1.328068155288572e+01f ) * r + 1.0f;
+    return num * q / den;
+}
+
+template <class R>
+__device__ __forceinline__ R normal_of( uint32_t x )
+{
+    if constexpr ( std::is_same_v<R, double> )
+        return normal_f64( x );
+    else
+        return normal_f32( x );
+}
+
+// Andersen QE variance step, FP64, in the reference's operation order
+// (heston_variance_node.cpp:17-57).
+__device__ __forceinline__ double qe_f64( double v, double dt, double z, double kappa, double theta, double xi )
+{
+    const double e = exp( -kappa * dt );
+    const double m = theta + ( v - theta ) * e;
+    const double s2 = v * xi * xi * e / kappa * ( 1 - e ) + theta * xi * xi / ( 2 * kappa ) * ( 1 - e ) * ( 1 - e );
+    const double psi = ( m > 0 ) ? s2 / ( m * m ) : 0;
+    if ( psi <= 1.5 )
+    {
+        const double c = 2.0 / psi;
+        const double b2 = c - 1.0 + sqrt( c ) * sqrt( c - 1.0 );
+        const double a = m / ( 1.0 + b2 );
+        const double bz = sqrt( b2 ) + z;
+        return a * bz * bz;
+    }
+    const double p = ( psi - 1.0 ) / ( psi + 1.0 );
+    const double beta = ( 1.0 - p ) / m;
+    const double u = 0.5 * erfc( -z / sqrt( 2.0 ) );
+    return ( u <= p ) ? 0.0 : log( ( 1.0 - p ) / ( 1.0 - u ) ) / beta;
+}
+
+// Same scheme in FP32 from host-precomputed step constants.  1-u is formed
+// directly as Phi(-z) so the exponential tail keeps its precision.
+__device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc )
+{
+    const float m = sc[SC_M0] + v * sc[SC_E];
+    const float s2 = sc[SC_S0] + v * sc[SC_S1];
+    const float psi = ( m > 0.0f ) ? s2 / ( m * m ) : 0.0f;
+    if ( psi <= 1.5f )
+    {
+        const float c = 2.0f / psi;
+        const float b2 = c - 1.0f + sqrtf( c * ( c - 1.0f ) );
+        const float a = m / ( 1.0f + b2 );
+        const float bz = sqrtf( b2 ) + z;
+        return a * bz * bz;
+    }
+    const float p = ( psi - 1.0f ) / ( psi + 1.0f );
+    const float beta = ( 1.0f - p ) / m;
+    const float one_minus_u = 0.5f * erfcf( z * 0.70710678118654752f );
+    return ( one_minus_u >= 1.0f - p ) ? 0.0f : __logf( ( 1.0f - p ) / one_minus_u ) / beta;
+}
+
+template <class R>
+__device__ __forceinline__ R wave_sum( R x )
+{
+#pragma unroll
+    for ( int off = 32; off >= 1; off >>= 1 )
+        x += __shfl_xor( x, off, 64 );
+    return x;
+}
+
+template <class R, int NA, int DEPTH>
+__global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restrict__ books,
+                                                         const uint32_t* __restrict__ V, uint32_t q0,
+                                                         uint32_t nwaves, uint32_t first_idx, uint32_t last_idx )
+{
+    constexpr bool F64 = std::is_same_v<R, double>;
+    constexpr int NF = 2 * NA;
+    constexpr int NFV = NF <= 2 ? 2 : NF <= 4 ? 4 : NF <= 8 ? 8 : 16;
+    constexpr int NAV = NA <= 1 ? 1 : NA <= 2 ? 2 : NA <= 4 ? 4 : 8;
+    typedef R stackv __attribute__( ( ext_vector_type( DEPTH ) ) );
+    typedef R facv __attribute__( ( ext_vector_type( NFV ) ) );
+    typedef R assetv __attribute__( ( ext_vector_type( NAV ) ) );
+
+    const DevBook& B = books[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane( blockIdx.x * kWavesPerBlock + wib );
+    const uint32_t q = q0 + wave;
+    const uint32_t idx = q * 64u + (uint32_t)lane;
+    const bool valid = wave < nwaves && idx >= first_idx && idx <= last_idx;
+    const uint32_t G = __builtin_amdgcn_readfirstlane( gray32( q * 64u ) );
+    const uint32_t g = gray32( (uint32_t)lane );
+
+    const int F = B.factors;
+    const int m = B.m;
+    const int na = B.n_assets;
+    const int nd = B.n_dates;
+    const int nc = B.n_contracts;
+
+    __shared__ double acc[kWavesPerBlock][kMaxContracts + 1][2];
+    for ( int k = threadIdx.x; k < kWavesPerBlock * ( kMaxContracts + 1 ) * 2; k += kBlock )
+        ( &acc[0][0][0] )[k] = 0.0;
+    __syncthreads();
+
+    // ---- bridge state: endpoint first (build index 0, path_generator.cpp:166-167)
+    stackv stk[NF];
+    R top[NF], prev[NF];
+#pragma unroll
+    for ( int f = 0; f < NF; f++ )
+    {
+        stk[f] = (stackv)( R( 0 ) );
+        prev[f] = R( 0 );
+        top[f] = R( 0 );
+        if ( f < F )
+        {
+            const R z = normal_of<R>( sobol_x( V, f, G, g ) );
+            if constexpr ( F64 )
+                top[f] = B.sd0 * z;
+            else
+                top[f] = B.sd0f * z;
+        }
+    }
+    int sp = 1;
+
+    // ---- equity state
+    assetv S, Var, W;
+#pragma unroll
+    for ( int a = 0; a < NAV; a++ )
+    {
+        S[a] = R( 0 );
+        Var[a] = R( 0 );
+        W[a] = R( 0 );
+    }
+#pragma unroll
+    for ( int a = 0; a < NA; a++ )
+        if ( a < na )
+        {
+            S[a] = R( B.asset[a].spot );
+            Var[a] = R( B.asset[a].v0 );
+        }
+    uint32_t hit = 0;
+    double book = 0.0;
+
+    auto udl_value = [&]( int u ) -> R
+    {
+        if ( u >= 0 )
+            return S[__builtin_amdgcn_readfirstlane( u )];
+        const int r = -1 - u;  // rainbow_node.cpp:14-27
+        const bool best = B.rb_best[r] != 0;
+        R x = best ? R( -INFINITY ) : R( INFINITY );
+        const int n = B.rb_n[r];
+        for ( int j = 0; j < n; j++ )
+        {
+            const R perf = S[__builtin_amdgcn_readfirstlane( B.rb_member[r][j] )] * R( B.rb_weight[r][j] );
+            x = best ? ( x < perf ? perf : x ) : ( perf < x ? perf : x );
+        }
+        return x;
+    };
+
+    auto observe = [&]( int i )
+    {
+        uint32_t mon = B.mon_mask[i];
+        while ( mon )
+        {
+            const int c = __builtin_ctz( mon );
+            mon &= mon - 1;
+            const R s = udl_value( B.c_udl[c] );
+            const R H = R( B.c_level[c] );
+            const bool h = B.c_up[c] ? ( s >= H ) : ( s <= H );  // barrier_flow_node.cpp:35-37
+            if ( h )
+                hit |= 1u << c;
+        }
+        uint32_t fl = B.flow_mask[i];
+        while ( fl )
+        {
+            const int c = __builtin_ctz( fl );
+            fl &= fl - 1;
+            const R s = udl_value( B.c_udl[c] );
+            const R K = R( B.c_strike[c] );
+            R vanilla = B.c_call[c] ? ( s - K ) : ( K - s );  // finance.cpp:9-32
+            vanilla = vanilla < R( 0 ) ? R( 0 ) : vanilla;
+            R flow = vanilla;
+            if ( B.c_kind[c] == 1 )
+            {
+                const bool h = ( hit >> c ) & 1u;
+                const bool alive = B.c_in[c] ? h : !h;
+                flow = alive ? vanilla : R( 0 );
+            }
+            double x = (double)flow * B.c_df[c];  // contract_node.cpp:33-47
+            if ( !valid )
+                x = 0.0;
+            book += x;
+            const double s1 = wave_sum<double>( x );
+            const double s2 = wave_sum<double>( x * x );
+            if ( lane == 0 )
+            {
+                acc[wib][1 + c][0] += s1;
+                acc[wib][1 + c][1] += s2;
+            }
+        }
+    };
+
+    observe( 0 );
+
+    for ( int p = 0; p < m; p++ )
+    {
+        const int i = p + 1;
+        // ---- bridge: build the pending points of this step (in-order tree walk)
+        const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
+        for ( int o = o0; o < o1; o++ )
+        {
+            const int bidx = B.op_b[o];
+            const bool left = B.op_left[o] != 0;
+            const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
+            R lw, rw, sd;
+            if constexpr ( F64 )
+            {
+                lw = B.op_w[3 * o];
+                rw = B.op_w[3 * o + 1];
+                sd = B.op_w[3 * o + 2];
+            }
+            else
+            {
+                lw = B.op_wf[3 * o];
+                rw = B.op_wf[3 * o + 1];
+                sd = B.op_wf[3 * o + 2];
+            }
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+            {
+                if ( f < F )
+                {
+                    const R z = normal_of<R>( sobol_x( V, bidx * F + f, G, g ) );
+                    // path_generator.cpp:175-182
+                    const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
+                    stk[f][pos] = top[f];
+                    top[f] = nv;
+                }
+            }
+            sp++;
+        }
+        // ---- pop W at this date -> normalized increment (path_generator.cpp:188-196)
+        facv nz;
+        {
+            const int pos = __builtin_amdgcn_readfirstlane( sp >= 2 ? sp - 2 : 0 );
+#pragma unroll
+            for ( int f = 0; f < NFV; f++ )
+                nz[f] = R( 0 );
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+            {
+                if ( f < F )
+                {
+                    const R w = top[f];
+                    if constexpr ( F64 )
+                        nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
+                    else
+                        nz[f] = ( w - prev[f] ) * B.binv[i];
+                    prev[f] = w;
+                    if ( sp >= 2 )
+                        top[f] = stk[f][pos];
+                }
+            }
+            sp--;
+        }
+        // ---- equities
+#pragma unroll
+        for ( int a = 0; a < NA; a++ )
+        {
+            if ( a >= na )
+                continue;
+            const int model = B.asset[a].model;
+            // correlated noise (correlated_noise_node.cpp:15-27, lower triangle, name order)
+            R eps = nz[a];
+            if ( na > 1 && ( model == 0 || B.corr_stoch ) )
+            {
+                eps = R( 0 );
+#pragma unroll
+                for ( int v = 0; v <= a; v++ )
+                {
+                    if constexpr ( F64 )
+                    {
+                        if ( B.chol[a * kMaxAssets + v] != 0.0 )
+                            eps += nz[v] * B.chol[a * kMaxAssets + v];
+                    }
+                    else
+                        eps += nz[v] * B.cholf[a * kMaxAssets + v];
+                }
+            }
+            const float* sc = B.sc + ( (size_t)a * nd + i ) * SC_COUNT;
+            R cd;
+            if constexpr ( F64 )
+                cd = B.drift[(size_t)a * nd + i] * B.t[i] - B.drift[(size_t)a * nd + i - 1] * B.t[i - 1];
+            else
+                cd = sc[SC_CARRY];
+            if ( model == 0 )
+            {
+                // brownian_node.cpp:12-24 + spot_diffusion_node.cpp:21-65
+                if constexpr ( F64 )
+                {
+                    const double w1 = W[a] + B.sqrt_dt[i] * eps;
+                    const double dw = w1 - W[a];
+                    W[a] = w1;
+                    const double v = B.asset[a].vol;
+                    const double expo = cd - v * v / 2 * B.dt[i] + v * dw;
+                    S[a] = ( S[a] - 0 ) * exp( expo ) + 0;
+                }
+                else
+                {
+                    const float w1 = W[a] + sc[SC_SQDT] * eps;
+                    const float dw = w1 - W[a];
+                    W[a] = w1;
+                    const float expo = cd - sc[SC_HALF_V2DT] + sc[SC_VOL] * dw;
+                    S[a] = S[a] * expf( expo );
+                }
+                continue;
+            }
+            // Heston / LSV (heston_variance_node.cpp, heston_spot_node.cpp)
+            const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+            const R zv = nz[vf];
+            const R vp = Var[a];
+            R vc;
+            if constexpr ( F64 )
+                vc = qe_f64( vp, B.dt[i], zv, B.asset[a].kappa, B.asset[a].theta, B.asset[a].xi );
+            else
+                vc = qe_f32( vp, zv, sc );
+            Var[a] = vc;
+            R L = R( 1 );
+            if ( model == 2 )
+            {
+                // local_volatility_node.cpp:19-47: layer i at the previous spot
+                const size_t li = (size_t)a * nd + i;
+                const int last = B.lev_points - 1;
+                R f;
+                if constexpr ( F64 )
+                    f = log( S[a] ) / B.lev_ln_step[li] - B.lev_offset[li];
+                else
+                    f = __logf( S[a] ) * B.lev_inv_step[li] - (float)B.lev_offset[li];
+                if constexpr ( F64 )
+                {
+                    const double* lv = B.lev + li * B.lev_points;
+                    if ( f <= 0.0 )
+                        L = lv[0];
+                    else if ( f >= (double)last )
+                        L = lv[last];
+                    else
+                    {
+                        const int k = (int)f;
+                        const double w = f - (double)k;
+                        L = ( 1.0 - w ) * lv[k] + w * lv[k + 1];
+                    }
+                }
+                else
+                {
+                    const float* lv = B.levf + li * B.lev_points;
+                    const float fc = fminf( fmaxf( f, 0.0f ), (float)last );
+                    const int k = min( (int)fc, last - 1 );
+                    const float w = fc - (float)k;
+                    L = lv[k] + w * ( lv[k + 1] - lv[k] );
+                }
+            }
+            const R z = eps;
+            R step;
+            if constexpr ( F64 )
+            {
+                const double xi = B.asset[a].xi, rho = B.asset[a].rho, kappa = B.asset[a].kappa,
+                             theta = B.asset[a].theta, dt = B.dt[i];
+                if ( xi > 1e-12 )
+                {
+                    const double gg = 0.5;
+                    const double K0 = -L * rho * kappa * theta / xi * dt;
+                    const double aa = gg * dt * ( L * kappa * rho / xi - 0.5 * L * L );
+                    const double K1 = aa - L * rho / xi;
+                    const double K2 = aa + L * rho / xi;
+                    const double K3 = gg * dt * ( 1.0 - rho * rho ) * L * L;
+                    const double K4 = K3;
+                    double var = K3 * vp + K4 * vc;
+                    if ( var < 0 )
+                        var = 0;
+                    step = cd + K0 + K1 * vp + K2 * vc + sqrt( var ) * z;
+                }
+                else
+                {
+                    step = cd - 0.5 * L * L * vp * dt + L * sqrt( vp * dt ) * z;
+                }
+                S[a] = S[a] * exp( step );
+            }
+            else
+            {
+                if ( B.asset[a].xi > 1e-12 )
+                {
+                    const float aa = L * sc[SC_A1] - L * L * sc[SC_A2];
+                    const float K1 = aa - L * sc[SC_RX];
+                    const float K2 = aa + L * sc[SC_RX];
+                    const float var = fmaxf( L * L * sc[SC_K3] * ( vp + vc ), 0.0f );
+                    step = cd + L * sc[SC_K0] + K1 * vp + K2 * vc + sqrtf( var ) * z;
+                }
+                else
+                {
+                    step = cd - 0.5f * L * L * vp * sc[SC_DT] + L * sqrtf( vp * sc[SC_DT] ) * z;
+                }
+                S[a] = S[a] * __expf( step );
+            }
+        }
+        observe( i );
+    }
+
+    // ---- book value reduction
+    if ( !valid )
+        book = 0.0;
+    {
+        const double s1 = wave_sum<double>( book );
+        const double s2 = wave_sum<double>( book * book );
+        if ( lane == 0 )
+        {
+            acc[wib][0][0] = s1;
+            acc[wib][0][1] = s2;
+        }
+    }
+    __syncthreads();
+    const int ne = ( 1 + nc ) * 2;
+    if ( (int)threadIdx.x < ne )
+    {
+        const int e = threadIdx.x >> 1, mom = threadIdx.x & 1;
+        double s = 0.0;
+        for ( int w = 0; w < kWavesPerBlock; w++ )
+            s += acc[w][e][mom];
+        B.partial[(size_t)blockIdx.x * ne + threadIdx.x] = s;
+    }
+}
+
+}  // namespace thoth

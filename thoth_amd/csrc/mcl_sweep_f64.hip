@@ -1,0 +1,21 @@
+// mcl_sweep_f64.hip -- FP64 (oracle-matched) instantiations of the path-sweep
+// kernel.  Built with -ffp-contract=off so that every expression rounds as the
+// reference's Release build does (no fused multiply-adds).
+#include "mcl_sweep.hpp"
+#include "mcl_launch.hpp"
+
+namespace thoth
+{
+template <int NA, int D>
+static void launch_one( dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V, uint32_t q0, uint32_t nw,
+                        uint32_t first, uint32_t last )
+{
+    hipLaunchKernelGGL( ( sweep_kernel<double, NA, D> ), grid, dim3( kBlock ), 0, s, books, V, q0, nw, first, last );
+}
+
+bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V,
+                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last )
+{
+    THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, books, V, q0, nw, first, last );
+}
+}  // namespace thoth

@@ -1,0 +1,216 @@
+"""ctypes binding of the gfx950 device library thoth_amd/lib/libthoth_mcl.so
+(C-ABI in include/thoth_mcl_gpu.h and include/thoth_mcl.h).
+
+There is no CPU fallback here: if the library is missing, loading fails
+loudly; if no gfx950 device is present, the entry points return
+THOTH_ERR_UNAVAILABLE and the wrappers raise `Unavailable` -- the caller keeps
+its own CPU engine, as the reference's GPU gate does.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+LIB = Path(__file__).resolve().parent / "lib" / "libthoth_mcl.so"
+_lib = None
+
+
+class McError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("thoth status %d: %s" % (code, msg))
+        self.code = code
+
+
+class Unavailable(McError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError("device library not built (%s): run `python -m thoth_amd.build`" % LIB)
+        L = C.CDLL(str(LIB))
+        P = C.POINTER
+        L.thoth_last_error.restype = C.c_char_p
+        L.thoth_gpu_available.restype = C.c_int
+        L.thoth_gpu_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.thoth_gpu_price_european_gbm.argtypes = [C.c_double] * 5 + [C.c_int, C.c_longlong, C.c_uint64,
+                                                                        P(C.c_double), P(C.c_double)]
+        L.thoth_gpu_sobol_points.argtypes = [C.c_int32, C.c_int64, C.c_int64, P(C.c_uint32)]
+        L.thoth_mcl_sweep.argtypes = [C.c_int32, P(abi.Book), P(abi.SweepCfg), P(abi.BookOut)]
+        L.thoth_mcl_price.argtypes = [P(abi.Request), P(abi.Result)]
+        L.thoth_mcl_prepare.argtypes = [P(abi.Request), P(C.c_void_p)]
+        L.thoth_mcl_run.argtypes = [C.c_void_p, P(abi.Result)]
+        L.thoth_mcl_release.argtypes = [C.c_void_p]
+        L.thoth_mcl_release.restype = None
+        L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
+                                                   P(C.c_double), P(C.c_int64), P(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != abi.OK:
+        msg = lib().thoth_last_error().decode()
+        raise (Unavailable if rc == abi.ERR_UNAVAILABLE else McError)(rc, msg)
+
+
+def available():
+    return bool(lib().thoth_gpu_available())
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    lib().thoth_gpu_device_info(buf, 256)
+    return buf.value.decode()
+
+
+def price_european_gbm(forward, strike, t, vol, df, is_call, paths, seed=0):
+    """Reference gpu::PriceEuropeanGbm contract; returns (premium, trust)."""
+    p, tr = C.c_double(), C.c_double()
+    _check(lib().thoth_gpu_price_european_gbm(forward, strike, t, vol, df, int(is_call), paths, seed,
+                                              C.byref(p), C.byref(tr)))
+    return p.value, tr.value
+
+
+def sobol_points(dim, n, skip=0):
+    out = np.zeros((n, dim), np.uint32)
+    _check(lib().thoth_gpu_sobol_points(dim, skip, n, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
+
+
+def price(built):
+    """Price a `book.BuiltRequest`; returns the filled `abi.Result`."""
+    res = built.result()
+    _check(lib().thoth_mcl_price(C.byref(built.req), C.byref(res)))
+    return res
+
+
+class Plan:
+    """Device-resident base + bump-scenario books of one request
+    (thoth_mcl_prepare / thoth_mcl_run / thoth_mcl_release)."""
+
+    def __init__(self, built):
+        self.built = built
+        self.handle = C.c_void_p()
+        _check(lib().thoth_mcl_prepare(C.byref(built.req), C.byref(self.handle)))
+
+    def run(self):
+        res = self.built.result()
+        _check(lib().thoth_mcl_run(self.handle, C.byref(res)))
+        return res
+
+    def close(self):
+        if self.handle:
+            lib().thoth_mcl_release(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def grid(built, cap=100000):
+    days, t, dt = (np.zeros(cap) for _ in range(3))
+    n = lib().thoth_mcl_grid(C.byref(built.req), cap, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in (days, t, dt)])
+    if n < 0:
+        _check(-n)
+    return days[:n].copy(), t[:n].copy(), dt[:n].copy()
+
+
+def calibrate_leverage(built, asset, spot=None):
+    days, _, _ = grid(built)
+    n = days.size
+    ln_step, offset, levels = np.zeros(n), np.zeros(n, np.int64), np.zeros((n, 201))
+    if spot is None:
+        spot = built.spec.equities[asset].spot
+    k = lib().thoth_mcl_calibrate_leverage(C.byref(built.req), asset, spot, n,
+                                           ln_step.ctypes.data_as(C.POINTER(C.c_double)),
+                                           offset.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           levels.ctypes.data_as(C.POINTER(C.c_double)))
+    if k < 0:
+        _check(-k)
+    return dict(ln_step=ln_step, offset=offset, levels=levels)
+
+
+class BookArrays:
+    """A `thoth_mcl_book` assembled from numpy arrays (kept alive here)."""
+
+    def __init__(self, t, dt, model, spot, drift, disc_rate, contracts, vol=None, heston=None, chol=None,
+                 correlate_stochastic=False, lev=None, rainbows=()):
+        self._keep = []
+        b = abi.Book()
+        na = len(model)
+        nd = len(t)
+
+        def arr(x, dtype, ctype):
+            a = np.ascontiguousarray(x, dtype)
+            self._keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        d = lambda x: arr(x, np.float64, C.c_double)
+        i32 = lambda x: arr(x, np.int32, C.c_int32)
+        b.n_dates = nd
+        b.t, b.dt = d(t), d(dt)
+        b.n_assets = na
+        b.model = i32(model)
+        b.spot = d(spot)
+        b.vol = d(vol if vol is not None else np.zeros(na))
+        h = heston or {}
+        for k in ("v0", "kappa", "theta", "xi", "rho"):
+            setattr(b, k, d(h.get(k, np.zeros(na))))
+        b.drift = d(np.asarray(drift, np.float64).reshape(na * nd))
+        b.chol = d(chol if chol is not None else np.eye(na))
+        b.correlate_stochastic = int(correlate_stochastic)
+        if lev is not None:
+            b.lev_points = lev["levels"].shape[-1]
+            b.lev_ln_step = d(np.asarray(lev["ln_step"]).reshape(-1))
+            b.lev_offset = arr(np.asarray(lev["offset"]).reshape(-1), np.int64, C.c_int64)
+            b.lev_values = d(np.asarray(lev["levels"]).reshape(-1))
+        b.disc_rate = d(disc_rate)
+        b.n_rainbows = len(rainbows)
+        if rainbows:
+            mem = np.zeros((len(rainbows), abi.MAX_ASSETS), np.int32)
+            w = np.zeros((len(rainbows), abi.MAX_ASSETS))
+            for r, (best, members, weights) in enumerate(rainbows):
+                mem[r, :len(members)] = members
+                w[r, :len(members)] = weights
+            b.rb_best = i32([int(r[0]) for r in rainbows])
+            b.rb_n = i32([len(r[1]) for r in rainbows])
+            b.rb_member = i32(mem.reshape(-1))
+            b.rb_weight = d(w.reshape(-1))
+        nc = len(contracts)
+        b.n_contracts = nc
+        get = lambda k, default=0: [c.get(k, default) for c in contracts]
+        b.c_kind = i32(get("kind"))
+        b.c_is_call = i32(get("is_call", 1))
+        b.c_underlying = i32(get("underlying"))
+        b.c_flow_index = i32(get("flow_index"))
+        b.c_strike = d(get("strike"))
+        b.c_is_up = i32(get("is_up"))
+        b.c_is_in = i32(get("is_in"))
+        b.c_level = d(get("level", 0.0))
+        off = [0]
+        mon = []
+        for c in contracts:
+            mon += list(c.get("monitor", []))
+            off.append(len(mon))
+        b.c_mon_offset = i32(off)
+        b.c_mon_index = i32(mon if mon else [0])
+        self.book = b
+
+
+def sweep(books, paths, precision=abi.FP64, sobol_skip=0):
+    """Run thoth_mcl_sweep over a list of BookArrays; returns the BookOut list."""
+    n = len(books)
+    arr = (abi.Book * n)(*[b.book for b in books])
+    cfg = abi.SweepCfg(paths=paths, sobol_skip=sobol_skip, precision=precision, block=0)
+    out = (abi.BookOut * n)()
+    _check(lib().thoth_mcl_sweep(n, arr, C.byref(cfg), out))
+    return list(out)
