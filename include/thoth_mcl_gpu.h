@@ -137,10 +137,18 @@ typedef struct thoth_mcl_book {
 } thoth_mcl_book;
 
 typedef struct thoth_mcl_sweep_cfg {
-    int64_t paths;        /* paths per book */
-    int64_t sobol_skip;   /* leading Sobol points discarded */
+    int64_t paths;        /* paths per book (per replicate in RQMC mode) */
+    int64_t sobol_skip;   /* leading Sobol points discarded (reference mode) */
     int32_t precision;    /* 0 = FP64 (oracle-matched), 1 = FP32 stepping, FP64 accumulation */
     int32_t block;        /* threads per block (0 = library default) */
+    /* Randomised QMC (no reference counterpart; the reference's trust is the
+     * iid formula, monte_carlo_node.cpp:35-40, which is not a valid Sobol CI).
+     * replicates = 0: reference mode, points sobol_skip+1 .. sobol_skip+paths.
+     * replicates = R > 0: R independent random digital shifts (seeded by
+     * rqmc_seed) of points 0 .. paths-1; out then holds n_books * R entries,
+     * out[b * R + r]. */
+    int32_t replicates;
+    uint64_t rqmc_seed;
 } thoth_mcl_sweep_cfg;
 
 /* Kahan-compensated FP64 sums over paths of the discounted book value and of
@@ -153,7 +161,8 @@ typedef struct thoth_mcl_book_out {
     double device_ms;     /* kernel time of this book's sweep */
 } thoth_mcl_book_out;
 
-/* Sweep n_books books over the same Sobol points (common random numbers). */
+/* Sweep n_books books over the same Sobol points (common random numbers).
+ * out holds n_books entries (n_books * replicates in RQMC mode). */
 int thoth_mcl_sweep(int32_t n_books, const thoth_mcl_book* books,
                     const thoth_mcl_sweep_cfg* cfg, thoth_mcl_book_out* out);
 

@@ -283,10 +283,15 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         return SetError( THOTH_ERR_INVALID, "paths must be >= 1 and sobol_skip >= 0" );
     if ( cfg->precision != 0 && cfg->precision != 1 )
         return SetError( THOTH_ERR_INVALID, "precision must be 0 (FP64) or 1 (FP32)" );
-    const uint64_t first = (uint64_t)cfg->sobol_skip + 1;
-    const uint64_t last = (uint64_t)cfg->sobol_skip + (uint64_t)cfg->paths;
+    if ( cfg->replicates < 0 || cfg->replicates > 4096 )
+        return SetError( THOTH_ERR_INVALID, "replicates must be in [0, 4096]" );
+    // reference mode: points skip+1 .. skip+paths; RQMC mode: points 0 .. paths-1
+    const bool rqmc = cfg->replicates > 0;
+    const uint64_t first = rqmc ? 0 : (uint64_t)cfg->sobol_skip + 1;
+    const uint64_t last = rqmc ? (uint64_t)cfg->paths - 1 : (uint64_t)cfg->sobol_skip + (uint64_t)cfg->paths;
     if ( last >= 0xffffffffull )
         return SetError( THOTH_ERR_UNSUPPORTED, "Sobol index beyond 2^32 - 2" );
+    B.replicates = rqmc ? cfg->replicates : 1;
     B.first = (uint32_t)first;
     B.last = (uint32_t)last;
     B.q0 = (uint32_t)( first / 64 );
@@ -313,8 +318,38 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     for ( int b = 0; b < n_books; b++ )
     {
         B.part_off[b] = B.part_total;
-        B.part_total += (size_t)B.nblocks * ( 1 + B.db[b].n_contracts ) * 2;
+        B.part_total += (size_t)B.replicates * B.nblocks * ( 1 + B.db[b].n_contracts ) * 2;
     }
+    // random digital shifts: one 32-bit word per (replicate, dimension) from
+    // xoshiro256++ seeded by splitmix64(rqmc_seed) (upper 32 bits of each draw)
+    std::vector<uint32_t> shifts;
+    if ( rqmc )
+    {
+        shifts.resize( (size_t)B.replicates * kSobolDims );
+        uint64_t z = cfg->rqmc_seed, st[4];
+        for ( int i = 0; i < 4; i++ )
+        {
+            z += 0x9e3779b97f4a7c15ULL;
+            uint64_t x = z;
+            x = ( x ^ ( x >> 30 ) ) * 0xbf58476d1ce4e5b9ULL;
+            x = ( x ^ ( x >> 27 ) ) * 0x94d049bb133111ebULL;
+            st[i] = x ^ ( x >> 31 );
+        }
+        auto rotl = []( uint64_t x, int k ) { return ( x << k ) | ( x >> ( 64 - k ) ); };
+        for ( uint32_t& w : shifts )
+        {
+            const uint64_t r = rotl( st[0] + st[3], 23 ) + st[0];
+            const uint64_t t = st[1] << 17;
+            st[2] ^= st[0];
+            st[3] ^= st[1];
+            st[1] ^= st[2];
+            st[0] ^= st[3];
+            st[2] ^= t;
+            st[3] = rotl( st[3], 45 );
+            w = (uint32_t)( r >> 32 );
+        }
+    }
+    const size_t shift_off = ar.put( shifts );
     const size_t arena_bytes = ( ar.data.size() + 255 ) & ~size_t( 255 );
     const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
     const size_t total = arena_bytes + books_bytes + B.part_total * sizeof( double );
@@ -352,6 +387,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         d.partial = B.dpart + B.part_off[b];
     }
     B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
+    B.dshifts = rqmc ? At<uint32_t>( dbase, shift_off ) : nullptr;
     hipStream_t s = Stream();
 #define HIP_OK( expr )                                                                                      \
     do                                                                                                      \
@@ -373,12 +409,12 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
     HIP_OK( hipEventCreate( &e0 ) );
     HIP_OK( hipEventCreate( &e1 ) );
     HIP_OK( hipEventRecord( e0, s ) );
-    const dim3 grid( B.nblocks, B.n_books );
+    const dim3 grid( B.nblocks, B.n_books, B.replicates );
     bool ok = B.precision == 0
                   ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.dbooks, DeviceDirections(), B.q0, B.nwaves, B.first,
-                                      B.last )
+                                      B.last, B.dshifts )
                   : launch_sweep_f32( B.na_max, B.depth, grid, s, B.dbooks, DeviceDirections(), B.q0, B.nwaves, B.first,
-                                      B.last );
+                                      B.last, B.dshifts );
     if ( !ok )
     {
         (void)hipEventDestroy( e0 );
@@ -395,11 +431,12 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
     for ( int b = 0; b < B.n_books; b++ )
+      for ( int r = 0; r < B.replicates; r++ )
     {
         const int nc = B.db[b].n_contracts;
         const int ne = 1 + nc;
         std::vector<double> sum( ne * 2, 0.0 ), comp( ne * 2, 0.0 );
-        const double* pb = part.data() + B.part_off[b];
+        const double* pb = part.data() + B.part_off[b] + (size_t)r * B.nblocks * ne * 2;
         for ( uint32_t k = 0; k < B.nblocks; k++ )
             for ( int j = 0; j < ne * 2; j++ )
             {
@@ -409,7 +446,7 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
                 comp[j] = ( t - sum[j] ) - y;
                 sum[j] = t;
             }
-        thoth_mcl_book_out& o = out[b];
+        thoth_mcl_book_out& o = out[(size_t)b * B.replicates + r];
         std::memset( &o, 0, sizeof( o ) );
         o.book_sum = sum[0];
         o.book_sum2 = sum[1];

@@ -11,6 +11,7 @@ constexpr int kMaxContracts = 32;
 constexpr int kMaxRainbows = 4;
 constexpr int kWavesPerBlock = 4;          // 256-thread blocks
 constexpr int kBlock = 64 * kWavesPerBlock;
+constexpr int kSobolDimsDev = 4097;       // implicit first dimension + 4096 Joe-Kuo rows
 
 // FP32 per-(equity, date) step constants, precomputed on the host in FP64
 // from the reference's QE / Andersen formulas (heston_variance_node.cpp:29-33,
@@ -78,7 +79,7 @@ struct DevBook
     const uint32_t* flow_mask;  // [n_dates] contracts paying at the date
     int rb_best[kMaxRainbows], rb_n[kMaxRainbows], rb_member[kMaxRainbows][kMaxAssets];
     double rb_weight[kMaxRainbows][kMaxAssets];
-    double* partial;            // [n_blocks][1 + n_contracts][2] per-block sums
+    double* partial;            // [replicates][n_blocks][1 + n_contracts][2] per-block sums
 };
 
 }  // namespace thoth

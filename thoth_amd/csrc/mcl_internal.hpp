@@ -39,6 +39,8 @@ struct DeviceBatch
     char* dbase = nullptr;
     double* dpart = nullptr;
     const DevBook* dbooks = nullptr;
+    const uint32_t* dshifts = nullptr;  // [replicates][kSobolDims] in RQMC mode
+    int replicates = 1;
     std::vector<DevBook> db;
     std::vector<size_t> part_off;
     size_t part_total = 0;

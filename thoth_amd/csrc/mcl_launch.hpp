@@ -38,8 +38,8 @@ namespace thoth
 {
 constexpr int kMaxStackDepth = 16;
 bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V,
-                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last );
+                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last, const uint32_t* shifts );
 bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V,
-                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last );
+                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last, const uint32_t* shifts );
 bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t first, uint32_t last, uint32_t* out );
 }  // namespace thoth

@@ -24,10 +24,11 @@ __device__ __forceinline__ uint32_t gray32( uint32_t n ) { return n ^ ( n >> 1 )
 // Integer Sobol coordinate of dimension d at index (wave block, lane).
 // Linear in the Gray-code bits over GF(2): reference sequential recurrence
 // (sobol_generator.cpp:80-100) gives x = XOR_{k in gray(n)} V[d][k].
-__device__ __forceinline__ uint32_t sobol_x( const uint32_t* __restrict__ V, int d, uint32_t G, uint32_t g )
+__device__ __forceinline__ uint32_t sobol_x( const uint32_t* __restrict__ V, int d, uint32_t G, uint32_t g,
+                                             const uint32_t* __restrict__ shift = nullptr )
 {
     const uint32_t* v = V + (size_t)d * 32;
-    uint32_t x = 0;
+    uint32_t x = shift ? shift[d] : 0u;  // random digital shift (RQMC), wave-uniform
     uint32_t gg = G;
     while ( gg )
     {
@@ -167,7 +168,8 @@ __device__ __forceinline__ R wave_sum( R x )
 template <class R, int NA, int DEPTH>
 __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restrict__ books,
                                                          const uint32_t* __restrict__ V, uint32_t q0,
-                                                         uint32_t nwaves, uint32_t first_idx, uint32_t last_idx )
+                                                         uint32_t nwaves, uint32_t first_idx, uint32_t last_idx,
+                                                         const uint32_t* __restrict__ shifts )
 {
     constexpr bool F64 = std::is_same_v<R, double>;
     constexpr int NF = 2 * NA;
@@ -186,6 +188,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     const bool valid = wave < nwaves && idx >= first_idx && idx <= last_idx;
     const uint32_t G = __builtin_amdgcn_readfirstlane( gray32( q * 64u ) );
     const uint32_t g = gray32( (uint32_t)lane );
+    const uint32_t* sh = shifts ? shifts + (size_t)blockIdx.z * kSobolDimsDev : nullptr;
 
     const int F = B.factors;
     const int m = B.m;
@@ -209,7 +212,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
         top[f] = R( 0 );
         if ( f < F )
         {
-            const R z = normal_of<R>( sobol_x( V, f, G, g ) );
+            const R z = normal_of<R>( sobol_x( V, f, G, g, sh ) );
             if constexpr ( F64 )
                 top[f] = B.sd0 * z;
             else
@@ -326,7 +329,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
             {
                 if ( f < F )
                 {
-                    const R z = normal_of<R>( sobol_x( V, bidx * F + f, G, g ) );
+                    const R z = normal_of<R>( sobol_x( V, bidx * F + f, G, g, sh ) );
                     // path_generator.cpp:175-182
                     const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
                     stk[f][pos] = top[f];
@@ -521,7 +524,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
         double s = 0.0;
         for ( int w = 0; w < kWavesPerBlock; w++ )
             s += acc[w][e][mom];
-        B.partial[(size_t)blockIdx.x * ne + threadIdx.x] = s;
+        B.partial[( (size_t)blockIdx.z * gridDim.x + blockIdx.x ) * ne + threadIdx.x] = s;
     }
 }
 

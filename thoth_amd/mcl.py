@@ -206,11 +206,13 @@ class BookArrays:
         self.book = b
 
 
-def sweep(books, paths, precision=abi.FP64, sobol_skip=0):
-    """Run thoth_mcl_sweep over a list of BookArrays; returns the BookOut list."""
+def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_seed=0):
+    """Run thoth_mcl_sweep over a list of BookArrays; returns the BookOut list
+    (len(books) * replicates entries in RQMC mode, book-major)."""
     n = len(books)
     arr = (abi.Book * n)(*[b.book for b in books])
-    cfg = abi.SweepCfg(paths=paths, sobol_skip=sobol_skip, precision=precision, block=0)
-    out = (abi.BookOut * n)()
+    cfg = abi.SweepCfg(paths=paths, sobol_skip=sobol_skip, precision=precision, block=0,
+                       replicates=replicates, rqmc_seed=rqmc_seed)
+    out = (abi.BookOut * (n * max(1, replicates)))()
     _check(lib().thoth_mcl_sweep(n, arr, C.byref(cfg), out))
     return list(out)
