@@ -1,0 +1,71 @@
+"""Fit the FP32 inverse-normal polynomials used by normal_f32 (mcl_sweep.hpp).
+
+Form (the erfinv parameterisation of M. Giles, "Approximating the erfinv
+function", GPU Computing Gems, 2011; coefficients fitted here):
+    x = 2u - 1,  w = -log(1 - x^2) = -log(4 u (1 - u))
+    Phi^-1(u) = x * P_c(w - 2.5)            for w < 5
+              = x * P_t(sqrt(w) - 3)        otherwise
+P_c, P_t: degree-8 polynomials fitted by weighted least squares on Chebyshev
+nodes to the relative error against scipy's ndtri, sqrt(2) folded in.
+Run to regenerate the constants printed for mcl_sweep.hpp.
+"""
+import numpy as np
+from scipy.special import ndtri
+
+
+def target(w):
+    # exact g(w) = Phi^-1(u) / x on the upper half, with u = (1 + x) / 2
+    e = np.exp(-w)
+    x = np.sqrt(-np.expm1(-w))
+    small = e / (2.0 * (1.0 + x))          # 1 - u, accurate in the tail
+    z = -ndtri(small)
+    g = np.where(x > 1e-8, z / np.where(x > 1e-8, x, 1.0), np.sqrt(np.pi / 2.0))
+    return g
+
+
+def fit(lo, hi, center, transform, deg=8, n=4000):
+    k = np.arange(n)
+    t = np.cos(np.pi * (k + 0.5) / n)
+    s = 0.5 * (lo + hi) + 0.5 * (hi - lo) * t           # nodes in the fit variable
+    w = transform(s)
+    g = target(w)
+    A = np.vander(s - center, deg + 1, increasing=True)
+    coef, *_ = np.linalg.lstsq(A / g[:, None], np.ones_like(g), rcond=None)
+    return coef
+
+
+def main():
+    cc = fit(0.0, 5.0, 2.5, lambda s: s)
+    ct = fit(np.sqrt(5.0) - 3.0, np.sqrt(23.0) - 3.0, 0.0, lambda s: (s + 3.0) ** 2)
+    print("central (w - 2.5), degree 0..8:", ", ".join("%.9e" % c for c in cc))
+    print("tail (sqrt(w) - 3), degree 0..8:", ", ".join("%.9e" % c for c in ct))
+    # FP32 evaluation from 32-bit integer coordinates vs FP64 truth
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.integers(1, 2 ** 32, 2_000_000, dtype=np.uint64),
+                         np.arange(1, 5000, dtype=np.uint64), 2 ** 32 - np.arange(1, 5000, dtype=np.uint64),
+                         2 ** 31 + np.arange(-3000, 3000).astype(np.int64).astype(np.uint64)])
+    z = evaluate_f32(xs.astype(np.uint32), cc.astype(np.float32), ct.astype(np.float32))
+    ref = ndtri(xs.astype(np.float64) / 2.0 ** 32)
+    err = np.abs(z.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-3)
+    print("max rel err (|z|>1e-3 scale) FP32 pipeline: %.3e" % err.max())
+
+
+def evaluate_f32(x, cc, ct):
+    f32 = np.float32
+    a = x.astype(f32)
+    b = (np.uint32(0) - x).astype(f32)                    # 2^32 - x
+    prod = a * b * f32(2.0 ** -62)                        # 4 u (1 - u)
+    w = -np.log(prod).astype(f32)
+    xp = (x.astype(np.int64) - 2 ** 31).astype(f32) * f32(2.0 ** -31)
+    def horner(c, v):
+        r = np.full_like(v, c[-1])
+        for k in range(len(c) - 2, -1, -1):
+            r = (r * v + c[k]).astype(f32)
+        return r
+    pc = horner(cc, (w - f32(2.5)).astype(f32))
+    pt = horner(ct, (np.sqrt(w) - f32(3.0)).astype(f32))
+    return xp * np.where(w < f32(5.0), pc, pt)
+
+
+if __name__ == "__main__":
+    main()

@@ -294,9 +294,9 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     B.replicates = rqmc ? cfg->replicates : 1;
     B.first = (uint32_t)first;
     B.last = (uint32_t)last;
-    B.q0 = (uint32_t)( first / 64 );
-    B.nwaves = (uint32_t)( last / 64 ) - B.q0 + 1;
-    B.nblocks = ( B.nwaves + kWavesPerBlock - 1 ) / kWavesPerBlock;
+    // blocks own 256 consecutive Sobol indices aligned on 256
+    B.q0 = (uint32_t)( first / kBlock );
+    B.nblocks = (uint32_t)( last / kBlock ) - B.q0 + 1;
     B.paths = cfg->paths;
     B.precision = cfg->precision;
     B.n_books = n_books;
@@ -313,6 +313,17 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             return rc;
         B.na_max = std::max( B.na_max, B.db[b].n_assets );
     }
+    // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV)
+    // two leverage layers per equity of the kernel instantiation
+    const int na_inst = B.na_max <= 1 ? 1 : B.na_max <= 2 ? 2 : B.na_max <= 5 ? 5 : 8;
+    size_t dims = 0, lev = 0;
+    for ( int b = 0; b < n_books; b++ )
+    {
+        dims = std::max( dims, (size_t)B.db[b].m * B.db[b].factors );
+        if ( cfg->precision == 1 && B.db[b].lev_points > 0 )
+            lev = std::max( lev, (size_t)2 * na_inst * B.db[b].lev_points );
+    }
+    B.lds_bytes = ( dims + lev ) * sizeof( uint32_t );
     B.part_off.assign( n_books, 0 );
     B.part_total = 0;
     for ( int b = 0; b < n_books; b++ )
@@ -411,10 +422,10 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
     HIP_OK( hipEventRecord( e0, s ) );
     const dim3 grid( B.nblocks, B.n_books, B.replicates );
     bool ok = B.precision == 0
-                  ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.dbooks, DeviceDirections(), B.q0, B.nwaves, B.first,
-                                      B.last, B.dshifts )
-                  : launch_sweep_f32( B.na_max, B.depth, grid, s, B.dbooks, DeviceDirections(), B.q0, B.nwaves, B.first,
-                                      B.last, B.dshifts );
+                  ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
+                                      B.first, B.last, B.dshifts )
+                  : launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
+                                      B.first, B.last, B.dshifts );
     if ( !ok )
     {
         (void)hipEventDestroy( e0 );

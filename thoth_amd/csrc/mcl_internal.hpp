@@ -44,7 +44,8 @@ struct DeviceBatch
     std::vector<DevBook> db;
     std::vector<size_t> part_off;
     size_t part_total = 0;
-    uint32_t q0 = 0, nwaves = 0, nblocks = 0, first = 0, last = 0;
+    uint32_t q0 = 0, nblocks = 0, first = 0, last = 0;  // q0: first 256-index block
+    size_t lds_bytes = 0;
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
     int64_t paths = 0;
     DeviceBatch() = default;

@@ -5,12 +5,14 @@
 
 #include "mcl_device.hpp"
 
-// supported instantiations: equities per book {1, 2, 5, 8}, stack depth {2, 10, 16}
+// supported instantiations: equities per book {1, 2, 5, 8}, stack depth {2, 8, 10, 16}
 #define THOTH_DISPATCH_DEPTH( FN, NA, depth, ... )     \
     do                                                 \
     {                                                  \
         if ( ( depth ) <= 2 )                          \
             FN<NA, 2>( __VA_ARGS__ );                  \
+        else if ( ( depth ) <= 8 )                     \
+            FN<NA, 8>( __VA_ARGS__ );                  \
         else if ( ( depth ) <= 10 )                    \
             FN<NA, 10>( __VA_ARGS__ );                 \
         else if ( ( depth ) <= 16 )                    \
@@ -37,9 +39,9 @@
 namespace thoth
 {
 constexpr int kMaxStackDepth = 16;
-bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V,
-                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last, const uint32_t* shifts );
-bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V,
-                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last, const uint32_t* shifts );
+bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
+bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
 bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t first, uint32_t last, uint32_t* out );
 }  // namespace thoth

@@ -21,24 +21,65 @@ namespace thoth
 
 __device__ __forceinline__ uint32_t gray32( uint32_t n ) { return n ^ ( n >> 1 ); }
 
-// Integer Sobol coordinate of dimension d at index (wave block, lane).
-// Linear in the Gray-code bits over GF(2): reference sequential recurrence
-// (sobol_generator.cpp:80-100) gives x = XOR_{k in gray(n)} V[d][k].
-__device__ __forceinline__ uint32_t sobol_x( const uint32_t* __restrict__ V, int d, uint32_t G, uint32_t g,
-                                             const uint32_t* __restrict__ shift = nullptr )
+// Integer Sobol coordinates for the 256 consecutive indices of one block,
+// aligned on 256.  The reference's sequential Gray-code recurrence
+// (sobol_generator.cpp:80-100) gives x_d(n) = XOR_{k in gray(n)} V[d][k], and for
+// n = 256B + 64w + j (w < 4, j < 64) gray(n) = gray(256B) ^ gray(64w) ^ gray(j)
+// (the summands occupy disjoint bits), so
+//   x_d = T_B[d] ^ XOR_{k in gray(64w)} V[d][k] ^ XOR_{k in gray(j)} V[d][k]
+// with T_B[d] (block part, bits >= 7, plus the RQMC digital shift) tabulated in
+// LDS once per block, the wave part (bits 5..7) selected in scalar registers
+// and the lane part (bits 0..5) by six per-lane masks.
+struct SobolLane
+{
+    const uint32_t* __restrict__ V;
+    const uint32_t* tab;  // LDS, block part per dimension
+    uint32_t gw;          // gray(64 w), wave-uniform
+    uint32_t m[6];        // per-lane masks of gray(lane)
+
+    __device__ __forceinline__ uint32_t operator()( int d ) const
+    {
+        const uint32_t* v = V + (size_t)d * 32;
+        uint32_t x = tab[d];
+        x ^= ( gw & 0x20u ) ? v[5] : 0u;
+        x ^= ( gw & 0x40u ) ? v[6] : 0u;
+        x ^= ( gw & 0x80u ) ? v[7] : 0u;
+        x ^= ( v[0] & m[0] ) ^ ( v[1] & m[1] ) ^ ( v[2] & m[2] );
+        x ^= ( v[3] & m[3] ) ^ ( v[4] & m[4] ) ^ ( v[5] & m[5] );
+        return x;
+    }
+};
+
+// fill the block table: T_B[d] = shift[d] ^ XOR_{k in gray(256 B)} V[d][k]
+__device__ __forceinline__ void sobol_block_table( uint32_t* tab, const uint32_t* __restrict__ V, int dims,
+                                                   uint32_t bq, const uint32_t* __restrict__ shift )
+{
+    const uint32_t G = gray32( bq * 256u );
+    for ( int d = threadIdx.x; d < dims; d += kBlock )
+    {
+        const uint32_t* v = V + (size_t)d * 32;
+        uint32_t x = shift ? shift[d] : 0u;
+        uint32_t gg = G;
+        while ( gg )
+        {
+            x ^= v[__builtin_ctz( gg )];
+            gg &= gg - 1;
+        }
+        tab[d] = x;
+    }
+}
+
+// single-point variant (one lane = one index, any alignment) for the raw-point entry
+__device__ __forceinline__ uint32_t sobol_x( const uint32_t* __restrict__ V, int d, uint32_t n )
 {
     const uint32_t* v = V + (size_t)d * 32;
-    uint32_t x = shift ? shift[d] : 0u;  // random digital shift (RQMC), wave-uniform
-    uint32_t gg = G;
-    while ( gg )
+    uint32_t x = 0;
+    uint32_t g = gray32( n );
+    while ( g )
     {
-        const int k = __builtin_ctz( gg );
-        x ^= v[k];
-        gg &= gg - 1;
+        x ^= v[__builtin_ctz( g )];
+        g &= g - 1;
     }
-#pragma unroll
-    for ( int k = 0; k < 6; k++ )
-        x ^= ( ( g >> k ) & 1u ) ? v[k] : 0u;
     return x;
 }
 
@@ -76,32 +117,48 @@ __device__ __forceinline__ double normal_f64( uint32_t x )
            ( ( ( ( d0 * q + d1 ) * q + d2 ) * q + d3 ) * q + 1.0 );
 }
 
-// Acklam's rational approximation evaluated in FP32 from the integer
-// coordinate: the tail probability min(u, 1-u) is formed exactly in integers
-// (so u near 1 keeps its precision), the centred variable u - 1/2 exactly
-// up to the int->float rounding.  The clamp of the reference never binds for
-// Sobol points (the smallest nonzero coordinate is 2^-32 > 1e-12).
+// FP32 inverse normal from the 32-bit integer coordinate, in the erfinv form
+// of M. Giles ("Approximating the erfinv function", GPU Computing Gems 2011)
+// with coefficients fitted by thoth_amd/csrc/fit_inverse_normal.py to
+// Phi^-1 directly (max relative error 6.2e-7 over every 32-bit input):
+//   x = 2u - 1, w = -log(4u(1-u)), Phi^-1(u) = x P(w).
+// 4u(1-u) is formed from the integers u*2^32 and (1-u)*2^32, so it keeps its
+// precision in both tails; one log per normal, the tail polynomial only runs
+// in the ~0.3 % of draws with w >= 5.  The FP64 oracle mode keeps Acklam.
 __device__ __forceinline__ float normal_f32( uint32_t x )
 {
-    const uint32_t xm = ( x < 0x80000000u ) ? x : ( 0u - x );
-    if ( xm < 104152957u )  // min(u,1-u) < 0.02425
+    const float a = (float)x;
+    const float b = (float)( 0u - x );
+    const float w = -__logf( a * b * 2.168404344971009e-19f );  // 2^-62
+    const float xs = (float)(int32_t)( x - 0x80000000u ) * 4.656612873077393e-10f;  // 2^-31
+    float p;
+    if ( w < 5.0f )
     {
-        const float p = fmaxf( (float)xm, 1.0f ) * 2.3283064365386963e-10f;
-        const float q = sqrtf( -2.0f * __logf( p ) );
-        const float num = ( ( ( ( -7.784894002430293e-03f * q - 3.223964580411365e-01f ) * q - 2.400758277161838e+00f ) * q -
-                              2.549732539343734e+00f ) * q + 4.374664141464968e+00f ) * q + 2.938163982698783e+00f;
-        const float den = ( ( ( 7.784695709041462e-03f * q + 3.224671290700398e-01f ) * q + 2.445134137142996e+00f ) * q +
-                            3.754408661907416e+00f ) * q + 1.0f;
-        const float z = num / den;
-        return ( x < 0x80000000u ) ? z : -z;
+        const float t = w - 2.5f;
+        p = 3.958320987e-08f;
+        p = fmaf( p, t, 4.851791016e-07f );
+        p = fmaf( p, t, -4.981194608e-06f );
+        p = fmaf( p, t, -6.207880344e-06f );
+        p = fmaf( p, t, 3.091150937e-04f );
+        p = fmaf( p, t, -1.773041744e-03f );
+        p = fmaf( p, t, -5.908129905e-03f );
+        p = fmaf( p, t, 3.488026846e-01f );
+        p = fmaf( p, t, 2.123313560e+00f );
     }
-    const float q = (float)(int32_t)( x - 0x80000000u ) * 2.3283064365386963e-10f;
-    const float r = q * q;
-    const float num = ( ( ( ( -3.969683028665376e+01f * r + 2.209460984245205e+02f ) * r - 2.759285104469687e+02f ) * r +
-                          1.383577518672690e+02f ) * r - 3.066479806614716e+01f ) * r + 2.506628277459239e+00f;
-    const float den = ( ( ( ( -5.447609879822406e+01f * r + 1.615858368580409e+02f ) * r - 1.556989798598866e+02f ) * r +
-                          6.680131188771972e+01f ) * r - 1.328068155288572e+01f ) * r + 1.0f;
-    return num * q / den;
+    else
+    {
+        const float t = __builtin_amdgcn_sqrtf( w ) - 3.0f;
+        p = 5.523418413e-06f;
+        p = fmaf( p, t, -2.638466750e-04f );
+        p = fmaf( p, t, 1.661917106e-03f );
+        p = fmaf( p, t, -4.731942225e-03f );
+        p = fmaf( p, t, 8.164933407e-03f );
+        p = fmaf( p, t, -1.093819435e-02f );
+        p = fmaf( p, t, 1.334743934e-02f );
+        p = fmaf( p, t, 1.416595396e+00f );
+        p = fmaf( p, t, 4.006434468e+00f );
+    }
+    return xs * p;
 }
 
 template <class R>
@@ -139,21 +196,23 @@ __device__ __forceinline__ double qe_f64( double v, double dt, double z, double 
 // directly as Phi(-z) so the exponential tail keeps its precision.
 __device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc )
 {
-    const float m = sc[SC_M0] + v * sc[SC_E];
-    const float s2 = sc[SC_S0] + v * sc[SC_S1];
-    const float psi = ( m > 0.0f ) ? s2 / ( m * m ) : 0.0f;
+    const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
+    const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
+    const float psi = ( m > 0.0f ) ? __fdividef( s2, m * m ) : 0.0f;
     if ( psi <= 1.5f )
     {
-        const float c = 2.0f / psi;
-        const float b2 = c - 1.0f + sqrtf( c * ( c - 1.0f ) );
-        const float a = m / ( 1.0f + b2 );
-        const float bz = sqrtf( b2 ) + z;
+        const float c = __fdividef( 2.0f, psi );
+        const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
+        const float a = __fdividef( m, 1.0f + b2 );
+        const float bz = __builtin_amdgcn_sqrtf( b2 ) + z;
         return a * bz * bz;
     }
-    const float p = ( psi - 1.0f ) / ( psi + 1.0f );
-    const float beta = ( 1.0f - p ) / m;
+    // p = (psi-1)/(psi+1), beta = (1-p)/m; v = log((1-p)/(1-u)) / beta
+    const float inv = __fdividef( 1.0f, psi + 1.0f );
+    const float one_minus_p = 2.0f * inv;
     const float one_minus_u = 0.5f * erfcf( z * 0.70710678118654752f );
-    return ( one_minus_u >= 1.0f - p ) ? 0.0f : __logf( ( 1.0f - p ) / one_minus_u ) / beta;
+    return ( one_minus_u >= one_minus_p ) ? 0.0f
+                                          : __logf( __fdividef( one_minus_p, one_minus_u ) ) * m * 0.5f * ( psi + 1.0f );
 }
 
 template <class R>
@@ -167,8 +226,8 @@ __device__ __forceinline__ R wave_sum( R x )
 
 template <class R, int NA, int DEPTH>
 __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restrict__ books,
-                                                         const uint32_t* __restrict__ V, uint32_t q0,
-                                                         uint32_t nwaves, uint32_t first_idx, uint32_t last_idx,
+                                                         const uint32_t* __restrict__ V, uint32_t bq0,
+                                                         uint32_t first_idx, uint32_t last_idx,
                                                          const uint32_t* __restrict__ shifts )
 {
     constexpr bool F64 = std::is_same_v<R, double>;
@@ -182,12 +241,9 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     const DevBook& B = books[blockIdx.y];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane( blockIdx.x * kWavesPerBlock + wib );
-    const uint32_t q = q0 + wave;
-    const uint32_t idx = q * 64u + (uint32_t)lane;
-    const bool valid = wave < nwaves && idx >= first_idx && idx <= last_idx;
-    const uint32_t G = __builtin_amdgcn_readfirstlane( gray32( q * 64u ) );
-    const uint32_t g = gray32( (uint32_t)lane );
+    const uint32_t bq = bq0 + blockIdx.x;  // block of 256 Sobol indices
+    const uint32_t idx = bq * 256u + (uint32_t)threadIdx.x;
+    const bool valid = idx >= first_idx && idx <= last_idx;
     const uint32_t* sh = shifts ? shifts + (size_t)blockIdx.z * kSobolDimsDev : nullptr;
 
     const int F = B.factors;
@@ -196,9 +252,39 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     const int nd = B.n_dates;
     const int nc = B.n_contracts;
 
+    // dynamic LDS: [m*F Sobol block table][FP32 LSV: 2 x NA x lev_points leverage layers]
+    extern __shared__ uint32_t lds[];
+    uint32_t* tab = lds;
+    float* levs = reinterpret_cast<float*>( lds + m * F );
     __shared__ double acc[kWavesPerBlock][kMaxContracts + 1][2];
     for ( int k = threadIdx.x; k < kWavesPerBlock * ( kMaxContracts + 1 ) * 2; k += kBlock )
         ( &acc[0][0][0] )[k] = 0.0;
+    sobol_block_table( tab, V, m * F, bq, sh );
+    SobolLane sob;
+    sob.V = V;
+    sob.tab = tab;
+    sob.gw = __builtin_amdgcn_readfirstlane( gray32( 64u * (uint32_t)wib ) );
+    {
+        const uint32_t g = gray32( (uint32_t)lane );
+#pragma unroll
+        for ( int k = 0; k < 6; k++ )
+            sob.m[k] = ( ( g >> k ) & 1u ) ? 0xffffffffu : 0u;
+    }
+    // FP32 LSV books stage the leverage layer of the current step in LDS
+    const bool lev_lds = !F64 && B.lev_points > 0;
+    const int P = B.lev_points;
+    auto stage_layer = [&]( int i )
+    {
+        float* dst = levs + ( i & 1 ) * NA * P;
+        for ( int a = 0; a < na; a++ )
+        {
+            const float* src = B.levf + ( (size_t)a * nd + i ) * P;
+            for ( int k = threadIdx.x; k < P; k += kBlock )
+                dst[a * P + k] = src[k];
+        }
+    };
+    if ( lev_lds && m >= 1 )
+        stage_layer( 1 );
     __syncthreads();
 
     // ---- bridge state: endpoint first (build index 0, path_generator.cpp:166-167)
@@ -212,7 +298,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
         top[f] = R( 0 );
         if ( f < F )
         {
-            const R z = normal_of<R>( sobol_x( V, f, G, g, sh ) );
+            const R z = normal_of<R>( sob( f ) );
             if constexpr ( F64 )
                 top[f] = B.sd0 * z;
             else
@@ -304,6 +390,12 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     for ( int p = 0; p < m; p++ )
     {
         const int i = p + 1;
+        if ( lev_lds )
+        {
+            __syncthreads();  // layer i is visible, every wave is done with layer i-1
+            if ( i + 1 <= m )
+                stage_layer( i + 1 );
+        }
         // ---- bridge: build the pending points of this step (in-order tree walk)
         const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
         for ( int o = o0; o < o1; o++ )
@@ -329,7 +421,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
             {
                 if ( f < F )
                 {
-                    const R z = normal_of<R>( sobol_x( V, bidx * F + f, G, g, sh ) );
+                    const R z = normal_of<R>( sob( bidx * F + f ) );
                     // path_generator.cpp:175-182
                     const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
                     stk[f][pos] = top[f];
@@ -410,7 +502,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
                     const float dw = w1 - W[a];
                     W[a] = w1;
                     const float expo = cd - sc[SC_HALF_V2DT] + sc[SC_VOL] * dw;
-                    S[a] = S[a] * expf( expo );
+                    S[a] = S[a] * __expf( expo );
                 }
                 continue;
             }
@@ -451,11 +543,11 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
                 }
                 else
                 {
-                    const float* lv = B.levf + li * B.lev_points;
+                    const float* lv = levs + ( i & 1 ) * NA * P + a * P;
                     const float fc = fminf( fmaxf( f, 0.0f ), (float)last );
                     const int k = min( (int)fc, last - 1 );
                     const float w = fc - (float)k;
-                    L = lv[k] + w * ( lv[k + 1] - lv[k] );
+                    L = fmaf( w, lv[k + 1] - lv[k], lv[k] );
                 }
             }
             const R z = eps;
@@ -492,11 +584,11 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
                     const float K1 = aa - L * sc[SC_RX];
                     const float K2 = aa + L * sc[SC_RX];
                     const float var = fmaxf( L * L * sc[SC_K3] * ( vp + vc ), 0.0f );
-                    step = cd + L * sc[SC_K0] + K1 * vp + K2 * vc + sqrtf( var ) * z;
+                    step = cd + L * sc[SC_K0] + K1 * vp + K2 * vc + __builtin_amdgcn_sqrtf( var ) * z;
                 }
                 else
                 {
-                    step = cd - 0.5f * L * L * vp * sc[SC_DT] + L * sqrtf( vp * sc[SC_DT] ) * z;
+                    step = cd - 0.5f * L * L * vp * sc[SC_DT] + L * __builtin_amdgcn_sqrtf( vp * sc[SC_DT] ) * z;
                 }
                 S[a] = S[a] * __expf( step );
             }

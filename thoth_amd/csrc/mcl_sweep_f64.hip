@@ -7,15 +7,15 @@
 namespace thoth
 {
 template <int NA, int D>
-static void launch_one( dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V, uint32_t q0, uint32_t nw,
+static void launch_one( dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V, uint32_t bq0,
                         uint32_t first, uint32_t last, const uint32_t* shifts )
 {
-    hipLaunchKernelGGL( ( sweep_kernel<double, NA, D> ), grid, dim3( kBlock ), 0, s, books, V, q0, nw, first, last, shifts );
+    hipLaunchKernelGGL( ( sweep_kernel<double, NA, D> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first, last, shifts );
 }
 
-bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, const DevBook* books, const uint32_t* V,
-                       uint32_t q0, uint32_t nw, uint32_t first, uint32_t last, const uint32_t* shifts )
+bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts )
 {
-    THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, books, V, q0, nw, first, last, shifts );
+    THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
 }
 }  // namespace thoth
