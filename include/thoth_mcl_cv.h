@@ -1,0 +1,90 @@
+/*
+ * thoth_mcl_cv.h -- learned martingale control variate (no reference
+ * counterpart: the reference engine has no control variates, SURVEY of
+ * pricer/src/tasks/pricer_mcl*.cpp).
+ *
+ * A small MLP f(t, log S, sqrt v) -> 5 hedge ratios is trained on paths the
+ * engine itself generates (FP32 sweep of the book) and then used as the
+ * control variate
+ *     CV = sum_j f(x_j) . sum_{i in interval j} df(t_i) (100/S0) M_i,
+ *     M_i = S_{i-1} e^{det_i} (e^{b_i z_i} - e^{b_i^2/2}),
+ * which has zero mean under the simulated scheme for ANY weights, so
+ * P - CV is an unbiased estimator of the premium P.  Networks: 11 -> 64 -> 64
+ * -> 64 -> 5, SiLU, BF16 operands with FP32 accumulation on the matrix cores,
+ * FP32 master weights, Adam.  Plain Monte-Carlo is reported beside every
+ * control-variate estimate as the judge of bias and variance reduction.
+ */
+#ifndef THOTH_MCL_CV_H
+#define THOTH_MCL_CV_H
+
+#include <stdint.h>
+
+#include "thoth_mcl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP32 master weights: per layer W^T row-major, padded [64x16][64x64][64x64][16x64],
+ * then the biases [64][64][64][16]. */
+#define THOTH_CV_WEIGHTS 10448
+
+typedef struct thoth_cv_weights {
+    float w[THOTH_CV_WEIGHTS];
+} thoth_cv_weights;
+
+typedef struct thoth_cv_config {
+    int32_t hedge_dates;       /* J: hedge ratios are re-evaluated at dates round(j m / J) */
+    int64_t train_paths;       /* training paths (Sobol points train_sobol_skip+1 ..) */
+    int64_t train_sobol_skip;
+    int32_t adam_steps;
+    int32_t batch_paths;       /* paths per Adam step, multiple of 64 */
+    float learning_rate;
+    uint64_t init_seed;        /* weight initialisation */
+} thoth_cv_config;
+
+typedef struct thoth_cv_train_report {
+    double loss_first;   /* mean Y^2 of the first minibatch (f = 0 => mean P^2) */
+    double loss_last;    /* mean Y^2 of the last minibatch */
+    double data_ms;      /* training-path sweep */
+    double train_ms;     /* all Adam steps (one hipGraph replayed per step) */
+    int32_t steps;
+} thoth_cv_train_report;
+
+typedef struct thoth_cv_result {
+    int32_t replicates;    /* R random digital shifts */
+    int64_t paths;         /* per replicate */
+    double plain_mean, plain_halfwidth;   /* 95 % CI from the R replicate means */
+    double cv_mean, cv_halfwidth;
+    double plain_var, cv_var;             /* pooled per-path variances of P and of P - CV */
+    double sweep_ms, eval_ms;             /* device time: FP32 sweep + CV inference */
+} thoth_cv_result;
+
+/* Train the hedge network for the base book of req (FP32 sweep).  w is the
+ * output (and is ignored on input). */
+int thoth_cv_train(const thoth_mcl_request* req, const thoth_cv_config* cfg, thoth_cv_weights* w,
+                   thoth_cv_train_report* rep);
+
+/* Price the base book of req with R = replicates randomised-QMC replicates of
+ * req->mcl.paths paths each, plain and with the control variate of weights w. */
+int thoth_cv_price(const thoth_mcl_request* req, const thoth_cv_config* cfg, const thoth_cv_weights* w,
+                   int32_t replicates, uint64_t rqmc_seed, thoth_cv_result* res);
+
+/* The control-variate samples of the base book of req, FP32 sweep: with
+ * replicates = 0, `paths` paths from Sobol points sobol_skip+1 ..; with
+ * replicates R > 0, points 0 .. paths-1 under R random digital shifts.
+ * feat [R'][J][paths][16] BF16 bits, dm [R'][J][paths][8], pay [R'][paths],
+ * R' = max(1, R) (the layouts thoth_cv_loss_grad consumes for R' = 1). */
+int thoth_cv_samples(const thoth_mcl_request* req, int32_t hedge_dates, int64_t paths, int64_t sobol_skip,
+                     int32_t replicates, uint64_t rqmc_seed, uint16_t* feat, float* dm, float* pay);
+
+/* Loss mean_p (P_p - sum_j f(x_pj).dm_pj)^2 and its gradient (same layout as
+ * thoth_cv_weights) for B paths of host data: feat [J][B][16] BF16, dm [J][B][8],
+ * pay [B]; B multiple of 64.  The device training step's forward/backward. */
+int thoth_cv_loss_grad(const thoth_cv_weights* w, const uint16_t* feat, const float* dm, const float* pay, int32_t B,
+                       int32_t J, double* loss, float* grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

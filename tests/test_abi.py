@@ -23,7 +23,8 @@ def declared_functions():
 def test_headers_declare_the_boundary():
     names = declared_functions()
     assert {"thoth_gpu_available", "thoth_gpu_device_info", "thoth_gpu_price_european_gbm",
-            "thoth_mcl_sweep", "thoth_mcl_price", "thoth_last_error", "thoth_gpu_sobol_points"} <= names
+            "thoth_mcl_sweep", "thoth_mcl_price", "thoth_last_error", "thoth_gpu_sobol_points",
+            "thoth_cv_train", "thoth_cv_price", "thoth_cv_loss_grad"} <= names
 
 
 def test_library_exports_every_declared_symbol():
@@ -35,19 +36,21 @@ def test_library_exports_every_declared_symbol():
         assert re.search(r"\bT %s\b" % name, exported), name
 
 
-def test_structs_match_header_sizes():
+def test_structs_match_header_sizes(tmp_path):
     # compile a probe of the header struct sizes and compare with the ctypes mirror
     from thoth_amd import abi
-    src = ROOT / "_work" / "abi_probe.c"
-    src.parent.mkdir(exist_ok=True)
-    src.write_text('#include <stdio.h>\n#include "thoth_mcl.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu\\n",'
-                   'sizeof(thoth_mcl_request),sizeof(thoth_mcl_result),sizeof(thoth_asset),sizeof(thoth_contract),'
-                   'sizeof(thoth_mcl_book),sizeof(thoth_mcl_sweep_cfg),sizeof(thoth_mcl_book_out));}\n')
-    exe = src.with_suffix("")
+    structs = [("thoth_mcl_request", abi.Request), ("thoth_mcl_result", abi.Result), ("thoth_asset", abi.Asset),
+               ("thoth_contract", abi.Contract), ("thoth_mcl_book", abi.Book), ("thoth_mcl_sweep_cfg", abi.SweepCfg),
+               ("thoth_mcl_book_out", abi.BookOut), ("thoth_cv_weights", abi.CvWeights),
+               ("thoth_cv_config", abi.CvConfig), ("thoth_cv_train_report", abi.CvTrainReport),
+               ("thoth_cv_result", abi.CvResult)]
+    src = tmp_path / "abi_probe.c"
+    src.write_text('#include <stdio.h>\n#include "thoth_mcl_gpu.h"\n#include "thoth_mcl_cv.h"\nint main(){'
+                   + "".join('printf("%%zu\\n", sizeof(%s));' % n for n, _ in structs) + "}\n")
+    exe = tmp_path / "abi_probe"
     subprocess.run(["gcc", "-I%s" % (ROOT / "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
-    want = [C.sizeof(t) for t in (abi.Request, abi.Result, abi.Asset, abi.Contract, abi.Book, abi.SweepCfg, abi.BookOut)]
-    assert got == want
+    assert got == [C.sizeof(t) for _, t in structs]
 
 
 def test_no_device_means_unavailable_not_fallback():

@@ -1,0 +1,143 @@
+"""Learned martingale control variate on the device: the training kernels'
+loss and gradient against the CPU oracle (oracle/mlp_oracle.py), the zero-mean
+property of the hedge increments (unbiasedness for any network), and the
+variance a trained network removes on the 5-asset LSV worst-of basket,
+judged against plain Monte-Carlo on the same randomised-QMC replicates."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from thoth_amd import abi, configs
+from thoth_amd.book import BuiltRequest
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def random_flat(rng, scale=0.25):
+    params = [(rng.normal(0, scale, (i, o)), rng.normal(0, 0.1, o)) for i, o in mo.SIZES]
+    return params, mo.pack(params)
+
+
+def device_samples(rng, B, J):
+    x = rng.normal(0, 1, (B, J, 11)).astype(np.float32)
+    feat = np.zeros((J, B, 16), np.uint16)
+    feat[:, :, :11] = mo.bf16_bits(x).transpose(1, 0, 2)
+    dm = rng.normal(0, 1, (B, J, 5)).astype(np.float32)
+    dmd = np.zeros((J, B, 8), np.float32)
+    dmd[:, :, :5] = dm.transpose(1, 0, 2)
+    pay = rng.normal(3, 2, B).astype(np.float32)
+    return mo.bf16(x), dm.astype(np.float64), pay.astype(np.float64), feat, dmd, pay
+
+
+@pytest.mark.parametrize("B,J", [(64, 1), (256, 5)])
+def test_loss_and_gradient_match_oracle(gpu, B, J):
+    rng = np.random.default_rng(100 + J)
+    params, flat = random_flat(rng)
+    x, dm, pay, feat, dmd, payf = device_samples(rng, B, J)
+    w = gpu.cv_weights_from_array(flat)
+    loss, grad = gpu.cv_loss_grad(w, feat, dmd, payf)
+    # the oracle evaluated at the FP32 weights the device holds
+    params32 = mo.unpack(np.float32(flat))
+    l_mc, g_mc = mo.loss_and_grads(params32, x, dm, pay, matrix_cores=True)
+    l_ex, g_ex = mo.loss_and_grads(params32, x, dm, pay)
+    ref_mc, ref_ex = mo.pack(g_mc), mo.pack(g_ex)
+    err_mc = np.linalg.norm(grad - ref_mc) / np.linalg.norm(ref_mc)
+    err_ex = np.linalg.norm(grad - ref_ex) / np.linalg.norm(ref_ex)
+    print("loss %.8g oracle(bf16) %.8g oracle(fp64) %.8g  grad rel err bf16 %.2e fp64 %.2e"
+          % (loss, l_mc, l_ex, err_mc, err_ex))
+    assert abs(loss - l_mc) <= 2e-4 * l_mc
+    assert err_mc <= 3e-3        # same rounding points as the matrix cores: FP32 accumulation order only
+    assert err_ex <= 5e-2        # BF16 operands vs exact arithmetic
+    # padding stays untouched: layer-1 inputs 11..15, output units 5..15
+    assert not grad[mo.W_OFF[0]:mo.W_OFF[1]].reshape(64, 16)[:, 11:].any()
+    assert not grad[mo.W_OFF[3]:mo.B_OFF[0]].reshape(16, 64)[5:].any()
+    assert not grad[mo.B_OFF[3] + 5:].any()
+
+
+def test_gradient_is_deterministic(gpu):
+    rng = np.random.default_rng(5)
+    _, flat = random_flat(rng)
+    _, _, _, feat, dmd, payf = device_samples(rng, 128, 3)
+    w = gpu.cv_weights_from_array(flat)
+    a = gpu.cv_loss_grad(w, feat, dmd, payf)
+    b = gpu.cv_loss_grad(w, feat, dmd, payf)
+    assert a[0] == b[0] and np.array_equal(a[1], b[1])
+
+
+@pytest.fixture(scope="module")
+def basket():
+    """5-asset LSV worst-of call, 2y, 365 steps, FP32 (the production basket)."""
+    return BuiltRequest(configs.lsv_worst_of_basket(paths=1 << 14, precision=abi.FP32))
+
+
+def test_rqmc_samples_are_finite(gpu, basket):
+    """Regression: under these digital shifts replicate 1 maps the coordinate
+    of dimension 662 at point 7765 to exactly 0; the FP32 inverse normal must
+    stay finite there (it returned -inf, and the variance step turned the path
+    into NaN, hidden from the worst-of payoff by fminf)."""
+    feat, dm, pay = gpu.cv_samples(basket, 73, 1 << 14, replicates=4, rqmc_seed=3)
+    f = (feat.astype(np.uint32) << 16).view(np.float32)
+    assert np.isfinite(f).all() and np.isfinite(dm).all() and np.isfinite(pay).all()
+    # the whole-path hedge sums have zero mean: replicate means within 4 standard errors
+    tot = dm[..., :5].sum(axis=1)                  # [R, paths, 5]
+    rep = tot.mean(axis=1)                          # [R, 5]
+    se = rep.std(axis=0, ddof=1) / np.sqrt(rep.shape[0])
+    assert (np.abs(rep.mean(axis=0)) <= 4 * se + 1e-3).all(), (rep.mean(axis=0), se)
+
+
+def test_zero_network_is_plain_monte_carlo(gpu, basket):
+    cfg = gpu.cv_config()
+    r = gpu.cv_price(basket, cfg, abi.CvWeights(), replicates=4, rqmc_seed=3)
+    assert r.cv_mean == r.plain_mean and r.cv_var == r.plain_var
+
+
+def test_hedge_increments_have_zero_mean(gpu, basket):
+    """A constant unit hedge on every equity: CV = the whole discounted,
+    rebased martingale sum, whose mean is zero under the simulated scheme.
+    The control-variate estimate must agree with plain MC within the
+    replicate confidence interval."""
+    flat = np.zeros(abi.CV_WEIGHTS)
+    flat[mo.B_OFF[3]:mo.B_OFF[3] + 5] = 1.0
+    cfg = gpu.cv_config()
+    r = gpu.cv_price(basket, cfg, gpu.cv_weights_from_array(flat), replicates=16, rqmc_seed=11)
+    print("plain %.6f +- %.6f   const-hedge %.6f +- %.6f" % (r.plain_mean, r.plain_halfwidth, r.cv_mean,
+                                                            r.cv_halfwidth))
+    assert r.cv_mean != r.plain_mean
+    assert abs(r.cv_mean - r.plain_mean) <= 1.5 * (r.cv_halfwidth + r.plain_halfwidth)
+
+
+def test_random_network_is_unbiased(gpu, basket):
+    rng = np.random.default_rng(21)
+    _, flat = random_flat(rng, scale=0.3)
+    r = gpu.cv_price(basket, gpu.cv_config(), gpu.cv_weights_from_array(flat), replicates=16, rqmc_seed=12)
+    print("plain %.6f +- %.6f   random-net %.6f +- %.6f" % (r.plain_mean, r.plain_halfwidth, r.cv_mean,
+                                                           r.cv_halfwidth))
+    assert abs(r.cv_mean - r.plain_mean) <= 1.5 * (r.cv_halfwidth + r.plain_halfwidth)
+
+
+def test_trained_network_reduces_variance(gpu, oracle, basket):
+    """What the learned control variate gains: trained on engine paths from a
+    disjoint Sobol block, evaluated on fresh randomised-QMC replicates, it must
+    cut the per-path variance several-fold while the estimate stays within the
+    confidence interval of plain MC and of the FP64 price."""
+    cfg = gpu.cv_config(train_paths=1 << 16, adam_steps=1000, batch_paths=2048)
+    w, rep = gpu.cv_train(basket, cfg)
+    print("train: loss %.5f -> %.5f, data %.2f ms, %d Adam steps %.2f ms"
+          % (rep.loss_first, rep.loss_last, rep.data_ms, rep.steps, rep.train_ms))
+    assert rep.loss_last < 0.8 * rep.loss_first
+    r = gpu.cv_price(basket, cfg, w, replicates=16, rqmc_seed=99)
+    vr = r.plain_var / r.cv_var
+    print("plain %.6f +- %.6f  cv %.6f +- %.6f  per-path variance %.4f -> %.4f (x%.1f)  sweep %.2f ms eval %.2f ms"
+          % (r.plain_mean, r.plain_halfwidth, r.cv_mean, r.cv_halfwidth, r.plain_var, r.cv_var, vr, r.sweep_ms,
+             r.eval_ms))
+    assert vr >= 3.0
+    assert abs(r.cv_mean - r.plain_mean) <= 1.5 * (r.cv_halfwidth + r.plain_halfwidth)
+    # against the FP64 device price on 2^18 Sobol points (its own iid trust as scale)
+    b64 = BuiltRequest(configs.lsv_worst_of_basket(paths=1 << 18, precision=abi.FP64))
+    p64 = gpu.price(b64)
+    assert abs(r.cv_mean - p64.premium) <= r.cv_halfwidth + 3 * p64.premium_trust

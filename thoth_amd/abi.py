@@ -102,3 +102,27 @@ class SweepCfg(C.Structure):
 class BookOut(C.Structure):
     _fields_ = [("book_sum", _d), ("book_sum2", _d), ("contract_sum", _d * MAX_CONTRACTS),
                 ("contract_sum2", _d * MAX_CONTRACTS), ("count", _i64), ("device_ms", _d)]
+
+
+# include/thoth_mcl_cv.h
+CV_WEIGHTS = 10448
+
+
+class CvWeights(C.Structure):
+    _fields_ = [("w", C.c_float * CV_WEIGHTS)]
+
+
+class CvConfig(C.Structure):
+    _fields_ = [("hedge_dates", _i32), ("train_paths", _i64), ("train_sobol_skip", _i64),
+                ("adam_steps", _i32), ("batch_paths", _i32), ("learning_rate", C.c_float),
+                ("init_seed", C.c_uint64)]
+
+
+class CvTrainReport(C.Structure):
+    _fields_ = [("loss_first", _d), ("loss_last", _d), ("data_ms", _d), ("train_ms", _d), ("steps", _i32)]
+
+
+class CvResult(C.Structure):
+    _fields_ = [("replicates", _i32), ("paths", _i64), ("plain_mean", _d), ("plain_halfwidth", _d),
+                ("cv_mean", _d), ("cv_halfwidth", _d), ("plain_var", _d), ("cv_var", _d),
+                ("sweep_ms", _d), ("eval_ms", _d)]

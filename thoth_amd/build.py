@@ -35,10 +35,12 @@ UNITS = [
     ("mcl_sweep_f64.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_sweep_f32.hip", "hipcc", []),
     ("mcl_util.hip", "hipcc", []),
+    ("mcl_cv.hip", "hipcc", []),
     ("mcl_gpu.cpp", "g++", []),
     ("mcl_books.cpp", "g++", []),
     ("mcl_market.cpp", "g++", []),
     ("mcl_host.cpp", "g++", []),
+    ("mcl_cv_host.cpp", "g++", []),
 ]
 
 

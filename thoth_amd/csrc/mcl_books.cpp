@@ -44,7 +44,7 @@ struct Arena
 struct Offsets
 {
     size_t t, dt, sqrt_dt, bsq, binv, step_ops, op_b, op_left, op_w, op_wf, drift, sc, lev, levf, lnstep, offset,
-        invstep, mon, flow;
+        invstep, mon, flow, cv_start, cv_end, cv_scale;
 };
 
 std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
@@ -258,6 +258,52 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
     return THOTH_OK;
 }
 
+// hedge dates d_j = round(j m / J) and the per-date increment scale
+// df(t_i) 100 / S0_a of the learned control variate
+int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Offsets& of, Arena& ar )
+{
+    const int nd = in.n_dates, m = nd - 1, na = in.n_assets;
+    if ( cv.J < 1 || cv.J > m )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "hedge dates must be in [1, steps]" ) );
+    if ( na > 5 )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "the hedge network covers at most 5 equities" ) );
+    std::vector<int> start( nd, -1 ), end( nd, -1 );
+    int prev = 0;
+    for ( int j = 0; j < cv.J; j++ )
+    {
+        const int d0 = (int)std::lround( (double)j * m / cv.J );
+        const int d1 = (int)std::lround( (double)( j + 1 ) * m / cv.J );
+        if ( d1 <= d0 || d0 != prev )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "hedge dates must be increasing" ) );
+        start[d0] = j;
+        end[d1] = j;
+        prev = d1;
+    }
+    std::vector<float> scale( (size_t)na * nd, 0.0f );
+    for ( int a = 0; a < na; a++ )
+        for ( int i = 1; i < nd; i++ )
+            scale[(size_t)a * nd + i] =
+                (float)( std::exp( in.t[0] * in.disc_rate[0] - in.t[i] * in.disc_rate[i] ) * 100.0 / in.spot[a] );
+    db.cv_J = cv.J;
+    db.cv_N = cv.N;
+    db.cv_inv_T = (float)( 1.0 / in.t[m] );
+    for ( int a = 0; a < na; a++ )
+    {
+        // reference vol: the Black-Scholes vol, else sqrt(v0)
+        const double ref = in.model[a] == THOTH_MODEL_BS ? in.vol[a] : std::sqrt( std::max( in.v0[a], 1e-8 ) );
+        db.cv_lnS0[a] = (float)std::log( in.spot[a] );
+        db.cv_ln_scale[a] = (float)( 1.0 / ( std::max( ref, 1e-4 ) * std::sqrt( in.t[m] ) ) );
+        db.cv_vol_scale[a] = (float)( 1.0 / std::max( ref, 1e-4 ) );
+    }
+    db.cv_feat = cv.feat;
+    db.cv_dm = cv.dm;
+    db.cv_pay = cv.pay;
+    of.cv_start = ar.put( start );
+    of.cv_end = ar.put( end );
+    of.cv_scale = ar.put( scale );
+    return THOTH_OK;
+}
+
 template <class T>
 T* At( char* base, size_t off )
 {
@@ -272,7 +318,8 @@ DeviceBatch::~DeviceBatch()
         (void)hipFree( dbase );
 }
 
-int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B )
+int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
+                  const CvSpec* cv )
 {
     int rc = EnsureDevice();
     if ( rc != THOTH_OK )
@@ -311,6 +358,15 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         rc = PackBook( b, books[b], B.db[b], of[b], ar, B.depth );
         if ( rc != THOTH_OK )
             return rc;
+        if ( cv && cv->J > 0 )
+        {
+            if ( cfg->precision != 1 || n_books != 1 )
+                return SetError( THOTH_ERR_INVALID, "the control-variate data comes from FP32 sweeps of one book" );
+            rc = PackCv( b, books[b], *cv, B.db[b], of[b], ar );
+            if ( rc != THOTH_OK )
+                return rc;
+            B.cv = true;
+        }
         B.na_max = std::max( B.na_max, B.db[b].n_assets );
     }
     // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV)
@@ -395,6 +451,12 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
         d.mon_mask = At<uint32_t>( dbase, o.mon );
         d.flow_mask = At<uint32_t>( dbase, o.flow );
+        if ( d.cv_J > 0 )
+        {
+            d.cv_start = At<int>( dbase, o.cv_start );
+            d.cv_end = At<int>( dbase, o.cv_end );
+            d.cv_scale = At<float>( dbase, o.cv_scale );
+        }
         d.partial = B.dpart + B.part_off[b];
     }
     B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
@@ -425,7 +487,7 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
                   ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
                                       B.first, B.last, B.dshifts )
                   : launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                      B.first, B.last, B.dshifts );
+                                      B.first, B.last, B.dshifts, B.cv );
     if ( !ok )
     {
         (void)hipEventDestroy( e0 );

@@ -1,0 +1,449 @@
+// mcl_cv_host.cpp -- host side of the learned martingale control variate
+// (include/thoth_mcl_cv.h): training-data sweep, Adam training as one
+// hipGraph replayed per step, and randomised-QMC pricing with and without
+// the control variate.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "mcl_hostlib.hpp"
+#include "mcl_internal.hpp"
+#include "mcl_launch.hpp"
+#include "thoth_mcl_cv.h"
+
+namespace thoth
+{
+
+namespace
+{
+
+constexpr int kWSize = 10240, kBSize = 208, kN = kWSize + kBSize;
+static_assert( kN == THOTH_CV_WEIGHTS, "weight layout" );
+constexpr int kWOff[4] = { 0, 64 * 16, 64 * 16 + 64 * 64, 64 * 16 + 2 * 64 * 64 };
+constexpr int kRows[4] = { 64, 64, 64, 16 };
+constexpr int kCols[4] = { 16, 64, 64, 64 };
+constexpr int kIn[4] = { 11, 64, 64, 64 };   // true fan-in
+constexpr int kOut[4] = { 64, 64, 64, 5 };   // true fan-out
+
+struct Dev
+{
+    std::vector<void*> p;
+    ~Dev()
+    {
+        for ( void* x : p )
+            (void)hipFree( x );
+    }
+    template <class T>
+    T* alloc( size_t n )
+    {
+        void* x = nullptr;
+        if ( hipMalloc( &x, n * sizeof( T ) ) != hipSuccess )
+            return nullptr;
+        p.push_back( x );
+        return static_cast<T*>( x );
+    }
+};
+
+#define HIP_OK( expr )                                                                                      \
+    do                                                                                                      \
+    {                                                                                                       \
+        hipError_t e_ = ( expr );                                                                           \
+        if ( e_ != hipSuccess )                                                                             \
+            return SetError( THOTH_ERR_DEVICE, std::string( #expr " failed: " ) + hipGetErrorString( e_ ) ); \
+    } while ( 0 )
+
+// seeded initialisation: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero
+// biases, zero output layer (the control variate starts at zero)
+void InitWeights( uint64_t seed, float* w )
+{
+    std::mt19937_64 rng( seed );
+    std::fill( w, w + kN, 0.0f );
+    for ( int L = 0; L < 3; L++ )
+    {
+        const float a = 1.0f / std::sqrt( (float)kIn[L] );
+        std::uniform_real_distribution<float> u( -a, a );
+        for ( int o = 0; o < kOut[L]; o++ )
+            for ( int i = 0; i < kIn[L]; i++ )
+                w[kWOff[L] + o * kCols[L] + i] = u( rng );
+    }
+}
+
+double StudentT975( int df )
+{
+    // two-sided 95 % quantiles of Student's t (df 1..30), normal beyond
+    static const double q[] = { 12.706, 4.303, 3.182, 2.776, 2.571, 2.447, 2.365, 2.306, 2.262, 2.228,
+                                2.201,  2.179, 2.160, 2.145, 2.131, 2.120, 2.110, 2.101, 2.093, 2.086,
+                                2.080,  2.074, 2.069, 2.064, 2.060, 2.056, 2.052, 2.048, 2.045, 2.042 };
+    if ( df < 1 )
+        return 0.0;
+    if ( df <= 30 )
+        return q[df - 1];
+    return 1.96 + 2.4 / df;  // Cornish-Fisher first correction
+}
+
+int CheckConfig( const thoth_cv_config& c )
+{
+    if ( c.hedge_dates < 1 || c.batch_paths < 64 || c.batch_paths % 64 || c.adam_steps < 0 || c.train_paths < c.batch_paths ||
+         c.train_sobol_skip < 0 || !( c.learning_rate > 0 ) )
+        return SetError( THOTH_ERR_INVALID, "control-variate config out of range" );
+    return THOTH_OK;
+}
+
+// FP32 CV-data sweep of the base book: `paths` paths per replicate
+int CvSweep( const thoth_mcl_request& req, int J, int64_t paths, int64_t skip, int R, uint64_t rqmc_seed, Dev& dev,
+             CvSpec& cv, std::vector<thoth_mcl_book_out>& out, double& ms )
+{
+    host::BookArrays A;
+    host::Grid g;
+    int rc = host::ResolveBaseBook( req, A, g );
+    if ( rc != THOTH_OK )
+        return rc;
+    const int Rr = std::max( 1, R );
+    cv.J = J;
+    cv.N = paths;
+    cv.feat = dev.alloc<uint16_t>( (size_t)Rr * J * paths * 16 );
+    cv.dm = dev.alloc<float>( (size_t)Rr * J * paths * 8 );
+    cv.pay = dev.alloc<float>( (size_t)Rr * paths );
+    if ( !cv.feat || !cv.dm || !cv.pay )
+        return SetError( THOTH_ERR_DEVICE, "cannot allocate the control-variate sample buffers" );
+    thoth_mcl_sweep_cfg cfg{};
+    cfg.paths = paths;
+    cfg.sobol_skip = skip;
+    cfg.precision = 1;
+    cfg.replicates = R;
+    cfg.rqmc_seed = rqmc_seed;
+    DeviceBatch B;
+    rc = PrepareBatch( 1, &A.book, &cfg, B, &cv );
+    if ( rc != THOTH_OK )
+        return rc;
+    out.assign( Rr, thoth_mcl_book_out{} );
+    rc = RunBatch( B, out.data() );
+    ms = out[0].device_ms;
+    return rc;
+}
+
+}  // namespace
+
+}  // namespace thoth
+
+using namespace thoth;
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_train( const thoth_mcl_request* req,
+                                                                        const thoth_cv_config* cfg,
+                                                                        thoth_cv_weights* w, thoth_cv_train_report* rep )
+{
+    if ( !req || !cfg || !w )
+        return SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( DeviceMutex() );
+    try
+    {
+        int rc = EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        if ( ( rc = CheckConfig( *cfg ) ) != THOTH_OK )
+            return rc;
+        Dev dev;
+        CvSpec cv;
+        std::vector<thoth_mcl_book_out> out;
+        double data_ms = 0;
+        const int J = cfg->hedge_dates;
+        const int64_t N = cfg->train_paths;
+        rc = CvSweep( *req, J, N, cfg->train_sobol_skip, 0, 0, dev, cv, out, data_ms );
+        if ( rc != THOTH_OK )
+            return rc;
+        const int B = cfg->batch_paths;
+        const CvTrainLayout lay = cv_train_layout( B, J );
+        float* dw = dev.alloc<float>( kN );
+        float* dm_ = dev.alloc<float>( kN );
+        float* dv = dev.alloc<float>( kN );
+        uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
+        uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
+        float* cvpart = dev.alloc<float>( (size_t)lay.chunks * B );
+        float* gpart = dev.alloc<float>( (size_t)lay.blocks * kN );
+        float* loss_part = dev.alloc<float>( lay.blocks );
+        int* step = dev.alloc<int>( 1 );
+        if ( !dw || !dm_ || !dv || !fwd || !bwd || !cvpart || !gpart || !loss_part || !step )
+            return SetError( THOTH_ERR_DEVICE, "cannot allocate the training buffers" );
+        HIP_OK( cv_train_init() );
+        std::vector<float> w0( kN );
+        InitWeights( cfg->init_seed, w0.data() );
+        hipStream_t s = Stream();
+        HIP_OK( hipMemcpyAsync( dw, w0.data(), kN * sizeof( float ), hipMemcpyHostToDevice, s ) );
+        HIP_OK( hipMemsetAsync( dm_, 0, kN * sizeof( float ), s ) );
+        HIP_OK( hipMemsetAsync( dv, 0, kN * sizeof( float ), s ) );
+        HIP_OK( hipMemsetAsync( step, 0, sizeof( int ), s ) );
+        launch_mlp_pack( s, dw, fwd, bwd );
+        // one Adam step = gradient + Adam + BF16 repack + step counter, captured once
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIP_OK( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
+        const bool launched = launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, cv.feat, cv.dm, cv.pay, N, step,
+                                                    cvpart, gpart, loss_part, cfg->learning_rate );
+        HIP_OK( hipStreamEndCapture( s, &graph ) );
+        if ( !launched )
+        {
+            (void)hipGraphDestroy( graph );
+            return SetError( THOTH_ERR_INVALID, "batch layout not supported" );
+        }
+        HIP_OK( hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 ) );
+        std::vector<float> lp( lay.blocks );
+        double loss_first = 0, loss_last = 0;
+        hipEvent_t e0, e1;
+        HIP_OK( hipEventCreate( &e0 ) );
+        HIP_OK( hipEventCreate( &e1 ) );
+        HIP_OK( hipEventRecord( e0, s ) );
+        for ( int k = 0; k < cfg->adam_steps; k++ )
+        {
+            HIP_OK( hipGraphLaunch( exec, s ) );
+            if ( k == 0 || k == cfg->adam_steps - 1 )
+            {
+                HIP_OK( hipMemcpyAsync( lp.data(), loss_part, lp.size() * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+                HIP_OK( hipStreamSynchronize( s ) );
+                double l = 0;
+                for ( float x : lp )
+                    l += x;
+                ( k == 0 ? loss_first : loss_last ) = l / B;
+            }
+        }
+        HIP_OK( hipEventRecord( e1, s ) );
+        HIP_OK( hipMemcpyAsync( w->w, dw, kN * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+        HIP_OK( hipStreamSynchronize( s ) );
+        float ms = 0;
+        HIP_OK( hipEventElapsedTime( &ms, e0, e1 ) );
+        (void)hipEventDestroy( e0 );
+        (void)hipEventDestroy( e1 );
+        (void)hipGraphExecDestroy( exec );
+        (void)hipGraphDestroy( graph );
+        if ( rep )
+        {
+            rep->loss_first = loss_first;
+            rep->loss_last = cfg->adam_steps > 1 ? loss_last : loss_first;
+            rep->data_ms = data_ms;
+            rep->train_ms = ms;
+            rep->steps = cfg->adam_steps;
+        }
+        return THOTH_OK;
+    }
+    catch ( const host::Invalid& e )
+    {
+        return SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_price( const thoth_mcl_request* req,
+                                                                        const thoth_cv_config* cfg,
+                                                                        const thoth_cv_weights* w, int32_t replicates,
+                                                                        uint64_t rqmc_seed, thoth_cv_result* res )
+{
+    if ( !req || !cfg || !w || !res )
+        return SetError( THOTH_ERR_INVALID, "null argument" );
+    if ( replicates < 2 )
+        return SetError( THOTH_ERR_INVALID, "at least 2 replicates are needed for a confidence interval" );
+    std::lock_guard<std::mutex> lk( DeviceMutex() );
+    try
+    {
+        int rc = EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        if ( ( rc = CheckConfig( *cfg ) ) != THOTH_OK )
+            return rc;
+        Dev dev;
+        CvSpec cv;
+        std::vector<thoth_mcl_book_out> out;
+        double sweep_ms = 0;
+        const int J = cfg->hedge_dates, R = replicates;
+        const int64_t N = req->mcl.paths;
+        rc = CvSweep( *req, J, N, 0, R, rqmc_seed, dev, cv, out, sweep_ms );
+        if ( rc != THOTH_OK )
+            return rc;
+        float* dw = dev.alloc<float>( kN );
+        uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
+        uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
+        const uint32_t nb = (uint32_t)( ( N + 63 ) / 64 );
+        double* part = dev.alloc<double>( (size_t)R * nb * 4 );
+        if ( !dw || !fwd || !bwd || !part )
+            return SetError( THOTH_ERR_DEVICE, "cannot allocate the evaluation buffers" );
+        hipStream_t s = Stream();
+        HIP_OK( hipMemcpyAsync( dw, w->w, kN * sizeof( float ), hipMemcpyHostToDevice, s ) );
+        launch_mlp_pack( s, dw, fwd, bwd );
+        hipEvent_t e0, e1;
+        HIP_OK( hipEventCreate( &e0 ) );
+        HIP_OK( hipEventCreate( &e1 ) );
+        HIP_OK( hipEventRecord( e0, s ) );
+        uint32_t nbl = 0;
+        launch_cv_eval( s, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part, &nbl );
+        HIP_OK( hipGetLastError() );
+        HIP_OK( hipEventRecord( e1, s ) );
+        std::vector<double> hp( (size_t)R * nb * 4 );
+        HIP_OK( hipMemcpyAsync( hp.data(), part, hp.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+        HIP_OK( hipStreamSynchronize( s ) );
+        float ems = 0;
+        HIP_OK( hipEventElapsedTime( &ems, e0, e1 ) );
+        (void)hipEventDestroy( e0 );
+        (void)hipEventDestroy( e1 );
+        // replicate means of P - CV and P; pooled per-path variances
+        std::vector<double> my( R ), mp( R );
+        double sy2 = 0, sp2 = 0, sy = 0, sp = 0;
+        for ( int r = 0; r < R; r++ )
+        {
+            double v[4] = { 0, 0, 0, 0 }, c[4] = { 0, 0, 0, 0 };
+            for ( uint32_t b = 0; b < nb; b++ )
+                for ( int k = 0; k < 4; k++ )
+                {
+                    const double y = hp[( (size_t)r * nb + b ) * 4 + k] - c[k];
+                    const double t = v[k] + y;
+                    c[k] = ( t - v[k] ) - y;
+                    v[k] = t;
+                }
+            my[r] = v[0] / N;
+            mp[r] = v[2] / N;
+            sy += v[0];
+            sy2 += v[1];
+            sp += v[2];
+            sp2 += v[3];
+        }
+        auto ci = [&]( const std::vector<double>& x, double& mean, double& hw )
+        {
+            mean = 0;
+            for ( double a : x )
+                mean += a;
+            mean /= R;
+            double var = 0;
+            for ( double a : x )
+                var += ( a - mean ) * ( a - mean );
+            var /= ( R - 1 );
+            hw = StudentT975( R - 1 ) * std::sqrt( var / R );
+        };
+        ci( my, res->cv_mean, res->cv_halfwidth );
+        ci( mp, res->plain_mean, res->plain_halfwidth );
+        const double n = (double)N * R;
+        res->cv_var = ( sy2 - sy * sy / n ) / ( n - 1 );
+        res->plain_var = ( sp2 - sp * sp / n ) / ( n - 1 );
+        res->replicates = R;
+        res->paths = N;
+        res->sweep_ms = sweep_ms;
+        res->eval_ms = ems;
+        return THOTH_OK;
+    }
+    catch ( const host::Invalid& e )
+    {
+        return SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_loss_grad( const thoth_cv_weights* w,
+                                                                            const uint16_t* feat, const float* dm,
+                                                                            const float* pay, int32_t B, int32_t J,
+                                                                            double* loss, float* grad )
+{
+    if ( !w || !feat || !dm || !pay || B < 64 || B % 64 || J < 1 )
+        return SetError( THOTH_ERR_INVALID, "B must be a positive multiple of 64" );
+    std::lock_guard<std::mutex> lk( DeviceMutex() );
+    int rc = EnsureDevice();
+    if ( rc != THOTH_OK )
+        return rc;
+    Dev dev;
+    const CvTrainLayout lay = cv_train_layout( B, J );
+    float* dw = dev.alloc<float>( kN );
+    float* dm_ = dev.alloc<float>( kN );
+    float* dv = dev.alloc<float>( kN );
+    uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
+    uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
+    uint16_t* dfeat = dev.alloc<uint16_t>( (size_t)J * B * 16 );
+    float* ddm = dev.alloc<float>( (size_t)J * B * 8 );
+    float* dpay = dev.alloc<float>( B );
+    float* cvpart = dev.alloc<float>( (size_t)lay.chunks * B );
+    float* gpart = dev.alloc<float>( (size_t)lay.blocks * kN );
+    float* loss_part = dev.alloc<float>( lay.blocks );
+    int* step = dev.alloc<int>( 1 );
+    if ( !dw || !dm_ || !dv || !fwd || !bwd || !dfeat || !ddm || !dpay || !cvpart || !gpart || !loss_part || !step )
+        return SetError( THOTH_ERR_DEVICE, "allocation failed" );
+    HIP_OK( cv_train_init() );
+    hipStream_t s = Stream();
+    HIP_OK( hipMemcpyAsync( dw, w->w, kN * sizeof( float ), hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipMemcpyAsync( dfeat, feat, (size_t)J * B * 16 * sizeof( uint16_t ), hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipMemcpyAsync( ddm, dm, (size_t)J * B * 8 * sizeof( float ), hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipMemcpyAsync( dpay, pay, B * sizeof( float ), hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipMemsetAsync( dm_, 0, kN * sizeof( float ), s ) );
+    HIP_OK( hipMemsetAsync( dv, 0, kN * sizeof( float ), s ) );
+    HIP_OK( hipMemsetAsync( step, 0, sizeof( int ), s ) );
+    launch_mlp_pack( s, dw, fwd, bwd );
+    // gradient only: the training step with a zero learning rate leaves w unchanged
+    if ( !launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, dfeat, ddm, dpay, B, step, cvpart, gpart, loss_part,
+                                0.0f ) )
+        return SetError( THOTH_ERR_INVALID, "batch layout not supported" );
+    HIP_OK( hipGetLastError() );
+    std::vector<float> gp( (size_t)lay.blocks * kN ), lp( lay.blocks );
+    HIP_OK( hipMemcpyAsync( gp.data(), gpart, gp.size() * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+    HIP_OK( hipMemcpyAsync( lp.data(), loss_part, lp.size() * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+    HIP_OK( hipStreamSynchronize( s ) );
+    double l = 0;
+    for ( float x : lp )
+        l += x;
+    if ( loss )
+        *loss = l / B;
+    if ( grad )
+        for ( int e = 0; e < kN; e++ )
+        {
+            double g = 0;
+            for ( int k = 0; k < lay.blocks; k++ )
+                g += gp[(size_t)k * kN + e];
+            grad[e] = (float)g;
+        }
+    return THOTH_OK;
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_samples( const thoth_mcl_request* req,
+                                                                          int32_t hedge_dates, int64_t paths,
+                                                                          int64_t sobol_skip, int32_t replicates,
+                                                                          uint64_t rqmc_seed, uint16_t* feat,
+                                                                          float* dm, float* pay )
+{
+    if ( !req || !feat || !dm || !pay || paths < 1 || sobol_skip < 0 || replicates < 0 )
+        return SetError( THOTH_ERR_INVALID, "null argument or empty sample" );
+    std::lock_guard<std::mutex> lk( DeviceMutex() );
+    try
+    {
+        int rc = EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        Dev dev;
+        CvSpec cv;
+        std::vector<thoth_mcl_book_out> out;
+        double ms = 0;
+        rc = CvSweep( *req, hedge_dates, paths, sobol_skip, replicates, rqmc_seed, dev, cv, out, ms );
+        if ( rc != THOTH_OK )
+            return rc;
+        hipStream_t s = Stream();
+        const size_t reps = (size_t)std::max( 1, replicates );
+        const size_t rows = reps * hedge_dates * paths;
+        HIP_OK( hipMemcpyAsync( feat, cv.feat, rows * 16 * sizeof( uint16_t ), hipMemcpyDeviceToHost, s ) );
+        HIP_OK( hipMemcpyAsync( dm, cv.dm, rows * 8 * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+        HIP_OK( hipMemcpyAsync( pay, cv.pay, reps * paths * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+        HIP_OK( hipStreamSynchronize( s ) );
+        return THOTH_OK;
+    }
+    catch ( const host::Invalid& e )
+    {
+        return SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}

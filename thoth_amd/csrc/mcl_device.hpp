@@ -80,6 +80,20 @@ struct DevBook
     int rb_best[kMaxRainbows], rb_n[kMaxRainbows], rb_member[kMaxRainbows][kMaxAssets];
     double rb_weight[kMaxRainbows][kMaxAssets];
     double* partial;            // [replicates][n_blocks][1 + n_contracts][2] per-block sums
+    // learned control variate data (FP32 sweeps only; cv_J = 0 disables):
+    // hedge dates d_0 = 0 < d_1 < .. < d_J = m split the steps into J intervals
+    int cv_J;
+    const int* cv_start;        // [n_dates] j when date i = d_j (j < J), else -1
+    const int* cv_end;          // [n_dates] j when date i = d_{j+1}, else -1
+    const float* cv_scale;      // [n_assets * n_dates] df(t_i) * 100 / S0_a
+    float cv_inv_T;             // 1 / t_m
+    float cv_lnS0[kMaxAssets];
+    float cv_ln_scale[kMaxAssets];   // 1 / (reference vol * sqrt(t_m)): standardised log-moneyness
+    float cv_vol_scale[kMaxAssets];  // 1 / reference vol
+    int64_t cv_N;               // paths per replicate (sample stride)
+    uint16_t* cv_feat;          // [R][J][N][16] BF16 features at d_j
+    float* cv_dm;               // [R][J][N][8] discounted rebased martingale increments over interval j
+    float* cv_pay;              // [R][N] discounted book value
 };
 
 }  // namespace thoth

@@ -25,12 +25,6 @@
 namespace thoth::host
 {
 
-struct Invalid : std::runtime_error
-{
-    int code;
-    Invalid( int c, const std::string& s ) : std::runtime_error( s ), code( c ) {}
-};
-
 namespace
 {
 constexpr double kSpotBump = 0.01;    // helpers/constants.hpp:36
@@ -50,19 +44,6 @@ std::set<double> MonitoringDates( const thoth_contract& c )
     return s;
 }
 
-// the arrays one thoth_mcl_book points to
-struct BookArrays
-{
-    std::vector<int32_t> model;
-    std::vector<double> spot, vol, v0, kappa, theta, xi, rho, drift, chol, disc;
-    std::vector<double> lev_ln_step, lev_values;
-    std::vector<int64_t> lev_offset;
-    std::vector<int32_t> rb_best, rb_n, rb_member;
-    std::vector<double> rb_weight;
-    std::vector<int32_t> c_kind, c_call, c_udl, c_flow, c_up, c_in, c_mon_off, c_mon;
-    std::vector<double> c_strike, c_level;
-    thoth_mcl_book book{};
-};
 
 }  // namespace
 
@@ -505,6 +486,40 @@ int RunPlan( const Plan& plan, thoth_mcl_result* res )
     return THOTH_OK;
 }
 
+
+// base scenario only (no Greek bumps): grid, Cholesky, calibration, resolution
+int ResolveBaseBook( const thoth_mcl_request& req, BookArrays& A, Grid& g )
+{
+    if ( req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS || !req.assets )
+        throw Invalid( THOTH_ERR_INVALID, "1..8 equities" );
+    if ( req.n_contracts < 1 || req.n_contracts > THOTH_MCL_MAX_CONTRACTS || !req.contracts )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "1..32 contracts per book" );
+    const Market mk = BuildMarket( req );
+    g = BuildGrid( req, mk );
+    std::vector<double> chol;
+    const int n = req.n_assets;
+    if ( n > 1 )
+    {
+        if ( !req.correlation )
+            throw Invalid( THOTH_ERR_INVALID, "book pricing requires a correlation matrix" );
+        chol.assign( req.correlation, req.correlation + n * n );
+        if ( !CholeskyLower( chol, n ) )
+            throw Invalid( THOTH_ERR_INVALID, "correlation is not SDP" );
+        for ( int i = 0; i < n; i++ )
+            for ( int j = i + 1; j < n; j++ )
+                chol[i * n + j] = 0;
+    }
+    std::vector<LeverageGrid> lev( n );
+    for ( int a = 0; a < n; a++ )
+        if ( mk.eq[a].kind == THOTH_VOL_LSV )
+        {
+            int rc = CalibrateLeverage( mk, a, g, lev[a] );
+            if ( rc != THOTH_OK )
+                return rc;
+        }
+    Resolve( req, mk, g, chol, lev, A );
+    return THOTH_OK;
+}
 
 int PriceImpl( const thoth_mcl_request& req, thoth_mcl_result* res )
 {

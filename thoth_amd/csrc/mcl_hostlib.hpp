@@ -5,9 +5,12 @@
 #pragma once
 #include <cmath>
 #include <cstdint>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "thoth_mcl.h"
+#include "thoth_mcl_gpu.h"
 
 namespace thoth::host
 {
@@ -77,9 +80,32 @@ struct LeverageGrid
     std::vector<double> values;  // [dates * points]
 };
 
+// the arrays one thoth_mcl_book points to (the book keeps pointers into them)
+struct BookArrays
+{
+    std::vector<int32_t> model;
+    std::vector<double> spot, vol, v0, kappa, theta, xi, rho, drift, chol, disc;
+    std::vector<double> lev_ln_step, lev_values;
+    std::vector<int64_t> lev_offset;
+    std::vector<int32_t> rb_best, rb_n, rb_member;
+    std::vector<double> rb_weight;
+    std::vector<int32_t> c_kind, c_call, c_udl, c_flow, c_up, c_in, c_mon_off, c_mon;
+    std::vector<double> c_strike, c_level;
+    thoth_mcl_book book{};
+};
+
 Market BuildMarket( const thoth_mcl_request& req );
 Grid BuildGrid( const thoth_mcl_request& req, const Market& mk );
 bool CholeskyLower( std::vector<double>& a, int n );  // maths.cpp:354-381
 int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& out );
+// the base scenario's device book (grid, Cholesky, LSV calibration, payoffs);
+// throws on invalid input like the other host entry points
+int ResolveBaseBook( const thoth_mcl_request& req, BookArrays& A, Grid& g );
+
+struct Invalid : std::runtime_error
+{
+    int code;
+    Invalid( int c, const std::string& s ) : std::runtime_error( s ), code( c ) {}
+};
 
 }  // namespace thoth::host

@@ -47,13 +47,25 @@ struct DeviceBatch
     uint32_t q0 = 0, nblocks = 0, first = 0, last = 0;  // q0: first 256-index block
     size_t lds_bytes = 0;
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
+    bool cv = false;  // the books carry control-variate sample buffers
     int64_t paths = 0;
     DeviceBatch() = default;
     DeviceBatch( const DeviceBatch& ) = delete;
     DeviceBatch& operator=( const DeviceBatch& ) = delete;
     ~DeviceBatch();
 };
-int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B );
+// Learned-CV data of an FP32 sweep: J hedge intervals, device output buffers
+// sized [replicates][J][N][16] / [replicates][J][N][8] / [replicates][N].
+struct CvSpec
+{
+    int J = 0;
+    int64_t N = 0;
+    uint16_t* feat = nullptr;
+    float* dm = nullptr;
+    float* pay = nullptr;
+};
+int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
+                  const CvSpec* cv = nullptr );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out );
 int SweepBooks( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg,
                 thoth_mcl_book_out* out );

@@ -42,6 +42,22 @@ constexpr int kMaxStackDepth = 16;
 bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
 bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
-                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
+                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts,
+                       bool cv = false );
+void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bwd );
+void launch_cv_eval( hipStream_t s, const uint16_t* fwd, const float* bias, const uint16_t* feat, const float* dm,
+                     const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks );
+// one Adam step of the hedge network: minibatch of B paths x J dates split
+// into `chunks` date chunks of dpc dates; one gradient slice per block
+struct CvTrainLayout
+{
+    int B, J, tiles, dpc, chunks, waves, blocks;
+};
+CvTrainLayout cv_train_layout( int B, int J );
+hipError_t cv_train_init();  // once per process before the first step (LDS limit of the gradient kernel)
+// buffers: cvpart [chunks * B], gpart [blocks * THOTH_CV_WEIGHTS], loss_part [blocks]
+bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd, float* w,
+                           float* m, float* v, const uint16_t* feat, const float* dm, const float* pay, int64_t N,
+                           int* step, float* cvpart, float* gpart, float* loss_part, float lr );
 bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t first, uint32_t last, uint32_t* out );
 }  // namespace thoth

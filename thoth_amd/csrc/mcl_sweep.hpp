@@ -127,6 +127,9 @@ __device__ __forceinline__ double normal_f64( uint32_t x )
 // in the ~0.3 % of draws with w >= 5.  The FP64 oracle mode keeps Acklam.
 __device__ __forceinline__ float normal_f32( uint32_t x )
 {
+    // u = 0 (x = 0, reachable under a random digital shift) is read as the
+    // smallest grid point 2^-32, keeping the normal finite (-6.23)
+    x = x > 1u ? x : 1u;
     const float a = (float)x;
     const float b = (float)( 0u - x );
     const float w = -__logf( a * b * 2.168404344971009e-19f );  // 2^-62
@@ -224,7 +227,36 @@ __device__ __forceinline__ R wave_sum( R x )
     return x;
 }
 
-template <class R, int NA, int DEPTH>
+// Log of E[exp(A v') | v] - 1 for the QE draw v' from v (Andersen 2008, the
+// moment generating functions behind his martingale correction): quadratic
+// branch v' = a (b + Z)^2 -> exp(A b^2 a / (1 - 2 A a)) / sqrt(1 - 2 A a);
+// exponential branch (mass p at 0, else Exp(beta)) -> p + (1 - p) beta /
+// (beta - A).  Returns false where the moment does not exist.
+__device__ __forceinline__ bool qe_mgf_m1( float v, float A, const float* __restrict__ sc, float& out )
+{
+    const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
+    const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
+    const float psi = ( m > 0.0f ) ? s2 / ( m * m ) : 0.0f;
+    if ( psi <= 1.5f )
+    {
+        const float c = 2.0f / psi;
+        const float b2 = c - 1.0f + sqrtf( c * ( c - 1.0f ) );
+        const float a = m / ( 1.0f + b2 );
+        const float q = 1.0f - 2.0f * A * a;
+        if ( !( q > 0.0f ) )
+            return false;
+        out = expm1f( A * b2 * a / q - 0.5f * log1pf( -2.0f * A * a ) );
+        return true;
+    }
+    const float p = ( psi - 1.0f ) / ( psi + 1.0f );
+    const float beta = ( 1.0f - p ) / m;
+    if ( !( A < beta ) )
+        return false;
+    out = ( 1.0f - p ) * A / ( beta - A );
+    return true;
+}
+
+template <class R, int NA, int DEPTH, bool CV = false>
 __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restrict__ books,
                                                          const uint32_t* __restrict__ V, uint32_t bq0,
                                                          uint32_t first_idx, uint32_t last_idx,
@@ -326,6 +358,70 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     uint32_t hit = 0;
     double book = 0.0;
 
+    // ---- learned control variate data (FP32 only)
+    constexpr bool cv_on = CV && !F64;
+    const int64_t pidx = (int64_t)idx - (int64_t)first_idx;
+    const bool cv_store = cv_on && valid && pidx >= 0 && pidx < B.cv_N;
+    float dmacc[NA];
+#pragma unroll
+    for ( int a = 0; a < NA; a++ )
+        dmacc[a] = 0.0f;
+    auto cv_features = [&]( int i, int j )
+    {
+        // standardised features, BF16: 5 log-moneyness / (vol_ref sqrt(T)),
+        // 5 x 2 (sqrt(v) / vol_ref - 1) (0 for Black-Scholes), 2 t / T - 1
+        uint32_t w[8];
+        float f[16];
+#pragma unroll
+        for ( int k = 0; k < 16; k++ )
+            f[k] = 0.0f;
+#pragma unroll
+        for ( int a = 0; a < NA; a++ )
+            if ( a < na && a < 5 )
+            {
+                f[a] = ( __logf( (float)S[a] ) - B.cv_lnS0[a] ) * B.cv_ln_scale[a];
+                if ( B.asset[a].model != 0 )
+                    f[5 + a] = 2.0f * fmaf( __builtin_amdgcn_sqrtf( fmaxf( (float)Var[a], 0.0f ) ),
+                                            B.cv_vol_scale[a], -1.0f );
+            }
+        f[10] = fmaf( 2.0f * (float)B.t[i], B.cv_inv_T, -1.0f );
+#pragma unroll
+        for ( int k = 0; k < 8; k++ )
+        {
+            uint32_t lo = __float_as_uint( f[2 * k] ), hi = __float_as_uint( f[2 * k + 1] );
+            lo = ( lo + 0x7fffu + ( ( lo >> 16 ) & 1u ) ) >> 16;
+            hi = ( hi + 0x7fffu + ( ( hi >> 16 ) & 1u ) ) >> 16;
+            w[k] = lo | ( hi << 16 );
+        }
+        uint4* dst = reinterpret_cast<uint4*>( B.cv_feat + ( ( (size_t)blockIdx.z * B.cv_J + j ) * B.cv_N + pidx ) * 16 );
+        dst[0] = uint4{ w[0], w[1], w[2], w[3] };
+        dst[1] = uint4{ w[4], w[5], w[6], w[7] };
+    };
+    auto cv_date = [&]( int i )
+    {
+        const int je = B.cv_end[i];
+        if ( je >= 0 )
+        {
+            if ( cv_store )
+            {
+                float o[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+                for ( int a = 0; a < NA; a++ )
+                    if ( a < 8 )
+                        o[a] = dmacc[a];
+                float4* dst = reinterpret_cast<float4*>( B.cv_dm + ( ( (size_t)blockIdx.z * B.cv_J + je ) * B.cv_N + pidx ) * 8 );
+                dst[0] = float4{ o[0], o[1], o[2], o[3] };
+                dst[1] = float4{ o[4], o[5], o[6], o[7] };
+            }
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+                dmacc[a] = 0.0f;
+        }
+        const int js = B.cv_start[i];
+        if ( js >= 0 && cv_store )
+            cv_features( i, js );
+    };
+
     auto udl_value = [&]( int u ) -> R
     {
         if ( u >= 0 )
@@ -386,6 +482,9 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     };
 
     observe( 0 );
+    if constexpr ( !F64 )
+        if ( cv_on )
+            cv_date( 0 );
 
     for ( int p = 0; p < m; p++ )
     {
@@ -502,7 +601,15 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
                     const float dw = w1 - W[a];
                     W[a] = w1;
                     const float expo = cd - sc[SC_HALF_V2DT] + sc[SC_VOL] * dw;
-                    S[a] = S[a] * __expf( expo );
+                    const float sold = S[a];
+                    S[a] = sold * __expf( expo );
+                    if constexpr ( cv_on )
+                    {
+                        // M = S e^{det} (e^{b z} - e^{b^2/2}): zero conditional mean
+                        const float M = sold * __expf( cd - sc[SC_HALF_V2DT] ) *
+                                        ( expm1f( sc[SC_VOL] * dw ) - expm1f( sc[SC_HALF_V2DT] ) );
+                        dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
+                    }
                 }
                 continue;
             }
@@ -578,23 +685,62 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
             }
             else
             {
+                float det, bvol;
                 if ( B.asset[a].xi > 1e-12 )
                 {
                     const float aa = L * sc[SC_A1] - L * L * sc[SC_A2];
                     const float K1 = aa - L * sc[SC_RX];
                     const float K2 = aa + L * sc[SC_RX];
                     const float var = fmaxf( L * L * sc[SC_K3] * ( vp + vc ), 0.0f );
-                    step = cd + L * sc[SC_K0] + K1 * vp + K2 * vc + __builtin_amdgcn_sqrtf( var ) * z;
+                    det = cd + L * sc[SC_K0] + K1 * vp + K2 * vc;
+                    bvol = __builtin_amdgcn_sqrtf( var );
                 }
                 else
                 {
-                    step = cd - 0.5f * L * L * vp * sc[SC_DT] + L * __builtin_amdgcn_sqrtf( vp * sc[SC_DT] ) * z;
+                    det = cd - 0.5f * L * L * vp * sc[SC_DT];
+                    bvol = L * __builtin_amdgcn_sqrtf( vp * sc[SC_DT] );
                 }
-                S[a] = S[a] * __expf( step );
+                step = det + bvol * z;
+                const float sold = S[a];
+                S[a] = sold * __expf( step );
+                if constexpr ( cv_on )
+                {
+                    // spot increment minus its conditional mean given the past:
+                    // log S_i - log S_{i-1} = c0 + X with c0 known at t_{i-1} and
+                    // X = A v_i + b z - b^2/2, E[e^X | past] = E[e^{A v_i} | v_{i-1}],
+                    // so M = S_{i-1} e^{c0} (expm1(X) - (E[e^{A v_i}] - 1)).
+                    // Where that moment does not exist (or xi = 0): the increment
+                    // given v_i as well, S_{i-1} e^{det} (e^{b z} - e^{b^2/2}).
+                    float M;
+                    float mgf = 0.0f;
+                    if ( B.asset[a].xi > 1e-12 )
+                    {
+                        const float half_l2k3 = 0.5f * L * L * sc[SC_K3];
+                        const float Acoef = L * sc[SC_A1] - L * L * sc[SC_A2] + L * sc[SC_RX] + half_l2k3;
+                        if ( qe_mgf_m1( vp, Acoef, sc, mgf ) )
+                        {
+                            const float c0 = cd + L * sc[SC_K0] + ( L * sc[SC_A1] - L * L * sc[SC_A2] - L * sc[SC_RX] +
+                                                                    half_l2k3 ) * vp;
+                            const float X = Acoef * vc + bvol * z - 0.5f * bvol * bvol;
+                            M = sold * __expf( c0 ) * ( expm1f( X ) - mgf );
+                        }
+                        else
+                            M = sold * __expf( det ) * ( expm1f( bvol * z ) - expm1f( 0.5f * bvol * bvol ) );
+                    }
+                    else
+                        M = sold * __expf( det ) * ( expm1f( bvol * z ) - expm1f( 0.5f * bvol * bvol ) );
+                    dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
+                }
             }
         }
         observe( i );
+        if constexpr ( !F64 )
+            if ( cv_on )
+                cv_date( i );
     }
+    if constexpr ( !F64 )
+        if ( cv_store )
+            B.cv_pay[(size_t)blockIdx.z * B.cv_N + pidx] = (float)book;
 
     // ---- book value reduction
     if ( !valid )

@@ -49,6 +49,13 @@ def lib():
         L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
                                                    P(C.c_double), P(C.c_int64), P(C.c_double)]
+        L.thoth_cv_train.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), P(abi.CvTrainReport)]
+        L.thoth_cv_price.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), C.c_int32, C.c_uint64,
+                                     P(abi.CvResult)]
+        L.thoth_cv_loss_grad.argtypes = [P(abi.CvWeights), P(C.c_uint16), P(C.c_float), P(C.c_float), C.c_int32,
+                                         C.c_int32, P(C.c_double), P(C.c_float)]
+        L.thoth_cv_samples.argtypes = [P(abi.Request), C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_uint64,
+                                       P(C.c_uint16), P(C.c_float), P(C.c_float)]
         _lib = L
     return _lib
 
@@ -216,3 +223,69 @@ def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_see
     out = (abi.BookOut * (n * max(1, replicates)))()
     _check(lib().thoth_mcl_sweep(n, arr, C.byref(cfg), out))
     return list(out)
+
+
+# ---- learned martingale control variate (include/thoth_mcl_cv.h) ----------
+
+def cv_config(hedge_dates=73, train_paths=1 << 18, train_sobol_skip=1 << 24, adam_steps=2000, batch_paths=1024,
+              learning_rate=1e-3, init_seed=7):
+    """Training/evaluation settings; the training paths come from a Sobol block
+    (points train_sobol_skip+1 ..) disjoint from the evaluation replicates."""
+    return abi.CvConfig(hedge_dates, train_paths, train_sobol_skip, adam_steps, batch_paths, learning_rate, init_seed)
+
+
+def cv_train(built, cfg):
+    """Train the hedge network on the base book of `built`; returns
+    (abi.CvWeights, abi.CvTrainReport)."""
+    w, rep = abi.CvWeights(), abi.CvTrainReport()
+    _check(lib().thoth_cv_train(C.byref(built.req), C.byref(cfg), C.byref(w), C.byref(rep)))
+    return w, rep
+
+
+def cv_price(built, cfg, weights, replicates=16, rqmc_seed=1):
+    """Plain and control-variate RQMC estimates of the base book premium
+    (built.req.mcl.paths paths per replicate); returns abi.CvResult."""
+    res = abi.CvResult()
+    _check(lib().thoth_cv_price(C.byref(built.req), C.byref(cfg), C.byref(weights), replicates, rqmc_seed,
+                                C.byref(res)))
+    return res
+
+
+def cv_samples(built, hedge_dates, paths, sobol_skip=0, replicates=0, rqmc_seed=0):
+    """Control-variate samples of the base book (FP32 sweep): feat uint16
+    [R', J, paths, 16] (BF16 bits), dm float32 [R', J, paths, 8], pay float32
+    [R', paths] with R' = max(1, replicates)."""
+    R = max(1, replicates)
+    feat = np.zeros((R, hedge_dates, paths, 16), np.uint16)
+    dm = np.zeros((R, hedge_dates, paths, 8), np.float32)
+    pay = np.zeros((R, paths), np.float32)
+    _check(lib().thoth_cv_samples(C.byref(built.req), hedge_dates, paths, sobol_skip, replicates, rqmc_seed,
+                                  feat.ctypes.data_as(C.POINTER(C.c_uint16)), dm.ctypes.data_as(C.POINTER(C.c_float)),
+                                  pay.ctypes.data_as(C.POINTER(C.c_float))))
+    return feat, dm, pay
+
+
+def cv_loss_grad(weights, feat, dm, pay):
+    """Device loss and gradient for host samples: feat uint16 [J, B, 16] (BF16
+    bits), dm float32 [J, B, 8], pay float32 [B]; returns (loss, grad[10448])."""
+    feat = np.ascontiguousarray(feat, np.uint16)
+    dm = np.ascontiguousarray(dm, np.float32)
+    pay = np.ascontiguousarray(pay, np.float32)
+    J, B, _ = feat.shape
+    assert feat.shape == (J, B, 16) and dm.shape == (J, B, 8) and pay.shape == (B,)
+    loss = C.c_double()
+    grad = np.zeros(abi.CV_WEIGHTS, np.float32)
+    _check(lib().thoth_cv_loss_grad(C.byref(weights), feat.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                    dm.ctypes.data_as(C.POINTER(C.c_float)), pay.ctypes.data_as(C.POINTER(C.c_float)),
+                                    B, J, C.byref(loss), grad.ctypes.data_as(C.POINTER(C.c_float))))
+    return loss.value, grad
+
+
+def cv_weights_from_array(a):
+    w = abi.CvWeights()
+    np.ctypeslib.as_array(w.w)[:] = np.asarray(a, np.float32)
+    return w
+
+
+def cv_weights_array(w):
+    return np.ctypeslib.as_array(w.w).copy()
