@@ -64,6 +64,45 @@ def cpu_baseline(name, sample_paths):
                       "%.1f s on %s" % (sample_paths, sec, platform.processor() or platform.machine())}
 
 
+def time_to_1bp(replicates, paths, seed=101, target=0.01):
+    """Wall time to a 95 % half-width of 1bp of notional (0.01 on the rebased
+    100 notional) on the 5-asset LSV worst-of basket, plain Monte-Carlo in the
+    FP64 oracle-matched mode against the FP32 learned control variate.  Each
+    estimator is run once at R randomised-QMC replicates of `paths` Sobol
+    points (the same digital shifts for all three) and its device time is
+    projected to the target by the 1/sqrt(N) law of the half-width."""
+    out = {"target_halfwidth": target, "replicates": replicates, "paths_per_replicate": paths,
+           "projection": "device ms x (halfwidth / target)^2"}
+
+    def entry(mean, hw, ms, **kw):
+        e = {"premium": mean, "halfwidth": hw, "ms": ms, "projected_ms": ms * (hw / target) ** 2}
+        e.update(kw)
+        return e
+
+    b64 = BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=abi.FP64))
+    r64 = mcl.price_rqmc(b64, replicates, seed)
+    out["plain_fp64"] = entry(r64.mean, r64.halfwidth, r64.sweep_ms, per_path_var=r64.var)
+    b32 = BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=abi.FP32))
+    r32 = mcl.price_rqmc(b32, replicates, seed)
+    out["plain_fp32"] = entry(r32.mean, r32.halfwidth, r32.sweep_ms, per_path_var=r32.var)
+    cfg = mcl.cv_config()
+    w, rep = mcl.cv_train(b32, cfg)
+    rcv = mcl.cv_price(b32, cfg, w, replicates, seed)
+    train_ms = rep.data_ms + rep.train_ms
+    out["learned_cv_fp32"] = entry(rcv.cv_mean, rcv.cv_halfwidth, rcv.sweep_ms + rcv.eval_ms,
+                                   per_path_var=rcv.cv_var, variance_ratio=rcv.plain_var / rcv.cv_var,
+                                   train_ms=train_ms,
+                                   training={"paths": cfg.train_paths, "adam_steps": cfg.adam_steps,
+                                             "batch_paths": cfg.batch_paths, "hedge_dates": cfg.hedge_dates,
+                                             "learning_rate": cfg.learning_rate,
+                                             "loss_first": rep.loss_first, "loss_last": rep.loss_last})
+    t64 = out["plain_fp64"]["projected_ms"]
+    tcv = out["learned_cv_fp32"]["projected_ms"]
+    out["speedup_vs_plain_fp64"] = t64 / tcv
+    out["speedup_vs_plain_fp64_incl_training"] = t64 / (tcv + train_ms)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +112,9 @@ def main():
     ap.add_argument("--workload", default="lsv_worst_of_basket")
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-t1bp", action="store_true", help="skip the time-to-1bp comparison")
+    ap.add_argument("--t1bp-replicates", type=int, default=32)
+    ap.add_argument("--t1bp-paths", type=int, default=1 << 14)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build is single-GPU (one MI355X); --gpus %d is not supported" % args.gpus,
@@ -126,6 +168,8 @@ def main():
                      "frac": achieved / peak, "traffic": None},
         "device": mcl.device_info(),
     }
+    if not args.no_t1bp and args.workload == "lsv_worst_of_basket":
+        line["time_to_1bp"] = time_to_1bp(args.t1bp_replicates, args.t1bp_paths)
     if not args.no_cpu:
         cb = cpu_baseline(args.workload, args.cpu_sample)
         line["cpu_baseline"] = cb

@@ -158,6 +158,21 @@ int thoth_mcl_prepare(const thoth_mcl_request* req, thoth_mcl_plan* plan);
 int thoth_mcl_run(thoth_mcl_plan plan, thoth_mcl_result* res);
 void thoth_mcl_release(thoth_mcl_plan plan);
 
+/* Randomised QMC price of the base book of req (no Greeks): R = replicates
+ * random digital shifts of Sobol points 0 .. req->mcl.paths - 1, each an
+ * independent estimate; the 95 % interval is Student-t over the R replicate
+ * means (the reference's trust is the iid formula, monte_carlo_node.cpp:35-40,
+ * which is not a confidence interval under Sobol points). */
+typedef struct thoth_rqmc_result {
+    int32_t replicates;
+    int64_t paths;       /* per replicate */
+    double mean;         /* premium */
+    double halfwidth;    /* 95 % confidence half-width of the mean */
+    double var;          /* pooled per-path variance */
+    double sweep_ms;     /* device time of the sweep */
+} thoth_rqmc_result;
+int thoth_mcl_price_rqmc(const thoth_mcl_request* req, int32_t replicates, uint64_t rqmc_seed, thoth_rqmc_result* res);
+
 /* Diffusion grid of a request (pricer_mcl.cpp:71-117): writes up to cap dates
  * (days from today, year fractions t and step dt) and returns the date count,
  * or a negative THOTH_* status. */

@@ -41,7 +41,14 @@ typedef struct thoth_cv_config {
     int32_t batch_paths;       /* paths per Adam step, multiple of 64 */
     float learning_rate;
     uint64_t init_seed;        /* weight initialisation */
+    int32_t eval_mode;         /* THOTH_CV_IN_LOOP or THOTH_CV_TWO_PHASE (pricing only) */
 } thoth_cv_config;
+
+/* pricing with a trained network: the hedge network evaluated on the matrix
+ * cores inside the FP32 stepping loop at each hedge date (production), or
+ * two phases -- the sweep stores features and increments, a separate
+ * inference kernel reads them (same estimator, used as a cross-check) */
+enum { THOTH_CV_IN_LOOP = 0, THOTH_CV_TWO_PHASE = 1 };
 
 typedef struct thoth_cv_train_report {
     double loss_first;   /* mean Y^2 of the first minibatch (f = 0 => mean P^2) */
@@ -57,7 +64,7 @@ typedef struct thoth_cv_result {
     double plain_mean, plain_halfwidth;   /* 95 % CI from the R replicate means */
     double cv_mean, cv_halfwidth;
     double plain_var, cv_var;             /* pooled per-path variances of P and of P - CV */
-    double sweep_ms, eval_ms;             /* device time: FP32 sweep + CV inference */
+    double sweep_ms, eval_ms;             /* device time: FP32 sweep (with in-loop inference) + separate inference */
 } thoth_cv_result;
 
 /* Train the hedge network for the base book of req (FP32 sweep).  w is the

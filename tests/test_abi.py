@@ -43,7 +43,7 @@ def test_structs_match_header_sizes(tmp_path):
                ("thoth_contract", abi.Contract), ("thoth_mcl_book", abi.Book), ("thoth_mcl_sweep_cfg", abi.SweepCfg),
                ("thoth_mcl_book_out", abi.BookOut), ("thoth_cv_weights", abi.CvWeights),
                ("thoth_cv_config", abi.CvConfig), ("thoth_cv_train_report", abi.CvTrainReport),
-               ("thoth_cv_result", abi.CvResult)]
+               ("thoth_cv_result", abi.CvResult), ("thoth_rqmc_result", abi.RqmcResult)]
     src = tmp_path / "abi_probe.c"
     src.write_text('#include <stdio.h>\n#include "thoth_mcl_gpu.h"\n#include "thoth_mcl_cv.h"\nint main(){'
                    + "".join('printf("%%zu\\n", sizeof(%s));' % n for n, _ in structs) + "}\n")

@@ -137,6 +137,15 @@ def test_trained_network_reduces_variance(gpu, oracle, basket):
              r.eval_ms))
     assert vr >= 3.0
     assert abs(r.cv_mean - r.plain_mean) <= 1.5 * (r.cv_halfwidth + r.plain_halfwidth)
+    # the two-phase evaluation (stored samples + separate inference kernel) is
+    # the same estimator: same replicates, same plain sums, CV within FP32 rounding
+    cfg2 = gpu.cv_config(train_paths=1 << 16, adam_steps=1000, batch_paths=2048, eval_mode=abi.CV_TWO_PHASE)
+    r2 = gpu.cv_price(basket, cfg2, w, replicates=16, rqmc_seed=99)
+    print("two-phase cv %.6f +- %.6f var %.4f  sweep %.2f ms eval %.2f ms" % (r2.cv_mean, r2.cv_halfwidth, r2.cv_var,
+                                                                             r2.sweep_ms, r2.eval_ms))
+    assert abs(r2.plain_mean - r.plain_mean) <= 1e-6 * abs(r.plain_mean)  # stored payoffs are FP32
+    assert abs(r2.cv_mean - r.cv_mean) <= 1e-3 * r.cv_halfwidth + 1e-6
+    assert abs(r2.cv_var / r.cv_var - 1) <= 1e-3
     # against the FP64 device price on 2^18 Sobol points (its own iid trust as scale)
     b64 = BuiltRequest(configs.lsv_worst_of_basket(paths=1 << 18, precision=abi.FP64))
     p64 = gpu.price(b64)

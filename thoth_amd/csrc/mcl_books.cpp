@@ -267,6 +267,10 @@ int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Off
         return SetError( THOTH_ERR_INVALID, Why( bi, "hedge dates must be in [1, steps]" ) );
     if ( na > 5 )
         return SetError( THOTH_ERR_INVALID, Why( bi, "the hedge network covers at most 5 equities" ) );
+    if ( cv.mode != 1 && cv.mode != 2 )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "unknown control-variate mode" ) );
+    if ( cv.mode == 2 && ( !cv.wimg || !cv.bias || in.n_contracts >= kMaxContracts ) )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "in-loop inference needs weights and a free result slot" ) );
     std::vector<int> start( nd, -1 ), end( nd, -1 );
     int prev = 0;
     for ( int j = 0; j < cv.J; j++ )
@@ -295,6 +299,8 @@ int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Off
         db.cv_ln_scale[a] = (float)( 1.0 / ( std::max( ref, 1e-4 ) * std::sqrt( in.t[m] ) ) );
         db.cv_vol_scale[a] = (float)( 1.0 / std::max( ref, 1e-4 ) );
     }
+    db.cv_wimg = reinterpret_cast<const uint32_t*>( cv.wimg );
+    db.cv_bias = cv.bias;
     db.cv_feat = cv.feat;
     db.cv_dm = cv.dm;
     db.cv_pay = cv.pay;
@@ -311,6 +317,10 @@ T* At( char* base, size_t off )
 }
 
 }  // namespace
+
+// result slots per block of book b: the book, its contracts, and P - CV when
+// the hedge network runs in the loop
+static int SlotsOf( const DeviceBatch& B, int b ) { return 1 + B.db[b].n_contracts + ( B.cv_mode == 2 ? 1 : 0 ); }
 
 DeviceBatch::~DeviceBatch()
 {
@@ -365,7 +375,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             rc = PackCv( b, books[b], *cv, B.db[b], of[b], ar );
             if ( rc != THOTH_OK )
                 return rc;
-            B.cv = true;
+            B.cv_mode = cv->mode;
         }
         B.na_max = std::max( B.na_max, B.db[b].n_assets );
     }
@@ -380,12 +390,14 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             lev = std::max( lev, (size_t)2 * na_inst * B.db[b].lev_points );
     }
     B.lds_bytes = ( dims + lev ) * sizeof( uint32_t );
+    if ( B.cv_mode == 2 )  // hedge-network area, 16-byte aligned (sweep_kernel's layout)
+        B.lds_bytes = ( ( dims + lev + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + kCvLdsBytes;
     B.part_off.assign( n_books, 0 );
     B.part_total = 0;
     for ( int b = 0; b < n_books; b++ )
     {
         B.part_off[b] = B.part_total;
-        B.part_total += (size_t)B.replicates * B.nblocks * ( 1 + B.db[b].n_contracts ) * 2;
+        B.part_total += (size_t)B.replicates * B.nblocks * SlotsOf( B, b ) * 2;
     }
     // random digital shifts: one 32-bit word per (replicate, dimension) from
     // xoshiro256++ seeded by splitmix64(rqmc_seed) (upper 32 bits of each draw)
@@ -475,7 +487,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     return THOTH_OK;
 }
 
-int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
 {
     hipStream_t s = Stream();
     hipEvent_t e0, e1;
@@ -487,7 +499,7 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
                   ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
                                       B.first, B.last, B.dshifts )
                   : launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                      B.first, B.last, B.dshifts, B.cv );
+                                      B.first, B.last, B.dshifts, B.cv_mode );
     if ( !ok )
     {
         (void)hipEventDestroy( e0 );
@@ -507,7 +519,7 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
       for ( int r = 0; r < B.replicates; r++ )
     {
         const int nc = B.db[b].n_contracts;
-        const int ne = 1 + nc;
+        const int ne = SlotsOf( B, b );
         std::vector<double> sum( ne * 2, 0.0 ), comp( ne * 2, 0.0 );
         const double* pb = part.data() + B.part_off[b] + (size_t)r * B.nblocks * ne * 2;
         for ( uint32_t k = 0; k < B.nblocks; k++ )
@@ -530,6 +542,11 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out )
         }
         o.count = B.paths;
         o.device_ms = ms;
+        if ( cv_out && B.cv_mode == 2 && b == 0 )
+        {
+            cv_out[r].sum = sum[2 * ( 1 + nc )];
+            cv_out[r].sum2 = sum[2 * ( 1 + nc ) + 1];
+        }
     }
     return THOTH_OK;
 }

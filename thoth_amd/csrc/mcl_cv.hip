@@ -86,12 +86,10 @@ __global__ void __launch_bounds__( 256 ) cv_eval_kernel( const uint2* __restrict
     const bool valid = p < N;
     const int64_t pp = valid ? p : 0;
     float cv = 0.0f;
-    MlpTile t;
     for ( int j = 0; j < J; j++ )
     {
         const size_t row = ( (size_t)r * J + j ) * N + pp;
-        mlp_forward( wimg, bias, load_feat( feat, row, lane ), lane, t );
-        cv += mlp_dot( t.out, load_dm( dm, row, lane ) );
+        cv += mlp_dot( mlp_infer( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
     }
     const float P = pay[(size_t)r * N + pp];
     double y = ( valid && lane < 16 ) ? (double)P - (double)cv : 0.0;
@@ -134,13 +132,11 @@ __global__ void __launch_bounds__( 256 ) cv_part_kernel( const uint2* __restrict
     const int b = tile * 16 + ( lane & 15 );
     const int64_t p = ( start + b ) % N;
     const int j0 = chunk * dpc, j1 = min( J, j0 + dpc );
-    MlpTile t;
     float cv = 0.0f;
     for ( int j = j0; j < j1; j++ )
     {
         const size_t row = (size_t)j * N + p;
-        mlp_forward( wimg, bias, load_feat( feat, row, lane ), lane, t );
-        cv += mlp_dot( t.out, load_dm( dm, row, lane ) );
+        cv += mlp_dot( mlp_infer( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
     }
     if ( lane < 16 )
         cvpart[(size_t)chunk * B + b] = cv;

@@ -75,19 +75,6 @@ void InitWeights( uint64_t seed, float* w )
     }
 }
 
-double StudentT975( int df )
-{
-    // two-sided 95 % quantiles of Student's t (df 1..30), normal beyond
-    static const double q[] = { 12.706, 4.303, 3.182, 2.776, 2.571, 2.447, 2.365, 2.306, 2.262, 2.228,
-                                2.201,  2.179, 2.160, 2.145, 2.131, 2.120, 2.110, 2.101, 2.093, 2.086,
-                                2.080,  2.074, 2.069, 2.064, 2.060, 2.056, 2.052, 2.048, 2.045, 2.042 };
-    if ( df < 1 )
-        return 0.0;
-    if ( df <= 30 )
-        return q[df - 1];
-    return 1.96 + 2.4 / df;  // Cornish-Fisher first correction
-}
-
 int CheckConfig( const thoth_cv_config& c )
 {
     if ( c.hedge_dates < 1 || c.batch_paths < 64 || c.batch_paths % 64 || c.adam_steps < 0 || c.train_paths < c.batch_paths ||
@@ -241,6 +228,158 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_train( cons
     }
 }
 
+namespace thoth
+{
+namespace
+{
+
+// 95 % confidence intervals from per-replicate sums (sum Y, sum Y^2, sum P,
+// sum P^2) of R replicates of N paths each
+void FinishCi( const std::vector<double>& sums, int R, int64_t N, thoth_cv_result* res )
+{
+    std::vector<double> my( R ), mp( R );
+    double sy = 0, sy2 = 0, sp = 0, sp2 = 0;
+    for ( int r = 0; r < R; r++ )
+    {
+        my[r] = sums[4 * r] / N;
+        mp[r] = sums[4 * r + 2] / N;
+        sy += sums[4 * r];
+        sy2 += sums[4 * r + 1];
+        sp += sums[4 * r + 2];
+        sp2 += sums[4 * r + 3];
+    }
+    auto ci = [&]( const std::vector<double>& x, double& mean, double& hw )
+    {
+        mean = 0;
+        for ( double a : x )
+            mean += a;
+        mean /= R;
+        double var = 0;
+        for ( double a : x )
+            var += ( a - mean ) * ( a - mean );
+        var /= ( R - 1 );
+        hw = StudentT975( R - 1 ) * std::sqrt( var / R );
+    };
+    ci( my, res->cv_mean, res->cv_halfwidth );
+    ci( mp, res->plain_mean, res->plain_halfwidth );
+    const double n = (double)N * R;
+    res->cv_var = ( sy2 - sy * sy / n ) / ( n - 1 );
+    res->plain_var = ( sp2 - sp * sp / n ) / ( n - 1 );
+    res->replicates = R;
+    res->paths = N;
+}
+
+// the hedge network inside the stepping loop: one FP32 sweep of R replicates
+int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const thoth_cv_weights& w, int R,
+                 uint64_t rqmc_seed, thoth_cv_result* res )
+{
+    host::BookArrays A;
+    host::Grid g;
+    int rc = host::ResolveBaseBook( req, A, g );
+    if ( rc != THOTH_OK )
+        return rc;
+    Dev dev;
+    float* dw = dev.alloc<float>( kN );
+    uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
+    uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
+    if ( !dw || !fwd || !bwd )
+        return SetError( THOTH_ERR_DEVICE, "cannot allocate the network buffers" );
+    hipStream_t s = Stream();
+    HIP_OK( hipMemcpyAsync( dw, w.w, kN * sizeof( float ), hipMemcpyHostToDevice, s ) );
+    launch_mlp_pack( s, dw, fwd, bwd );
+    CvSpec cv;
+    cv.mode = 2;
+    cv.J = cfg.hedge_dates;
+    cv.N = req.mcl.paths;
+    cv.wimg = fwd;
+    cv.bias = dw + kWSize;
+    thoth_mcl_sweep_cfg sc{};
+    sc.paths = req.mcl.paths;
+    sc.precision = 1;
+    sc.replicates = R;
+    sc.rqmc_seed = rqmc_seed;
+    DeviceBatch B;
+    if ( ( rc = PrepareBatch( 1, &A.book, &sc, B, &cv ) ) != THOTH_OK )
+        return rc;
+    std::vector<thoth_mcl_book_out> out( R );
+    std::vector<CvSums> cs( R );
+    if ( ( rc = RunBatch( B, out.data(), cs.data() ) ) != THOTH_OK )
+        return rc;
+    std::vector<double> sums( 4 * (size_t)R );
+    for ( int r = 0; r < R; r++ )
+    {
+        sums[4 * r] = cs[r].sum;
+        sums[4 * r + 1] = cs[r].sum2;
+        sums[4 * r + 2] = out[r].book_sum;
+        sums[4 * r + 3] = out[r].book_sum2;
+    }
+    FinishCi( sums, R, req.mcl.paths, res );
+    res->sweep_ms = out[0].device_ms;
+    res->eval_ms = 0;
+    return THOTH_OK;
+}
+
+// two phases: FP32 sweep writing the samples, then the inference kernel
+int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, const thoth_cv_weights& w, int R,
+                   uint64_t rqmc_seed, thoth_cv_result* res )
+{
+    Dev dev;
+    CvSpec cv;
+    std::vector<thoth_mcl_book_out> out;
+    double sweep_ms = 0;
+    const int J = cfg.hedge_dates;
+    const int64_t N = req.mcl.paths;
+    int rc = CvSweep( req, J, N, 0, R, rqmc_seed, dev, cv, out, sweep_ms );
+    if ( rc != THOTH_OK )
+        return rc;
+    float* dw = dev.alloc<float>( kN );
+    uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
+    uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
+    const uint32_t nb = (uint32_t)( ( N + 63 ) / 64 );
+    double* part = dev.alloc<double>( (size_t)R * nb * 4 );
+    if ( !dw || !fwd || !bwd || !part )
+        return SetError( THOTH_ERR_DEVICE, "cannot allocate the evaluation buffers" );
+    hipStream_t s = Stream();
+    HIP_OK( hipMemcpyAsync( dw, w.w, kN * sizeof( float ), hipMemcpyHostToDevice, s ) );
+    launch_mlp_pack( s, dw, fwd, bwd );
+    hipEvent_t e0, e1;
+    HIP_OK( hipEventCreate( &e0 ) );
+    HIP_OK( hipEventCreate( &e1 ) );
+    HIP_OK( hipEventRecord( e0, s ) );
+    uint32_t nbl = 0;
+    launch_cv_eval( s, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part, &nbl );
+    HIP_OK( hipGetLastError() );
+    HIP_OK( hipEventRecord( e1, s ) );
+    std::vector<double> hp( (size_t)R * nb * 4 );
+    HIP_OK( hipMemcpyAsync( hp.data(), part, hp.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+    HIP_OK( hipStreamSynchronize( s ) );
+    float ems = 0;
+    HIP_OK( hipEventElapsedTime( &ems, e0, e1 ) );
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    std::vector<double> sums( 4 * (size_t)R, 0.0 );
+    for ( int r = 0; r < R; r++ )
+    {
+        double c[4] = { 0, 0, 0, 0 };
+        for ( uint32_t b = 0; b < nb; b++ )
+            for ( int k = 0; k < 4; k++ )
+            {
+                // Kahan over blocks in block order
+                const double y = hp[( (size_t)r * nb + b ) * 4 + k] - c[k];
+                const double t = sums[4 * r + k] + y;
+                c[k] = ( t - sums[4 * r + k] ) - y;
+                sums[4 * r + k] = t;
+            }
+    }
+    FinishCi( sums, R, N, res );
+    res->sweep_ms = sweep_ms;
+    res->eval_ms = ems;
+    return THOTH_OK;
+}
+
+}  // namespace
+}  // namespace thoth
+
 extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_price( const thoth_mcl_request* req,
                                                                         const thoth_cv_config* cfg,
                                                                         const thoth_cv_weights* w, int32_t replicates,
@@ -258,83 +397,11 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_price( cons
             return rc;
         if ( ( rc = CheckConfig( *cfg ) ) != THOTH_OK )
             return rc;
-        Dev dev;
-        CvSpec cv;
-        std::vector<thoth_mcl_book_out> out;
-        double sweep_ms = 0;
-        const int J = cfg->hedge_dates, R = replicates;
-        const int64_t N = req->mcl.paths;
-        rc = CvSweep( *req, J, N, 0, R, rqmc_seed, dev, cv, out, sweep_ms );
-        if ( rc != THOTH_OK )
-            return rc;
-        float* dw = dev.alloc<float>( kN );
-        uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
-        uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
-        const uint32_t nb = (uint32_t)( ( N + 63 ) / 64 );
-        double* part = dev.alloc<double>( (size_t)R * nb * 4 );
-        if ( !dw || !fwd || !bwd || !part )
-            return SetError( THOTH_ERR_DEVICE, "cannot allocate the evaluation buffers" );
-        hipStream_t s = Stream();
-        HIP_OK( hipMemcpyAsync( dw, w->w, kN * sizeof( float ), hipMemcpyHostToDevice, s ) );
-        launch_mlp_pack( s, dw, fwd, bwd );
-        hipEvent_t e0, e1;
-        HIP_OK( hipEventCreate( &e0 ) );
-        HIP_OK( hipEventCreate( &e1 ) );
-        HIP_OK( hipEventRecord( e0, s ) );
-        uint32_t nbl = 0;
-        launch_cv_eval( s, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part, &nbl );
-        HIP_OK( hipGetLastError() );
-        HIP_OK( hipEventRecord( e1, s ) );
-        std::vector<double> hp( (size_t)R * nb * 4 );
-        HIP_OK( hipMemcpyAsync( hp.data(), part, hp.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) );
-        HIP_OK( hipStreamSynchronize( s ) );
-        float ems = 0;
-        HIP_OK( hipEventElapsedTime( &ems, e0, e1 ) );
-        (void)hipEventDestroy( e0 );
-        (void)hipEventDestroy( e1 );
-        // replicate means of P - CV and P; pooled per-path variances
-        std::vector<double> my( R ), mp( R );
-        double sy2 = 0, sp2 = 0, sy = 0, sp = 0;
-        for ( int r = 0; r < R; r++ )
-        {
-            double v[4] = { 0, 0, 0, 0 }, c[4] = { 0, 0, 0, 0 };
-            for ( uint32_t b = 0; b < nb; b++ )
-                for ( int k = 0; k < 4; k++ )
-                {
-                    const double y = hp[( (size_t)r * nb + b ) * 4 + k] - c[k];
-                    const double t = v[k] + y;
-                    c[k] = ( t - v[k] ) - y;
-                    v[k] = t;
-                }
-            my[r] = v[0] / N;
-            mp[r] = v[2] / N;
-            sy += v[0];
-            sy2 += v[1];
-            sp += v[2];
-            sp2 += v[3];
-        }
-        auto ci = [&]( const std::vector<double>& x, double& mean, double& hw )
-        {
-            mean = 0;
-            for ( double a : x )
-                mean += a;
-            mean /= R;
-            double var = 0;
-            for ( double a : x )
-                var += ( a - mean ) * ( a - mean );
-            var /= ( R - 1 );
-            hw = StudentT975( R - 1 ) * std::sqrt( var / R );
-        };
-        ci( my, res->cv_mean, res->cv_halfwidth );
-        ci( mp, res->plain_mean, res->plain_halfwidth );
-        const double n = (double)N * R;
-        res->cv_var = ( sy2 - sy * sy / n ) / ( n - 1 );
-        res->plain_var = ( sp2 - sp * sp / n ) / ( n - 1 );
-        res->replicates = R;
-        res->paths = N;
-        res->sweep_ms = sweep_ms;
-        res->eval_ms = ems;
-        return THOTH_OK;
+        if ( cfg->eval_mode == THOTH_CV_IN_LOOP )
+            return PriceInLoop( *req, *cfg, *w, replicates, rqmc_seed, res );
+        if ( cfg->eval_mode == THOTH_CV_TWO_PHASE )
+            return PriceTwoPhase( *req, *cfg, *w, replicates, rqmc_seed, res );
+        return SetError( THOTH_ERR_INVALID, "unknown evaluation mode" );
     }
     catch ( const host::Invalid& e )
     {

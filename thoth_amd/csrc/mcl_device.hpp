@@ -13,6 +13,14 @@ constexpr int kWavesPerBlock = 4;          // 256-thread blocks
 constexpr int kBlock = 64 * kWavesPerBlock;
 constexpr int kSobolDimsDev = 4097;       // implicit first dimension + 4096 Joe-Kuo rows
 
+// LDS of the in-loop hedge inference: BF16 A-operand image of the network
+// (40 tiles x 64 lanes x 8 bytes), FP32 biases (208), and per wave a
+// [64][16] BF16 feature block plus a [64][8] FP32 hedge-ratio block
+constexpr int kCvImgBytes = 40 * 64 * 8;
+constexpr int kCvBiasBytes = 208 * 4;
+constexpr int kCvWaveBytes = 64 * 16 * 2 + 64 * 8 * 4;
+constexpr int kCvLdsBytes = kCvImgBytes + kCvBiasBytes + kWavesPerBlock * kCvWaveBytes;
+
 // FP32 per-(equity, date) step constants, precomputed on the host in FP64
 // from the reference's QE / Andersen formulas (heston_variance_node.cpp:29-33,
 // heston_spot_node.cpp:60-75).
@@ -91,6 +99,8 @@ struct DevBook
     float cv_ln_scale[kMaxAssets];   // 1 / (reference vol * sqrt(t_m)): standardised log-moneyness
     float cv_vol_scale[kMaxAssets];  // 1 / reference vol
     int64_t cv_N;               // paths per replicate (sample stride)
+    const uint32_t* cv_wimg;    // in-loop inference: BF16 A-operand image of the hedge network
+    const float* cv_bias;       //   and its FP32 biases
     uint16_t* cv_feat;          // [R][J][N][16] BF16 features at d_j
     float* cv_dm;               // [R][J][N][8] discounted rebased martingale increments over interval j
     float* cv_pay;              // [R][N] discounted book value

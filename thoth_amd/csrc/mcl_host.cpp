@@ -555,6 +555,83 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price( con
     }
 }
 
+namespace thoth
+{
+double StudentT975( int df )
+{
+    // two-sided 95 % quantiles of Student's t (df 1..30), normal beyond
+    static const double q[] = { 12.706, 4.303, 3.182, 2.776, 2.571, 2.447, 2.365, 2.306, 2.262, 2.228,
+                                2.201,  2.179, 2.160, 2.145, 2.131, 2.120, 2.110, 2.101, 2.093, 2.086,
+                                2.080,  2.074, 2.069, 2.064, 2.060, 2.056, 2.052, 2.048, 2.045, 2.042 };
+    if ( df < 1 )
+        return 0.0;
+    if ( df <= 30 )
+        return q[df - 1];
+    return 1.96 + 2.4 / df;  // first Cornish-Fisher correction
+}
+}  // namespace thoth
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_rqmc( const thoth_mcl_request* req,
+                                                                              int32_t replicates, uint64_t rqmc_seed,
+                                                                              thoth_rqmc_result* res )
+{
+    if ( !req || !res )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    if ( replicates < 2 )
+        return thoth::SetError( THOTH_ERR_INVALID, "at least 2 replicates are needed for a confidence interval" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    try
+    {
+        int rc = thoth::EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        thoth::host::BookArrays A;
+        thoth::host::Grid g;
+        if ( ( rc = thoth::host::ResolveBaseBook( *req, A, g ) ) != THOTH_OK )
+            return rc;
+        thoth_mcl_sweep_cfg sc{};
+        sc.paths = req->mcl.paths;
+        sc.precision = req->precision;
+        sc.replicates = replicates;
+        sc.rqmc_seed = rqmc_seed;
+        thoth::DeviceBatch B;
+        if ( ( rc = thoth::PrepareBatch( 1, &A.book, &sc, B ) ) != THOTH_OK )
+            return rc;
+        std::vector<thoth_mcl_book_out> out( replicates );
+        if ( ( rc = thoth::RunBatch( B, out.data() ) ) != THOTH_OK )
+            return rc;
+        const int64_t N = req->mcl.paths;
+        double mean = 0, s1 = 0, s2 = 0;
+        for ( const auto& o : out )
+        {
+            mean += o.book_sum / N;
+            s1 += o.book_sum;
+            s2 += o.book_sum2;
+        }
+        mean /= replicates;
+        double v = 0;
+        for ( const auto& o : out )
+            v += ( o.book_sum / N - mean ) * ( o.book_sum / N - mean );
+        v /= ( replicates - 1 );
+        const double n = (double)N * replicates;
+        res->replicates = replicates;
+        res->paths = N;
+        res->mean = mean;
+        res->halfwidth = thoth::StudentT975( replicates - 1 ) * std::sqrt( v / replicates );
+        res->var = ( s2 - s1 * s1 / n ) / ( n - 1 );
+        res->sweep_ms = out[0].device_ms;
+        return THOTH_OK;
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
 extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_prepare( const thoth_mcl_request* req,
                                                                            thoth_mcl_plan* plan )
 {

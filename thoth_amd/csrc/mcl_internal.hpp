@@ -32,6 +32,7 @@ std::mutex& DeviceMutex();  // serialises every public entry point
 hipStream_t Stream();
 const uint32_t* DeviceDirections();
 int SetError( int code, const std::string& msg );
+double StudentT975( int df );  // two-sided 95 % quantile of Student's t
 struct DevBook;
 // A packed, uploaded batch of books ready to sweep (device-resident inputs).
 struct DeviceBatch
@@ -47,26 +48,37 @@ struct DeviceBatch
     uint32_t q0 = 0, nblocks = 0, first = 0, last = 0;  // q0: first 256-index block
     size_t lds_bytes = 0;
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
-    bool cv = false;  // the books carry control-variate sample buffers
+    int cv_mode = 0;  // 0 none, 1 control-variate samples, 2 in-loop hedge inference
     int64_t paths = 0;
     DeviceBatch() = default;
     DeviceBatch( const DeviceBatch& ) = delete;
     DeviceBatch& operator=( const DeviceBatch& ) = delete;
     ~DeviceBatch();
 };
-// Learned-CV data of an FP32 sweep: J hedge intervals, device output buffers
-// sized [replicates][J][N][16] / [replicates][J][N][8] / [replicates][N].
+// Learned control variate of an FP32 sweep over J hedge intervals.
+// mode 1 writes the training samples into device buffers sized
+// [replicates][J][N][16] / [replicates][J][N][8] / [replicates][N];
+// mode 2 evaluates the hedge network (BF16 A-operand image + FP32 biases,
+// device pointers) inside the stepping loop and accumulates P - CV.
 struct CvSpec
 {
+    int mode = 1;
     int J = 0;
     int64_t N = 0;
     uint16_t* feat = nullptr;
     float* dm = nullptr;
     float* pay = nullptr;
+    const uint16_t* wimg = nullptr;
+    const float* bias = nullptr;
+};
+// sums of P - CV per replicate (mode 2)
+struct CvSums
+{
+    double sum = 0, sum2 = 0;
 };
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
                   const CvSpec* cv = nullptr );
-int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out );
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
 int SweepBooks( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg,
                 thoth_mcl_book_out* out );
 

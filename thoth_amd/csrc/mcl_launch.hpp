@@ -43,7 +43,7 @@ bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, 
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
 bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts,
-                       bool cv = false );
+                       int cv_mode = 0 );
 void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bwd );
 void launch_cv_eval( hipStream_t s, const uint16_t* fwd, const float* bias, const uint16_t* feat, const float* dm,
                      const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks );

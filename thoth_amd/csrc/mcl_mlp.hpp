@@ -52,6 +52,16 @@ __device__ __forceinline__ short to_bf16( float x )
 
 __device__ __forceinline__ float from_bf16( short h ) { return __uint_as_float( ( (uint32_t)(uint16_t)h ) << 16 ); }
 
+__device__ __forceinline__ bf16x4 bits_bf16x4( uint2 w )
+{
+    bf16x4 r;
+    r[0] = (short)( w.x & 0xffffu );
+    r[1] = (short)( w.x >> 16 );
+    r[2] = (short)( w.y & 0xffffu );
+    r[3] = (short)( w.y >> 16 );
+    return r;
+}
+
 __device__ __forceinline__ bf16x4 pack_bf16( f32x4 v )
 {
     bf16x4 r;
@@ -129,6 +139,47 @@ __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bia
     for ( int kt = 0; kt < 4; kt++ )
         c = mfma16( weight_frag( wimg, 3, 0, kt, lane ), t.h[2][kt], c );
     t.out = c;
+}
+
+// forward of one tile keeping nothing for a backward pass: the output tile
+__device__ __forceinline__ f32x4 mlp_infer( const uint2* wimg, const float* bias, bf16x4 x, int lane )
+{
+    bf16x4 h0, h1, h2, h3;
+    {
+        f32x4 c0 = mfma16( weight_frag( wimg, 0, 0, 0, lane ), x, bias_frag( bias, 0, 0, lane ) );
+        f32x4 c1 = mfma16( weight_frag( wimg, 0, 1, 0, lane ), x, bias_frag( bias, 0, 1, lane ) );
+        f32x4 c2 = mfma16( weight_frag( wimg, 0, 2, 0, lane ), x, bias_frag( bias, 0, 2, lane ) );
+        f32x4 c3 = mfma16( weight_frag( wimg, 0, 3, 0, lane ), x, bias_frag( bias, 0, 3, lane ) );
+        h0 = pack_bf16( f32x4{ silu( c0[0] ), silu( c0[1] ), silu( c0[2] ), silu( c0[3] ) } );
+        h1 = pack_bf16( f32x4{ silu( c1[0] ), silu( c1[1] ), silu( c1[2] ), silu( c1[3] ) } );
+        h2 = pack_bf16( f32x4{ silu( c2[0] ), silu( c2[1] ), silu( c2[2] ), silu( c2[3] ) } );
+        h3 = pack_bf16( f32x4{ silu( c3[0] ), silu( c3[1] ), silu( c3[2] ), silu( c3[3] ) } );
+    }
+    // hidden layers 2 and 3: one output tile at a time (fewest live registers)
+    auto unit_tile = [&]( int L, int ot ) __attribute__( ( always_inline ) ) -> bf16x4
+    {
+        f32x4 c = bias_frag( bias, L, ot, lane );
+        c = mfma16( weight_frag( wimg, L, ot, 0, lane ), h0, c );
+        c = mfma16( weight_frag( wimg, L, ot, 1, lane ), h1, c );
+        c = mfma16( weight_frag( wimg, L, ot, 2, lane ), h2, c );
+        c = mfma16( weight_frag( wimg, L, ot, 3, lane ), h3, c );
+        return pack_bf16( f32x4{ silu( c[0] ), silu( c[1] ), silu( c[2] ), silu( c[3] ) } );
+    };
+#pragma unroll
+    for ( int L = 1; L < 3; L++ )
+    {
+        const bf16x4 n0 = unit_tile( L, 0 ), n1 = unit_tile( L, 1 ), n2 = unit_tile( L, 2 ), n3 = unit_tile( L, 3 );
+        h0 = n0;
+        h1 = n1;
+        h2 = n2;
+        h3 = n3;
+    }
+    f32x4 o = bias_frag( bias, 3, 0, lane );
+    o = mfma16( weight_frag( wimg, 3, 0, 0, lane ), h0, o );
+    o = mfma16( weight_frag( wimg, 3, 0, 1, lane ), h1, o );
+    o = mfma16( weight_frag( wimg, 3, 0, 2, lane ), h2, o );
+    o = mfma16( weight_frag( wimg, 3, 0, 3, lane ), h3, o );
+    return o;
 }
 
 // hedge-ratio . increments of the tile's 16 samples: lane l (l < 16) returns

@@ -15,6 +15,7 @@
 #include <type_traits>
 
 #include "mcl_device.hpp"
+#include "mcl_mlp.hpp"
 
 namespace thoth
 {
@@ -195,9 +196,18 @@ __device__ __forceinline__ double qe_f64( double v, double dt, double z, double 
     return ( u <= p ) ? 0.0 : log( ( 1.0 - p ) / ( 1.0 - u ) ) / beta;
 }
 
+// Parameters of the QE draw actually taken: quadratic branch v' = a (sqrt(b2)
+// + Z)^2, or exponential branch (mass p = 1 - omp at 0, else Exp(omp / m)).
+struct QeDraw
+{
+    bool quad;
+    float a, b2;   // quadratic
+    float omp, m;  // exponential
+};
+
 // Same scheme in FP32 from host-precomputed step constants.  1-u is formed
 // directly as Phi(-z) so the exponential tail keeps its precision.
-__device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc )
+__device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc, QeDraw* draw = nullptr )
 {
     const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
     const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
@@ -208,11 +218,23 @@ __device__ __forceinline__ float qe_f32( float v, float z, const float* __restri
         const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
         const float a = __fdividef( m, 1.0f + b2 );
         const float bz = __builtin_amdgcn_sqrtf( b2 ) + z;
+        if ( draw )
+        {
+            draw->quad = true;
+            draw->a = a;
+            draw->b2 = b2;
+        }
         return a * bz * bz;
     }
     // p = (psi-1)/(psi+1), beta = (1-p)/m; v = log((1-p)/(1-u)) / beta
     const float inv = __fdividef( 1.0f, psi + 1.0f );
     const float one_minus_p = 2.0f * inv;
+    if ( draw )
+    {
+        draw->quad = false;
+        draw->omp = one_minus_p;
+        draw->m = m;
+    }
     const float one_minus_u = 0.5f * erfcf( z * 0.70710678118654752f );
     return ( one_minus_u >= one_minus_p ) ? 0.0f
                                           : __logf( __fdividef( one_minus_p, one_minus_u ) ) * m * 0.5f * ( psi + 1.0f );
@@ -227,37 +249,76 @@ __device__ __forceinline__ R wave_sum( R x )
     return x;
 }
 
-// Log of E[exp(A v') | v] - 1 for the QE draw v' from v (Andersen 2008, the
-// moment generating functions behind his martingale correction): quadratic
-// branch v' = a (b + Z)^2 -> exp(A b^2 a / (1 - 2 A a)) / sqrt(1 - 2 A a);
-// exponential branch (mass p at 0, else Exp(beta)) -> p + (1 - p) beta /
-// (beta - A).  Returns false where the moment does not exist.
-__device__ __forceinline__ bool qe_mgf_m1( float v, float A, const float* __restrict__ sc, float& out )
+// expm1 and log1p for the small arguments of the hedge increments: Taylor
+// polynomials where they are exact to FP32 rounding (truncation below 3e-11
+// relative), the library functions elsewhere.
+__device__ __forceinline__ float expm1_small( float x )
 {
-    const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
-    const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
-    const float psi = ( m > 0.0f ) ? s2 / ( m * m ) : 0.0f;
-    if ( psi <= 1.5f )
+    if ( fabsf( x ) < 0.125f )
     {
-        const float c = 2.0f / psi;
-        const float b2 = c - 1.0f + sqrtf( c * ( c - 1.0f ) );
-        const float a = m / ( 1.0f + b2 );
-        const float q = 1.0f - 2.0f * A * a;
+        float p = 1.0f / 5040.0f;
+        p = fmaf( p, x, 1.0f / 720.0f );
+        p = fmaf( p, x, 1.0f / 120.0f );
+        p = fmaf( p, x, 1.0f / 24.0f );
+        p = fmaf( p, x, 1.0f / 6.0f );
+        p = fmaf( p, x, 0.5f );
+        p = fmaf( p, x, 1.0f );
+        return p * x;
+    }
+    return expm1f( x );
+}
+
+__device__ __forceinline__ float log1p_small( float x )
+{
+    if ( fabsf( x ) < 0.0625f )
+    {
+        float p = -1.0f / 8.0f;
+        p = fmaf( p, x, 1.0f / 7.0f );
+        p = fmaf( p, x, -1.0f / 6.0f );
+        p = fmaf( p, x, 1.0f / 5.0f );
+        p = fmaf( p, x, -1.0f / 4.0f );
+        p = fmaf( p, x, 1.0f / 3.0f );
+        p = fmaf( p, x, -0.5f );
+        p = fmaf( p, x, 1.0f );
+        return p * x;
+    }
+    return log1pf( x );
+}
+
+// E[exp(A v') | v] - 1 for the QE draw v' (Andersen 2008, the moment
+// generating functions behind his martingale correction), from the draw's own
+// FP32 parameters: quadratic branch exp(A b2 a / (1 - 2 A a)) / sqrt(1 - 2 A a);
+// exponential branch p + (1 - p) beta / (beta - A), beta = (1 - p) / m.
+// Returns false where the moment does not exist.
+__device__ __forceinline__ bool qe_mgf_m1( const QeDraw& d, float A, float& out )
+{
+    if ( d.quad )
+    {
+        const float q = 1.0f - 2.0f * A * d.a;
         if ( !( q > 0.0f ) )
             return false;
-        out = expm1f( A * b2 * a / q - 0.5f * log1pf( -2.0f * A * a ) );
+        out = expm1_small( __fdividef( A * d.b2 * d.a, q ) - 0.5f * log1p_small( -2.0f * A * d.a ) );
         return true;
     }
-    const float p = ( psi - 1.0f ) / ( psi + 1.0f );
-    const float beta = ( 1.0f - p ) / m;
-    if ( !( A < beta ) )
+    const float Am = A * d.m;
+    if ( !( Am < d.omp ) )
         return false;
-    out = ( 1.0f - p ) * A / ( beta - A );
+    out = __fdividef( d.omp * Am, d.omp - Am );
     return true;
 }
 
-template <class R, int NA, int DEPTH, bool CV = false>
-__global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restrict__ books,
+// LDS of the in-loop hedge inference (after the Sobol table and leverage
+// layers, 16-byte aligned): the BF16 A-operand image of the network, its
+// biases, and per wave a [64 paths][16] BF16 feature block and a [64][8]
+// FP32 hedge-ratio block for the lane <-> MFMA-layout exchange.
+static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 4, "hedge-network LDS area" );
+
+// CV: 0 plain sweep; 1 also writes the control-variate training samples;
+// 2 evaluates the hedge network at every hedge date inside the stepping loop
+// and accumulates P - CV in an extra result slot after the contracts.
+template <class R, int NA, int DEPTH, int CV = 0>
+__global__ void __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( CV == 2 ? 2 : 1 ) ) )
+sweep_kernel( const DevBook* __restrict__ books,
                                                          const uint32_t* __restrict__ V, uint32_t bq0,
                                                          uint32_t first_idx, uint32_t last_idx,
                                                          const uint32_t* __restrict__ shifts )
@@ -284,10 +345,27 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     const int nd = B.n_dates;
     const int nc = B.n_contracts;
 
-    // dynamic LDS: [m*F Sobol block table][FP32 LSV: 2 x NA x lev_points leverage layers]
+    // dynamic LDS: [m*F Sobol block table][FP32 LSV: 2 x NA x lev_points leverage
+    // layers][in-loop hedge inference area, 16-byte aligned]
     extern __shared__ uint32_t lds[];
     uint32_t* tab = lds;
     float* levs = reinterpret_cast<float*>( lds + m * F );
+    constexpr bool cv_on = CV != 0 && !F64;
+    constexpr bool cv_samples = CV == 1 && !F64;
+    constexpr bool cv_infer = CV == 2 && !F64;
+    [[maybe_unused]] char* cvl = nullptr;
+    if constexpr ( cv_infer )
+    {
+        const int lev_words = B.lev_points > 0 ? 2 * NA * B.lev_points : 0;
+        cvl = reinterpret_cast<char*>( lds + ( ( m * F + lev_words + 3 ) & ~3 ) );
+        const uint32_t* src = B.cv_wimg;
+        uint32_t* dst = reinterpret_cast<uint32_t*>( cvl );
+        for ( int k = threadIdx.x; k < kCvImgBytes / 4; k += kBlock )
+            dst[k] = src[k];
+        float* bd = reinterpret_cast<float*>( cvl + kCvImgBytes );
+        for ( int k = threadIdx.x; k < kMlpBSize; k += kBlock )
+            bd[k] = B.cv_bias[k];
+    }
     __shared__ double acc[kWavesPerBlock][kMaxContracts + 1][2];
     for ( int k = threadIdx.x; k < kWavesPerBlock * ( kMaxContracts + 1 ) * 2; k += kBlock )
         ( &acc[0][0][0] )[k] = 0.0;
@@ -305,7 +383,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     // FP32 LSV books stage the leverage layer of the current step in LDS
     const bool lev_lds = !F64 && B.lev_points > 0;
     const int P = B.lev_points;
-    auto stage_layer = [&]( int i )
+    auto stage_layer = [&]( int i ) __attribute__( ( always_inline ) )
     {
         float* dst = levs + ( i & 1 ) * NA * P;
         for ( int a = 0; a < na; a++ )
@@ -358,19 +436,22 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     uint32_t hit = 0;
     double book = 0.0;
 
-    // ---- learned control variate data (FP32 only)
-    constexpr bool cv_on = CV && !F64;
+    // ---- learned control variate (FP32 only)
     const int64_t pidx = (int64_t)idx - (int64_t)first_idx;
-    const bool cv_store = cv_on && valid && pidx >= 0 && pidx < B.cv_N;
+    const bool cv_store = cv_samples && valid && pidx >= 0 && pidx < B.cv_N;
     float dmacc[NA];
+    float hr[NA];     // hedge ratios of the current interval (in-loop inference)
+    float cvacc = 0.0f;
 #pragma unroll
     for ( int a = 0; a < NA; a++ )
-        dmacc[a] = 0.0f;
-    auto cv_features = [&]( int i, int j )
     {
-        // standardised features, BF16: 5 log-moneyness / (vol_ref sqrt(T)),
-        // 5 x 2 (sqrt(v) / vol_ref - 1) (0 for Black-Scholes), 2 t / T - 1
-        uint32_t w[8];
+        dmacc[a] = 0.0f;
+        hr[a] = 0.0f;
+    }
+    // standardised features, BF16 pairs: 5 log-moneyness / (vol_ref sqrt(T)),
+    // 5 x 2 (sqrt(v) / vol_ref - 1) (0 for Black-Scholes), 2 t / T - 1
+    auto cv_feature_words = [&]( int i, uint32_t* w ) __attribute__( ( always_inline ) )
+    {
         float f[16];
 #pragma unroll
         for ( int k = 0; k < 16; k++ )
@@ -393,12 +474,56 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
             hi = ( hi + 0x7fffu + ( ( hi >> 16 ) & 1u ) ) >> 16;
             w[k] = lo | ( hi << 16 );
         }
+    };
+    auto cv_features = [&]( int i, int j ) __attribute__( ( always_inline ) )
+    {
+        uint32_t w[8];
+        cv_feature_words( i, w );
         uint4* dst = reinterpret_cast<uint4*>( B.cv_feat + ( ( (size_t)blockIdx.z * B.cv_J + j ) * B.cv_N + pidx ) * 16 );
         dst[0] = uint4{ w[0], w[1], w[2], w[3] };
         dst[1] = uint4{ w[4], w[5], w[6], w[7] };
     };
-    auto cv_date = [&]( int i )
+    // in-loop inference: the wave's 64 paths as 4 MFMA sample tiles; features
+    // go lane -> LDS -> B operands, hedge ratios D layout -> LDS -> lanes
+    auto hedge_eval = [&]( int i ) __attribute__( ( always_inline ) )
     {
+        uint32_t w[8];
+        cv_feature_words( i, w );
+        char* wl = cvl + kCvImgBytes + kCvBiasBytes + wib * kCvWaveBytes;
+        uint4* frow = reinterpret_cast<uint4*>( wl ) + 2 * lane;
+        frow[0] = uint4{ w[0], w[1], w[2], w[3] };
+        frow[1] = uint4{ w[4], w[5], w[6], w[7] };
+        float* hbuf = reinterpret_cast<float*>( wl + 64 * 16 * 2 );
+        wave_lds_sync();
+        const uint2* wimg = reinterpret_cast<const uint2*>( cvl );
+        const float* wb = reinterpret_cast<const float*>( cvl + kCvImgBytes );
+#pragma unroll 1
+        for ( int t = 0; t < 4; t++ )
+        {
+            const uint2 xw = *reinterpret_cast<const uint2*>(
+                reinterpret_cast<const uint16_t*>( wl ) + ( 16 * t + ( lane & 15 ) ) * 16 + 4 * ( lane >> 4 ) );
+            const f32x4 out = mlp_infer( wimg, wb, bits_bf16x4( xw ), lane );
+            const int row = 16 * t + ( lane & 15 );
+            if ( lane < 16 )
+                *reinterpret_cast<float4*>( hbuf + row * 8 ) = float4{ out[0], out[1], out[2], out[3] };
+            else if ( lane < 32 )
+                hbuf[row * 8 + 4] = out[0];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for ( int a = 0; a < NA; a++ )
+            if ( a < 5 )
+                hr[a] = hbuf[lane * 8 + a];
+        wave_lds_sync();
+    };
+    auto cv_date = [&]( int i ) __attribute__( ( always_inline ) )
+    {
+        if constexpr ( cv_infer )
+        {
+            if ( B.cv_start[i] >= 0 )
+                hedge_eval( i );
+            return;
+        }
         const int je = B.cv_end[i];
         if ( je >= 0 )
         {
@@ -422,7 +547,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
             cv_features( i, js );
     };
 
-    auto udl_value = [&]( int u ) -> R
+    auto udl_value = [&]( int u ) __attribute__( ( always_inline ) ) -> R
     {
         if ( u >= 0 )
             return S[__builtin_amdgcn_readfirstlane( u )];
@@ -438,7 +563,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
         return x;
     };
 
-    auto observe = [&]( int i )
+    auto observe = [&]( int i ) __attribute__( ( always_inline ) )
     {
         uint32_t mon = B.mon_mask[i];
         while ( mon )
@@ -482,9 +607,8 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
     };
 
     observe( 0 );
-    if constexpr ( !F64 )
-        if ( cv_on )
-            cv_date( 0 );
+    if constexpr ( cv_on )
+        cv_date( 0 );
 
     for ( int p = 0; p < m; p++ )
     {
@@ -607,8 +731,11 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
                     {
                         // M = S e^{det} (e^{b z} - e^{b^2/2}): zero conditional mean
                         const float M = sold * __expf( cd - sc[SC_HALF_V2DT] ) *
-                                        ( expm1f( sc[SC_VOL] * dw ) - expm1f( sc[SC_HALF_V2DT] ) );
-                        dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
+                                        ( expm1_small( sc[SC_VOL] * dw ) - expm1_small( sc[SC_HALF_V2DT] ) );
+                        if constexpr ( cv_infer )
+                            cvacc = fmaf( hr[a] * B.cv_scale[(size_t)a * nd + i], M, cvacc );
+                        else
+                            dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
                     }
                 }
                 continue;
@@ -618,8 +745,11 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
             const R zv = nz[vf];
             const R vp = Var[a];
             R vc;
+            [[maybe_unused]] QeDraw draw;
             if constexpr ( F64 )
                 vc = qe_f64( vp, B.dt[i], zv, B.asset[a].kappa, B.asset[a].theta, B.asset[a].xi );
+            else if constexpr ( cv_on )
+                vc = qe_f32( vp, zv, sc, &draw );
             else
                 vc = qe_f32( vp, zv, sc );
             Var[a] = vc;
@@ -717,30 +847,43 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
                     {
                         const float half_l2k3 = 0.5f * L * L * sc[SC_K3];
                         const float Acoef = L * sc[SC_A1] - L * L * sc[SC_A2] + L * sc[SC_RX] + half_l2k3;
-                        if ( qe_mgf_m1( vp, Acoef, sc, mgf ) )
+                        if ( qe_mgf_m1( draw, Acoef, mgf ) )
                         {
                             const float c0 = cd + L * sc[SC_K0] + ( L * sc[SC_A1] - L * L * sc[SC_A2] - L * sc[SC_RX] +
                                                                     half_l2k3 ) * vp;
                             const float X = Acoef * vc + bvol * z - 0.5f * bvol * bvol;
-                            M = sold * __expf( c0 ) * ( expm1f( X ) - mgf );
+                            M = sold * __expf( c0 ) * ( expm1_small( X ) - mgf );
                         }
                         else
-                            M = sold * __expf( det ) * ( expm1f( bvol * z ) - expm1f( 0.5f * bvol * bvol ) );
+                            M = sold * __expf( det ) * ( expm1_small( bvol * z ) - expm1_small( 0.5f * bvol * bvol ) );
                     }
                     else
-                        M = sold * __expf( det ) * ( expm1f( bvol * z ) - expm1f( 0.5f * bvol * bvol ) );
-                    dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
+                        M = sold * __expf( det ) * ( expm1_small( bvol * z ) - expm1_small( 0.5f * bvol * bvol ) );
+                    if constexpr ( cv_infer )
+                        cvacc = fmaf( hr[a] * B.cv_scale[(size_t)a * nd + i], M, cvacc );
+                    else
+                        dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
                 }
             }
         }
         observe( i );
-        if constexpr ( !F64 )
-            if ( cv_on )
-                cv_date( i );
+        if constexpr ( cv_on )
+            cv_date( i );
     }
-    if constexpr ( !F64 )
+    if constexpr ( cv_samples )
         if ( cv_store )
             B.cv_pay[(size_t)blockIdx.z * B.cv_N + pidx] = (float)book;
+    if constexpr ( cv_infer )
+    {
+        const double y = valid ? book - (double)cvacc : 0.0;
+        const double s1 = wave_sum<double>( y );
+        const double s2 = wave_sum<double>( y * y );
+        if ( lane == 0 )
+        {
+            acc[wib][1 + nc][0] = s1;
+            acc[wib][1 + nc][1] = s2;
+        }
+    }
 
     // ---- book value reduction
     if ( !valid )
@@ -755,7 +898,7 @@ __global__ void __launch_bounds__( kBlock ) sweep_kernel( const DevBook* __restr
         }
     }
     __syncthreads();
-    const int ne = ( 1 + nc ) * 2;
+    const int ne = ( 1 + nc + ( cv_infer ? 1 : 0 ) ) * 2;
     if ( (int)threadIdx.x < ne )
     {
         const int e = threadIdx.x >> 1, mom = threadIdx.x & 1;

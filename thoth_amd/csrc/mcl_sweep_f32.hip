@@ -12,26 +12,31 @@ static void launch_one( dim3 grid, hipStream_t s, size_t lds, const DevBook* boo
     hipLaunchKernelGGL( ( sweep_kernel<float, NA, D> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first, last, shifts );
 }
 
-// the same sweep writing the control-variate samples (hedge features and
-// increments per hedge date, discounted payoff per path); at most 5 equities
-template <int NA, int D>
-static void launch_one_cv( dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V, uint32_t bq0,
-                           uint32_t first, uint32_t last, const uint32_t* shifts )
+// the same sweep with the learned control variate (at most 5 equities):
+// CV = 1 writes the training samples, CV = 2 runs the hedge network in the loop
+template <int CV>
+struct CvLaunch
 {
-    if constexpr ( NA <= 5 )
-        hipLaunchKernelGGL( ( sweep_kernel<float, NA, D, true> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first,
-                            last, shifts );
-}
+    template <int NA, int D>
+    static void one( dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V, uint32_t bq0,
+                     uint32_t first, uint32_t last, const uint32_t* shifts )
+    {
+        if constexpr ( NA <= 5 )
+            hipLaunchKernelGGL( ( sweep_kernel<float, NA, D, CV> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first,
+                                last, shifts );
+    }
+};
 
 bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
-                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts, bool cv )
+                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts,
+                       int cv_mode )
 {
-    if ( cv )
-    {
-        if ( na > 5 )
-            return false;
-        THOTH_DISPATCH_SWEEP( launch_one_cv, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
-    }
+    if ( cv_mode != 0 && na > 5 )
+        return false;
+    if ( cv_mode == 1 )
+        THOTH_DISPATCH_SWEEP( CvLaunch<1>::one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
+    if ( cv_mode == 2 )
+        THOTH_DISPATCH_SWEEP( CvLaunch<2>::one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
 }
 }  // namespace thoth

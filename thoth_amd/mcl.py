@@ -49,6 +49,7 @@ def lib():
         L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
                                                    P(C.c_double), P(C.c_int64), P(C.c_double)]
+        L.thoth_mcl_price_rqmc.argtypes = [P(abi.Request), C.c_int32, C.c_uint64, P(abi.RqmcResult)]
         L.thoth_cv_train.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), P(abi.CvTrainReport)]
         L.thoth_cv_price.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), C.c_int32, C.c_uint64,
                                      P(abi.CvResult)]
@@ -94,6 +95,14 @@ def price(built):
     """Price a `book.BuiltRequest`; returns the filled `abi.Result`."""
     res = built.result()
     _check(lib().thoth_mcl_price(C.byref(built.req), C.byref(res)))
+    return res
+
+
+def price_rqmc(built, replicates=16, rqmc_seed=1):
+    """Randomised-QMC premium of the base book with a 95 % confidence interval
+    (built.req.mcl.paths paths per replicate); returns abi.RqmcResult."""
+    res = abi.RqmcResult()
+    _check(lib().thoth_mcl_price_rqmc(C.byref(built.req), replicates, rqmc_seed, C.byref(res)))
     return res
 
 
@@ -227,11 +236,12 @@ def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_see
 
 # ---- learned martingale control variate (include/thoth_mcl_cv.h) ----------
 
-def cv_config(hedge_dates=73, train_paths=1 << 18, train_sobol_skip=1 << 24, adam_steps=2000, batch_paths=1024,
-              learning_rate=1e-3, init_seed=7):
+def cv_config(hedge_dates=73, train_paths=1 << 18, train_sobol_skip=1 << 24, adam_steps=2000, batch_paths=4096,
+              learning_rate=1e-3, init_seed=7, eval_mode=abi.CV_IN_LOOP):
     """Training/evaluation settings; the training paths come from a Sobol block
     (points train_sobol_skip+1 ..) disjoint from the evaluation replicates."""
-    return abi.CvConfig(hedge_dates, train_paths, train_sobol_skip, adam_steps, batch_paths, learning_rate, init_seed)
+    return abi.CvConfig(hedge_dates, train_paths, train_sobol_skip, adam_steps, batch_paths, learning_rate, init_seed,
+                        eval_mode)
 
 
 def cv_train(built, cfg):
