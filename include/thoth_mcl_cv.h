@@ -1,18 +1,20 @@
 /*
  * thoth_mcl_cv.h -- learned martingale control variate (no reference
- * counterpart: the reference engine has no control variates, SURVEY of
- * pricer/src/tasks/pricer_mcl*.cpp).
+ * counterpart: the reference engine, pricer/src/tasks/pricer_mcl*.cpp, has no
+ * control variates).
  *
- * A small MLP f(t, log S, sqrt v) -> 5 hedge ratios is trained on paths the
+ * A small MLP f(t, log S, v) -> 5 hedge ratios is trained on paths the
  * engine itself generates (FP32 sweep of the book) and then used as the
  * control variate
  *     CV = sum_j f(x_j) . sum_{i in interval j} df(t_i) (100/S0) M_i,
- *     M_i = S_{i-1} e^{det_i} (e^{b_i z_i} - e^{b_i^2/2}),
- * which has zero mean under the simulated scheme for ANY weights, so
- * P - CV is an unbiased estimator of the premium P.  Networks: 11 -> 64 -> 64
+ *     M_i = S_i - E[S_i | path up to t_{i-1}]   (per equity),
+ * the conditional mean taken under the simulated scheme itself (Black-Scholes
+ * step: lognormal mean; Andersen QE step: the moment generating function of
+ * the QE variance draw).  CV has zero mean for ANY weights, so P - CV is an
+ * unbiased estimator of the premium P.  Networks: 11 -> 64 -> 64
  * -> 64 -> 5, SiLU, BF16 operands with FP32 accumulation on the matrix cores,
  * FP32 master weights, Adam.  Plain Monte-Carlo is reported beside every
- * control-variate estimate as the judge of bias and variance reduction.
+ * control-variate estimate as the yardstick of bias and variance reduction.
  */
 #ifndef THOTH_MCL_CV_H
 #define THOTH_MCL_CV_H

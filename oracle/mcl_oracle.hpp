@@ -9,8 +9,8 @@
 // order of floating-point operations, and the library is compiled with
 // -ffp-contract=off (the reference Release build forms no FMAs).
 //
-// Pinning (what checks each part independently of this restatement) is listed
-// per function in DESIGN.md and exercised by tests/test_oracle.py.
+// What checks each part independently of this restatement (published vectors,
+// closed forms, invariants) is exercised by tests/test_oracle.py.
 #pragma once
 #include <cstdint>
 #include <string>

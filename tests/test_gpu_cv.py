@@ -2,7 +2,7 @@
 loss and gradient against the CPU oracle (oracle/mlp_oracle.py), the zero-mean
 property of the hedge increments (unbiasedness for any network), and the
 variance a trained network removes on the 5-asset LSV worst-of basket,
-judged against plain Monte-Carlo on the same randomised-QMC replicates."""
+measured against plain Monte-Carlo on the same randomised-QMC replicates."""
 import sys
 from pathlib import Path
 

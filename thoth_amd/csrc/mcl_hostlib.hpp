@@ -1,7 +1,8 @@
 // mcl_hostlib.hpp -- market model of the host engine: curves, equities with
 // their bs / heston / lsv volatility, the SABR target surface and its Dupire
 // local volatility, the diffusion grid, the Cholesky factor and the LSV
-// leverage calibration.  All FP64 (the north-star keeps these steps in FP64).
+// leverage calibration.  All FP64 (the FP32 production sweep keeps these
+// steps in FP64).
 #pragma once
 #include <cmath>
 #include <cstdint>
