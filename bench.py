@@ -115,7 +115,18 @@ def main():
     ap.add_argument("--no-t1bp", action="store_true", help="skip the time-to-1bp comparison")
     ap.add_argument("--t1bp-replicates", type=int, default=32)
     ap.add_argument("--t1bp-paths", type=int, default=1 << 14)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (calibrates rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n}))
+        return
     if args.gpus != 1:
         print("bench.py: this build is single-GPU (one MI355X); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)

@@ -44,7 +44,7 @@ struct Arena
 struct Offsets
 {
     size_t t, dt, sqrt_dt, bsq, binv, step_ops, op_b, op_left, op_w, op_wf, drift, sc, lev, levf, lnstep, offset,
-        invstep, mon, flow, cv_start, cv_end, cv_scale;
+        offsetf, invstep, mon, flow, cv_start, cv_end, cv_scale;
 };
 
 std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
@@ -182,7 +182,7 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
     of.sc = ar.put( sc );
     // leverage layers
     db.lev_points = 0;
-    of.lev = of.levf = of.lnstep = of.offset = of.invstep = 0;
+    of.lev = of.levf = of.lnstep = of.offset = of.offsetf = of.invstep = 0;
     if ( any_lsv )
     {
         const int P = in.lev_points;
@@ -192,16 +192,18 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
         const size_t nl = (size_t)na * nd;
         std::vector<float> levf( in.lev_values, in.lev_values + nl * P );
         std::vector<double> off( nl );
-        std::vector<float> inv( nl );
+        std::vector<float> offf( nl ), inv( nl );
         for ( size_t k = 0; k < nl; k++ )
         {
             off[k] = (double)in.lev_offset[k];
+            offf[k] = (float)in.lev_offset[k];
             inv[k] = in.lev_ln_step[k] > 0 ? (float)( 1.0 / in.lev_ln_step[k] ) : 0.0f;
         }
         of.lev = ar.put( in.lev_values, nl * P );
         of.levf = ar.put( levf );
         of.lnstep = ar.put( in.lev_ln_step, nl );
         of.offset = ar.put( off );
+        of.offsetf = ar.put( offf );
         of.invstep = ar.put( inv );
     }
     // rainbows (rainbow_node.cpp:14-27)
@@ -379,15 +381,18 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
         B.na_max = std::max( B.na_max, B.db[b].n_assets );
     }
-    // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV)
-    // two leverage layers per equity of the kernel instantiation
+    // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV) a
+    // ring of 2G leverage layers per equity of the kernel instantiation (G = 4,
+    // or 2 when the hedge network also needs its LDS area: two blocks per CU)
     const int na_inst = B.na_max <= 1 ? 1 : B.na_max <= 2 ? 2 : B.na_max <= 5 ? 5 : 8;
+    const int lev_group = B.cv_mode == 2 ? 2 : 4;
     size_t dims = 0, lev = 0;
     for ( int b = 0; b < n_books; b++ )
     {
+        B.db[b].lev_group = lev_group;
         dims = std::max( dims, (size_t)B.db[b].m * B.db[b].factors );
         if ( cfg->precision == 1 && B.db[b].lev_points > 0 )
-            lev = std::max( lev, (size_t)2 * na_inst * B.db[b].lev_points );
+            lev = std::max( lev, (size_t)2 * lev_group * na_inst * B.db[b].lev_points );
     }
     B.lds_bytes = ( dims + lev ) * sizeof( uint32_t );
     if ( B.cv_mode == 2 )  // hedge-network area, 16-byte aligned (sweep_kernel's layout)
@@ -459,6 +464,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.levf = At<float>( dbase, o.levf );
             d.lev_ln_step = At<double>( dbase, o.lnstep );
             d.lev_offset = At<double>( dbase, o.offset );
+            d.lev_offset_f = At<float>( dbase, o.offsetf );
             d.lev_inv_step = At<float>( dbase, o.invstep );
         }
         d.mon_mask = At<uint32_t>( dbase, o.mon );

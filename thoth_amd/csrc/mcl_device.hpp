@@ -55,6 +55,7 @@ struct DevBook
     // grid
     int n_dates, m, n_assets, factors;
     int n_contracts, n_rainbows, corr_stoch, lev_points;
+    int lev_group;        // FP32 LSV: leverage layers staged in LDS per barrier (ring of 2 groups)
     int n_ops;
     double sd0;           // bridge endpoint std dev sqrt(t_m)
     float sd0f;
@@ -78,6 +79,7 @@ struct DevBook
     const float* levf;
     const double* lev_ln_step;  // [n_assets * n_dates]
     const double* lev_offset;   // [n_assets * n_dates] (integral values)
+    const float* lev_offset_f;  // FP32 copy (exact: |offset| < 2^24)
     const float* lev_inv_step;  // [n_assets * n_dates] 1 / ln_step
     // payoffs
     int c_kind[kMaxContracts], c_call[kMaxContracts], c_udl[kMaxContracts];

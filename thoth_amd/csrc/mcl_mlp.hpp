@@ -72,11 +72,11 @@ __device__ __forceinline__ bf16x4 pack_bf16( f32x4 v )
     return r;
 }
 
-__device__ __forceinline__ float silu( float z ) { return __fdividef( z, 1.0f + __expf( -z ) ); }
+__device__ __forceinline__ float silu( float z ) { return z * __builtin_amdgcn_rcpf( 1.0f + __expf( -z ) ); }
 
 __device__ __forceinline__ float dsilu( float z )
 {
-    const float s = __fdividef( 1.0f, 1.0f + __expf( -z ) );
+    const float s = __builtin_amdgcn_rcpf( 1.0f + __expf( -z ) );
     return s * ( 1.0f + z * ( 1.0f - s ) );
 }
 

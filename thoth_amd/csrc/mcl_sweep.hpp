@@ -22,6 +22,33 @@ namespace thoth
 
 __device__ __forceinline__ uint32_t gray32( uint32_t n ) { return n ^ ( n >> 1 ); }
 
+// FP32 hardware transcendentals of the production path (v_rcp_f32, v_log_f32:
+// about 1 ulp on the normal range, which is all these arguments take)
+__device__ __forceinline__ float fast_rcp( float x ) { return __builtin_amdgcn_rcpf( x ); }
+__device__ __forceinline__ float fast_div( float a, float b ) { return a * __builtin_amdgcn_rcpf( b ); }
+__device__ __forceinline__ float fast_log( float x ) { return __builtin_amdgcn_logf( x ) * 0.693147180559945309f; }
+
+// erfc(x) by the Chebyshev-fitted form of Numerical Recipes in C (2nd ed.,
+// section 6.2, "erfcc"): fractional error below 1.2e-7 for every x, one
+// reciprocal and one exponential.
+__device__ __forceinline__ float erfc_fast( float x )
+{
+    const float z = fabsf( x );
+    const float t = fast_rcp( fmaf( 0.5f, z, 1.0f ) );
+    float p = 0.17087277f;
+    p = fmaf( p, t, -0.82215223f );
+    p = fmaf( p, t, 1.48851587f );
+    p = fmaf( p, t, -1.13520398f );
+    p = fmaf( p, t, 0.27886807f );
+    p = fmaf( p, t, -0.18628806f );
+    p = fmaf( p, t, 0.09678418f );
+    p = fmaf( p, t, 0.37409196f );
+    p = fmaf( p, t, 1.00002368f );
+    p = fmaf( p, t, -1.26551223f );
+    const float r = t * __expf( fmaf( -z, z, p ) );
+    return x >= 0.0f ? r : 2.0f - r;
+}
+
 // Integer Sobol coordinates for the 256 consecutive indices of one block,
 // aligned on 256.  The reference's sequential Gray-code recurrence
 // (sobol_generator.cpp:80-100) gives x_d(n) = XOR_{k in gray(n)} V[d][k], and for
@@ -40,7 +67,9 @@ struct SobolLane
 
     __device__ __forceinline__ uint32_t operator()( int d ) const
     {
-        const uint32_t* v = V + (size_t)d * 32;
+        // d is wave-uniform: the eight direction numbers come in through the
+        // scalar cache (one s_load_dwordx8) instead of per-lane vector loads
+        const uint32_t* v = V + (size_t)__builtin_amdgcn_readfirstlane( d ) * 32;
         uint32_t x = tab[d];
         x ^= ( gw & 0x20u ) ? v[5] : 0u;
         x ^= ( gw & 0x40u ) ? v[6] : 0u;
@@ -133,7 +162,7 @@ __device__ __forceinline__ float normal_f32( uint32_t x )
     x = x > 1u ? x : 1u;
     const float a = (float)x;
     const float b = (float)( 0u - x );
-    const float w = -__logf( a * b * 2.168404344971009e-19f );  // 2^-62
+    const float w = -fast_log( a * b * 2.168404344971009e-19f );  // 2^-62
     const float xs = (float)(int32_t)( x - 0x80000000u ) * 4.656612873077393e-10f;  // 2^-31
     float p;
     if ( w < 5.0f )
@@ -211,12 +240,12 @@ __device__ __forceinline__ float qe_f32( float v, float z, const float* __restri
 {
     const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
     const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
-    const float psi = ( m > 0.0f ) ? __fdividef( s2, m * m ) : 0.0f;
+    const float psi = ( m > 0.0f ) ? fast_div( s2, m * m ) : 0.0f;
     if ( psi <= 1.5f )
     {
-        const float c = __fdividef( 2.0f, psi );
+        const float c = fast_div( 2.0f, psi );
         const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
-        const float a = __fdividef( m, 1.0f + b2 );
+        const float a = fast_div( m, 1.0f + b2 );
         const float bz = __builtin_amdgcn_sqrtf( b2 ) + z;
         if ( draw )
         {
@@ -227,7 +256,7 @@ __device__ __forceinline__ float qe_f32( float v, float z, const float* __restri
         return a * bz * bz;
     }
     // p = (psi-1)/(psi+1), beta = (1-p)/m; v = log((1-p)/(1-u)) / beta
-    const float inv = __fdividef( 1.0f, psi + 1.0f );
+    const float inv = fast_rcp( psi + 1.0f );
     const float one_minus_p = 2.0f * inv;
     if ( draw )
     {
@@ -235,9 +264,9 @@ __device__ __forceinline__ float qe_f32( float v, float z, const float* __restri
         draw->omp = one_minus_p;
         draw->m = m;
     }
-    const float one_minus_u = 0.5f * erfcf( z * 0.70710678118654752f );
+    const float one_minus_u = 0.5f * erfc_fast( z * 0.70710678118654752f );
     return ( one_minus_u >= one_minus_p ) ? 0.0f
-                                          : __logf( __fdividef( one_minus_p, one_minus_u ) ) * m * 0.5f * ( psi + 1.0f );
+                                          : fast_log( fast_div( one_minus_p, one_minus_u ) ) * m * 0.5f * ( psi + 1.0f );
 }
 
 template <class R>
@@ -297,13 +326,13 @@ __device__ __forceinline__ bool qe_mgf_m1( const QeDraw& d, float A, float& out 
         const float q = 1.0f - 2.0f * A * d.a;
         if ( !( q > 0.0f ) )
             return false;
-        out = expm1_small( __fdividef( A * d.b2 * d.a, q ) - 0.5f * log1p_small( -2.0f * A * d.a ) );
+        out = expm1_small( fast_div( A * d.b2 * d.a, q ) - 0.5f * log1p_small( -2.0f * A * d.a ) );
         return true;
     }
     const float Am = A * d.m;
     if ( !( Am < d.omp ) )
         return false;
-    out = __fdividef( d.omp * Am, d.omp - Am );
+    out = fast_div( d.omp * Am, d.omp - Am );
     return true;
 }
 
@@ -356,7 +385,7 @@ sweep_kernel( const DevBook* __restrict__ books,
     [[maybe_unused]] char* cvl = nullptr;
     if constexpr ( cv_infer )
     {
-        const int lev_words = B.lev_points > 0 ? 2 * NA * B.lev_points : 0;
+        const int lev_words = B.lev_points > 0 ? 2 * B.lev_group * NA * B.lev_points : 0;
         cvl = reinterpret_cast<char*>( lds + ( ( m * F + lev_words + 3 ) & ~3 ) );
         const uint32_t* src = B.cv_wimg;
         uint32_t* dst = reinterpret_cast<uint32_t*>( cvl );
@@ -380,21 +409,28 @@ sweep_kernel( const DevBook* __restrict__ books,
         for ( int k = 0; k < 6; k++ )
             sob.m[k] = ( ( g >> k ) & 1u ) ? 0xffffffffu : 0u;
     }
-    // FP32 LSV books stage the leverage layer of the current step in LDS
+    // FP32 LSV books stage the leverage layers in LDS, G layers at a time into
+    // a ring of 2G slots (layer i in slot (i - 1) mod 2G): one barrier per G
+    // steps publishes the current group and frees the previous group's slots
     const bool lev_lds = !F64 && B.lev_points > 0;
     const int P = B.lev_points;
-    auto stage_layer = [&]( int i ) __attribute__( ( always_inline ) )
+    const int G = B.lev_group;
+    auto stage_group = [&]( int first ) __attribute__( ( always_inline ) )
     {
-        float* dst = levs + ( i & 1 ) * NA * P;
-        for ( int a = 0; a < na; a++ )
+        const int last = min( first + G - 1, m );
+        for ( int i = first; i <= last; i++ )
         {
-            const float* src = B.levf + ( (size_t)a * nd + i ) * P;
-            for ( int k = threadIdx.x; k < P; k += kBlock )
-                dst[a * P + k] = src[k];
+            float* dst = levs + ( ( i - 1 ) % ( 2 * G ) ) * NA * P;
+            for ( int a = 0; a < na; a++ )
+            {
+                const float* src = B.levf + ( (size_t)a * nd + i ) * P;
+                for ( int k = threadIdx.x; k < P; k += kBlock )
+                    dst[a * P + k] = src[k];
+            }
         }
     };
     if ( lev_lds && m >= 1 )
-        stage_layer( 1 );
+        stage_group( 1 );
     __syncthreads();
 
     // ---- bridge state: endpoint first (build index 0, path_generator.cpp:166-167)
@@ -460,7 +496,7 @@ sweep_kernel( const DevBook* __restrict__ books,
         for ( int a = 0; a < NA; a++ )
             if ( a < na && a < 5 )
             {
-                f[a] = ( __logf( (float)S[a] ) - B.cv_lnS0[a] ) * B.cv_ln_scale[a];
+                f[a] = ( fast_log( (float)S[a] ) - B.cv_lnS0[a] ) * B.cv_ln_scale[a];
                 if ( B.asset[a].model != 0 )
                     f[5 + a] = 2.0f * fmaf( __builtin_amdgcn_sqrtf( fmaxf( (float)Var[a], 0.0f ) ),
                                             B.cv_vol_scale[a], -1.0f );
@@ -610,16 +646,13 @@ sweep_kernel( const DevBook* __restrict__ books,
     if constexpr ( cv_on )
         cv_date( 0 );
 
-    for ( int p = 0; p < m; p++ )
+    // ---- bridge step p: build the pending points of step p (in-order tree
+    // walk), then pop W(t_{p+1}) as the normalised increment.  It depends on
+    // the Sobol points only, so it runs one step ahead of the equities and the
+    // scheduler overlaps it with their (path-dependent) arithmetic.
+    auto bridge_step = [&]( int p, facv& nz ) __attribute__( ( always_inline ) )
     {
         const int i = p + 1;
-        if ( lev_lds )
-        {
-            __syncthreads();  // layer i is visible, every wave is done with layer i-1
-            if ( i + 1 <= m )
-                stage_layer( i + 1 );
-        }
-        // ---- bridge: build the pending points of this step (in-order tree walk)
         const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
         for ( int o = o0; o < o1; o++ )
         {
@@ -654,7 +687,6 @@ sweep_kernel( const DevBook* __restrict__ books,
             sp++;
         }
         // ---- pop W at this date -> normalized increment (path_generator.cpp:188-196)
-        facv nz;
         {
             const int pos = __builtin_amdgcn_readfirstlane( sp >= 2 ? sp - 2 : 0 );
 #pragma unroll
@@ -676,6 +708,20 @@ sweep_kernel( const DevBook* __restrict__ books,
                 }
             }
             sp--;
+        }
+    };
+
+    facv nz;
+    if ( m >= 1 )
+        bridge_step( 0, nz );
+    for ( int p = 0; p < m; p++ )
+    {
+        const int i = p + 1;
+        if ( lev_lds && ( i - 1 ) % G == 0 )
+        {
+            __syncthreads();  // the group of layer i is visible, the previous group is free
+            if ( i + G <= m )
+                stage_group( i + G );
         }
         // ---- equities
 #pragma unroll
@@ -763,7 +809,7 @@ sweep_kernel( const DevBook* __restrict__ books,
                 if constexpr ( F64 )
                     f = log( S[a] ) / B.lev_ln_step[li] - B.lev_offset[li];
                 else
-                    f = __logf( S[a] ) * B.lev_inv_step[li] - (float)B.lev_offset[li];
+                    f = fast_log( S[a] ) * B.lev_inv_step[li] - B.lev_offset_f[li];
                 if constexpr ( F64 )
                 {
                     const double* lv = B.lev + li * B.lev_points;
@@ -780,7 +826,7 @@ sweep_kernel( const DevBook* __restrict__ books,
                 }
                 else
                 {
-                    const float* lv = levs + ( i & 1 ) * NA * P + a * P;
+                    const float* lv = levs + ( ( i - 1 ) % ( 2 * G ) ) * NA * P + a * P;
                     const float fc = fminf( fmaxf( f, 0.0f ), (float)last );
                     const int k = min( (int)fc, last - 1 );
                     const float w = fc - (float)k;
@@ -869,6 +915,8 @@ sweep_kernel( const DevBook* __restrict__ books,
         observe( i );
         if constexpr ( cv_on )
             cv_date( i );
+        if ( i < m )
+            bridge_step( i, nz );  // the increments of the next step
     }
     if constexpr ( cv_samples )
         if ( cv_store )
