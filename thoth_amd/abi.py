@@ -134,3 +134,22 @@ class CvResult(C.Structure):
 class RqmcResult(C.Structure):
     _fields_ = [("replicates", _i32), ("paths", _i64), ("mean", _d), ("halfwidth", _d), ("var", _d),
                 ("sweep_ms", _d)]
+
+
+# include/thoth_mcl_grid.h
+GRID_MAX_MATURITIES = 16
+GRID_MAX_STRIKES = 64
+
+
+class VanillaGrid(C.Structure):
+    _fields_ = [("underlying", _i32), ("n_maturities", _i32), ("maturity_days", _i32 * GRID_MAX_MATURITIES),
+                ("n_strikes", _i32), ("strikes", _d * GRID_MAX_STRIKES), ("is_absolute_strike", _i32)]
+
+
+class GridResult(C.Structure):
+    _fields_ = [("call", _pd), ("put", _pd), ("call_trust", _pd), ("put_trust", _pd), ("call_delta", _pd),
+                ("put_delta", _pd), ("call_vega", _pd), ("put_vega", _pd)]
+
+
+class GridTuning(C.Structure):
+    _fields_ = [("blocks_per_cu", _i32), ("lev_group", _i32), ("device_ms", _d), ("candidates", _i32)]

@@ -41,6 +41,7 @@ UNITS = [
     ("mcl_market.cpp", "g++", []),
     ("mcl_host.cpp", "g++", []),
     ("mcl_cv_host.cpp", "g++", []),
+    ("mcl_grid.cpp", "g++", []),
 ]
 
 

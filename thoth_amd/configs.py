@@ -124,3 +124,23 @@ def lsv_worst_of_basket(paths=1 << 20, precision=abi.FP32, n_assets=5, maturity_
                     correlation=corr.tolist(), rate_days=[0], rate_values=[3.0],
                     rainbows=[RainbowUdl(members=list(range(n_assets)), best=False)],
                     greeks=greeks, precision=precision, correlate_stochastic=correlate_stochastic)
+
+
+# Vanilla grids on the LSV underlyings: 41 strikes 60..140 % of spot, 12 monthly
+# maturities (days round(365 k / 12)), calls and puts, one book per underlying.
+GRID_STRIKES = [60.0 + 2.0 * k for k in range(41)]
+GRID_MATURITIES = [round(365 * k / 12) for k in range(1, 13)]
+
+
+def lsv_vanilla_grids(paths=1 << 18, precision=abi.FP32, greeks=abi.GREEK_DELTA | abi.GREEK_VEGA, n_underlyings=5,
+                      strikes=None, maturities=None):
+    """(BookSpec, grid arguments) per underlying: single-equity LSV books of
+    the basket's underlyings, no contracts beyond the grid."""
+    eqs, _ = lsv_assets(5)
+    out = []
+    for a in range(n_underlyings):
+        spec = BookSpec(mcl=McConfig(paths=paths, max_day_step=1), equities=[eqs[a]], contracts=[],
+                        rate_days=[0], rate_values=[3.0], greeks=greeks, precision=precision)
+        out.append((spec, dict(underlying=0, maturity_days=maturities or GRID_MATURITIES,
+                               strikes=strikes or GRID_STRIKES)))
+    return out

@@ -44,7 +44,7 @@ struct Arena
 struct Offsets
 {
     size_t t, dt, sqrt_dt, bsq, binv, step_ops, op_b, op_left, op_w, op_wf, drift, sc, lev, levf, lnstep, offset,
-        offsetf, invstep, mon, flow, cv_start, cv_end, cv_scale;
+        offsetf, invstep, mon, flow, cv_start, cv_end, cv_scale, grid_mat_at, grid_kf, grid_kd;
 };
 
 std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
@@ -57,8 +57,8 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
         return SetError( THOTH_ERR_INVALID, Why( bi, "need at least 2 dates and t/dt arrays" ) );
     if ( na < 1 || na > kMaxAssets )
         return SetError( THOTH_ERR_INVALID, Why( bi, "1..8 equities supported" ) );
-    if ( in.n_contracts < 1 || in.n_contracts > kMaxContracts )
-        return SetError( THOTH_ERR_UNSUPPORTED, Why( bi, "1..32 contracts per book" ) );
+    if ( in.n_contracts < 0 || in.n_contracts > kMaxContracts )
+        return SetError( THOTH_ERR_UNSUPPORTED, Why( bi, "at most 32 contracts per book" ) );
     if ( in.n_rainbows < 0 || in.n_rainbows > kMaxRainbows )
         return SetError( THOTH_ERR_UNSUPPORTED, Why( bi, "at most 4 rainbows per book" ) );
     if ( in.t[0] != 0.0 )
@@ -312,6 +312,32 @@ int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Off
     return THOTH_OK;
 }
 
+int PackGrid( int bi, const thoth_mcl_book& in, const GridSpec& gs, DevBook& db, Offsets& of, Arena& ar )
+{
+    const int nd = in.n_dates, m = nd - 1;
+    if ( gs.nm < 1 || gs.nm > kGridMaxMat || gs.nk < 1 || gs.nk > kGridMaxStrikes )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "vanilla grid: 1..16 maturities x 1..64 strikes" ) );
+    if ( gs.udl >= in.n_assets || ( gs.udl < 0 && -1 - gs.udl >= in.n_rainbows ) )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "vanilla grid underlying" ) );
+    std::vector<int> at( nd, -1 );
+    for ( int j = 0; j < gs.nm; j++ )
+    {
+        const int d = gs.mat_date[j];
+        if ( d < 1 || d > m || at[d] >= 0 )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "vanilla grid maturities must be distinct diffusion dates" ) );
+        at[d] = j;
+    }
+    std::vector<double> kd( gs.strikes, gs.strikes + gs.nk );
+    std::vector<float> kf( kd.begin(), kd.end() );
+    db.grid_nm = gs.nm;
+    db.grid_nk = gs.nk;
+    db.grid_udl = gs.udl;
+    of.grid_mat_at = ar.put( at );
+    of.grid_kf = ar.put( kf );
+    of.grid_kd = ar.put( kd );
+    return THOTH_OK;
+}
+
 template <class T>
 T* At( char* base, size_t off )
 {
@@ -319,6 +345,14 @@ T* At( char* base, size_t off )
 }
 
 }  // namespace
+
+static size_t LdsBytes( const DeviceBatch& B, int G )
+{
+    const size_t words = B.lds_dims + (size_t)G * B.lds_lev_words;
+    if ( !B.lds_tail )
+        return words * sizeof( uint32_t );
+    return ( ( words + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + B.lds_tail;
+}
 
 // result slots per block of book b: the book, its contracts, and P - CV when
 // the hedge network runs in the loop
@@ -331,7 +365,7 @@ DeviceBatch::~DeviceBatch()
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv )
+                  const CvSpec* cv, const GridSpec* grids )
 {
     int rc = EnsureDevice();
     if ( rc != THOTH_OK )
@@ -379,6 +413,14 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
                 return rc;
             B.cv_mode = cv->mode;
         }
+        if ( grids && grids[b].nm > 0 )
+        {
+            if ( cv )
+                return SetError( THOTH_ERR_INVALID, "vanilla grids and the control variate are separate sweeps" );
+            rc = PackGrid( b, books[b], grids[b], B.db[b], of[b], ar );
+            if ( rc != THOTH_OK )
+                return rc;
+        }
         B.na_max = std::max( B.na_max, B.db[b].n_assets );
     }
     // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV) a
@@ -386,17 +428,35 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     // or 2 when the hedge network also needs its LDS area: two blocks per CU)
     const int na_inst = B.na_max <= 1 ? 1 : B.na_max <= 2 ? 2 : B.na_max <= 5 ? 5 : 8;
     const int lev_group = B.cv_mode == 2 ? 2 : 4;
-    size_t dims = 0, lev = 0;
+    size_t dims = 0, levw = 0;
+    bool any_grid = false;
     for ( int b = 0; b < n_books; b++ )
     {
         B.db[b].lev_group = lev_group;
         dims = std::max( dims, (size_t)B.db[b].m * B.db[b].factors );
         if ( cfg->precision == 1 && B.db[b].lev_points > 0 )
-            lev = std::max( lev, (size_t)2 * lev_group * na_inst * B.db[b].lev_points );
+            levw = std::max( levw, (size_t)2 * na_inst * B.db[b].lev_points );
+        any_grid |= B.db[b].grid_nm > 0;
     }
-    B.lds_bytes = ( dims + lev ) * sizeof( uint32_t );
-    if ( B.cv_mode == 2 )  // hedge-network area, 16-byte aligned (sweep_kernel's layout)
-        B.lds_bytes = ( ( dims + lev + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + kCvLdsBytes;
+    B.lds_dims = dims;
+    B.lds_lev_words = levw;
+    // tail areas, 16-byte aligned after the leverage ring (sweep_block's layout)
+    B.lds_tail = B.cv_mode == 2 ? (size_t)kCvLdsBytes : any_grid ? (size_t)grid_lds_bytes( cfg->precision == 0 ? 8 : 4 ) : 0;
+    B.lds_bytes = LdsBytes( B, lev_group );
+    // grid partials [R][blocks][nm][2 nk][2] and their sums [nm][2 nk][2], per book
+    B.grid_off.assign( n_books, 0 );
+    B.grid_out_off.assign( n_books, 0 );
+    B.grid_cells.assign( n_books, 0 );
+    B.grid_total = B.grid_out_total = 0;
+    for ( int b = 0; b < n_books; b++ )
+    {
+        const DevBook& d = B.db[b];
+        B.grid_cells[b] = d.grid_nm * 2 * d.grid_nk;
+        B.grid_off[b] = B.grid_total;
+        B.grid_out_off[b] = B.grid_out_total;
+        B.grid_total += (size_t)B.replicates * B.nblocks * B.grid_cells[b] * 2;
+        B.grid_out_total += (size_t)B.grid_cells[b] * 2;
+    }
     B.part_off.assign( n_books, 0 );
     B.part_total = 0;
     for ( int b = 0; b < n_books; b++ )
@@ -436,12 +496,15 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     const size_t shift_off = ar.put( shifts );
     const size_t arena_bytes = ( ar.data.size() + 255 ) & ~size_t( 255 );
     const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
-    const size_t total = arena_bytes + books_bytes + B.part_total * sizeof( double );
+    const size_t total =
+        arena_bytes + books_bytes + ( B.part_total + B.grid_total + B.grid_out_total ) * sizeof( double );
     hipError_t e = hipMalloc( &B.dbase, total );
     if ( e != hipSuccess )
         return SetError( THOTH_ERR_DEVICE, std::string( "hipMalloc: " ) + hipGetErrorString( e ) );
     char* dbase = B.dbase;
     B.dpart = reinterpret_cast<double*>( dbase + arena_bytes + books_bytes );
+    B.dgrid = B.dpart + B.part_total;
+    B.dgrid_out = B.dgrid + B.grid_total;
     for ( int b = 0; b < n_books; b++ )
     {
         DevBook& d = B.db[b];
@@ -476,6 +539,13 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.cv_scale = At<float>( dbase, o.cv_scale );
         }
         d.partial = B.dpart + B.part_off[b];
+        if ( d.grid_nm > 0 )
+        {
+            d.grid_mat_at = At<int>( dbase, o.grid_mat_at );
+            d.grid_kf = At<float>( dbase, o.grid_kf );
+            d.grid_kd = At<double>( dbase, o.grid_kd );
+            d.grid_part = B.dgrid + B.grid_off[b];
+        }
     }
     B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
     B.dshifts = rqmc ? At<uint32_t>( dbase, shift_off ) : nullptr;
@@ -554,6 +624,43 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
             cv_out[r].sum2 = sum[2 * ( 1 + nc ) + 1];
         }
     }
+    return THOTH_OK;
+}
+
+int SetLevGroup( DeviceBatch& B, int G )
+{
+    if ( G < 1 || G > 16 )
+        return SetError( THOTH_ERR_INVALID, "leverage group out of range" );
+    for ( DevBook& d : B.db )
+        d.lev_group = G;
+    B.lds_bytes = LdsBytes( B, G );
+    hipStream_t s = Stream();
+    HIP_OK( hipMemcpyAsync( const_cast<DevBook*>( B.dbooks ), B.db.data(), B.db.size() * sizeof( DevBook ),
+                            hipMemcpyHostToDevice, s ) );
+    HIP_OK( hipStreamSynchronize( s ) );
+    return THOTH_OK;
+}
+
+int ReduceGrids( const DeviceBatch& B, hipStream_t s )
+{
+    for ( int b = 0; b < B.n_books; b++ )
+        if ( B.grid_cells[b] > 0 )
+            launch_slot_sum( s, B.dgrid + B.grid_off[b], (int)( B.replicates * B.nblocks ), B.grid_cells[b] * 2,
+                             B.dgrid_out + B.grid_out_off[b] );
+    HIP_OK( hipGetLastError() );
+    return THOTH_OK;
+}
+
+int CopyGrids( const DeviceBatch& B, std::vector<std::vector<double>>& out )
+{
+    hipStream_t s = Stream();
+    out.assign( B.n_books, {} );
+    std::vector<double> all( B.grid_out_total );
+    if ( B.grid_out_total )
+        HIP_OK( hipMemcpyAsync( all.data(), B.dgrid_out, all.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+    HIP_OK( hipStreamSynchronize( s ) );
+    for ( int b = 0; b < B.n_books; b++ )
+        out[b].assign( all.begin() + B.grid_out_off[b], all.begin() + B.grid_out_off[b] + B.grid_cells[b] * 2 );
     return THOTH_OK;
 }
 

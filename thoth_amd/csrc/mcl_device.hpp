@@ -21,6 +21,18 @@ constexpr int kCvBiasBytes = 208 * 4;
 constexpr int kCvWaveBytes = 64 * 16 * 2 + 64 * 8 * 4;
 constexpr int kCvLdsBytes = kCvImgBytes + kCvBiasBytes + kWavesPerBlock * kCvWaveBytes;
 
+// vanilla grids: per book up to 16 maturities x 64 strikes, calls and puts
+constexpr int kGridMaxMat = 16;
+constexpr int kGridMaxStrikes = 64;
+constexpr int kGridMaxCells = 2 * kGridMaxStrikes;  // cell 2k = call k, 2k + 1 = put k
+// LDS of the grid observation: per wave a [16 cells][64 lanes] transposition
+// block (FP32 or FP64 values), then the [4 waves][128 cells][2] FP64 stage
+constexpr int kGridRows = 16;
+constexpr int grid_lds_bytes( int value_bytes )
+{
+    return kWavesPerBlock * kGridRows * 64 * value_bytes + kWavesPerBlock * kGridMaxCells * 2 * 8;
+}
+
 // FP32 per-(equity, date) step constants, precomputed on the host in FP64
 // from the reference's QE / Andersen formulas (heston_variance_node.cpp:29-33,
 // heston_spot_node.cpp:60-75).
@@ -106,6 +118,13 @@ struct DevBook
     uint16_t* cv_feat;          // [R][J][N][16] BF16 features at d_j
     float* cv_dm;               // [R][J][N][8] discounted rebased martingale increments over interval j
     float* cv_pay;              // [R][N] discounted book value
+    // vanilla grid (grid_nm = 0: none): calls and puts on one underlying,
+    // summed per (replicate, Sobol block, maturity, cell) into grid_part
+    int grid_nm, grid_nk, grid_udl;
+    const int* grid_mat_at;     // [n_dates] maturity index j at the date, else -1
+    const float* grid_kf;       // [nk] absolute strikes
+    const double* grid_kd;
+    double* grid_part;          // [R][tiles][nm][2 nk][2] undiscounted payoff sums
 };
 
 }  // namespace thoth

@@ -27,11 +27,6 @@ namespace thoth::host
 
 namespace
 {
-constexpr double kSpotBump = 0.01;    // helpers/constants.hpp:36
-constexpr double kGammaBump = 0.10;   // tasks/pricer.hpp:20
-constexpr double kVolBump = 0.01;     // tasks/pricer.hpp:25
-constexpr double kRateBump = 0.0001;  // tasks/pricer.hpp:26
-
 
 // barrier.cpp:56-69
 std::set<double> MonitoringDates( const thoth_contract& c )
@@ -48,10 +43,19 @@ std::set<double> MonitoringDates( const thoth_contract& c )
 }  // namespace
 
 // pricer_mcl.cpp:71-117 (+ vanilla/barrier GetFixingDates); uniform-steps extension
-Grid BuildGrid( const thoth_mcl_request& req, const Market& mk )
+Grid BuildGrid( const thoth_mcl_request& req, const Market& mk, const std::vector<double>* extra_days )
 {
     Grid g;
     std::set<double> fixing;
+    if ( extra_days )
+        for ( double d : *extra_days )
+        {
+            if ( d <= 0 )
+                throw Invalid( THOTH_ERR_INVALID, "fixing dates must be after today" );
+            fixing.insert( d );
+        }
+    if ( fixing.empty() && req.n_contracts < 1 )
+        throw Invalid( THOTH_ERR_INVALID, "nothing to price" );
     for ( int c = 0; c < req.n_contracts; c++ )
     {
         const thoth_contract& k = req.contracts[c];
@@ -67,8 +71,8 @@ Grid BuildGrid( const thoth_mcl_request& req, const Market& mk )
     const double last = *fixing.rbegin();
     if ( req.uniform_steps > 0 )
     {
-        if ( fixing.size() != 1 && !( req.n_contracts > 0 ) )
-            throw Invalid( THOTH_ERR_INVALID, "uniform grid needs a common maturity" );
+        if ( extra_days || req.n_contracts < 1 )
+            throw Invalid( THOTH_ERR_UNSUPPORTED, "uniform grid needs every contract at the last date" );
         for ( int c = 0; c < req.n_contracts; c++ )
             if ( req.contracts[c].maturity_days != last )
                 throw Invalid( THOTH_ERR_UNSUPPORTED, "uniform grid needs every contract at the last date" );
@@ -106,9 +110,6 @@ Grid BuildGrid( const thoth_mcl_request& req, const Market& mk )
     }
     return g;
 }
-
-namespace
-{
 
 void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, const std::vector<double>& chol,
               const std::vector<LeverageGrid>& lev, BookArrays& A )
@@ -279,7 +280,6 @@ void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, con
     b.c_mon_index = A.c_mon.data();
 }
 
-}  // namespace
 
 // A request resolved into device-resident books (base + bump scenarios).
 struct Plan

@@ -95,8 +95,17 @@ struct BookArrays
     thoth_mcl_book book{};
 };
 
+constexpr double kSpotBump = 0.01;    // helpers/constants.hpp:36
+constexpr double kGammaBump = 0.10;   // tasks/pricer.hpp:20
+constexpr double kVolBump = 0.01;     // tasks/pricer.hpp:25
+constexpr double kRateBump = 0.0001;  // tasks/pricer.hpp:26
+
 Market BuildMarket( const thoth_mcl_request& req );
-Grid BuildGrid( const thoth_mcl_request& req, const Market& mk );
+// the diffusion grid; extra_days adds fixing dates (vanilla-grid maturities)
+Grid BuildGrid( const thoth_mcl_request& req, const Market& mk, const std::vector<double>* extra_days = nullptr );
+// the device book arrays of one scenario market (pointers into A)
+void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, const std::vector<double>& chol,
+              const std::vector<LeverageGrid>& lev, BookArrays& A );
 bool CholeskyLower( std::vector<double>& a, int n );  // maths.cpp:354-381
 int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& out );
 // the base scenario's device book (grid, Cholesky, LSV calibration, payoffs);

@@ -49,6 +49,16 @@ struct DeviceBatch
     size_t lds_bytes = 0;
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
     int cv_mode = 0;  // 0 none, 1 control-variate samples, 2 in-loop hedge inference
+    // vanilla grids: per-(replicate, block) partials and their per-book sums
+    // [nm][2 nk][2] (device), offsets per book; grid_cells[b] = nm * 2 nk
+    double* dgrid = nullptr;
+    double* dgrid_out = nullptr;
+    std::vector<size_t> grid_off, grid_out_off;
+    std::vector<int> grid_cells;
+    size_t grid_total = 0, grid_out_total = 0;
+    // dynamic-LDS recipe: Sobol table words, leverage words per staged layer,
+    // fixed tail bytes (hedge-network or grid area)
+    size_t lds_dims = 0, lds_lev_words = 0, lds_tail = 0;
     int64_t paths = 0;
     DeviceBatch() = default;
     DeviceBatch( const DeviceBatch& ) = delete;
@@ -71,14 +81,29 @@ struct CvSpec
     const uint16_t* wimg = nullptr;
     const float* bias = nullptr;
 };
+// Vanilla grid of one book: calls and puts of nk absolute strikes at nm
+// maturities (date indices of the book's grid) on underlying udl (equity
+// index, or -(1 + rainbow index)).
+struct GridSpec
+{
+    int nm = 0, nk = 0, udl = 0;
+    int mat_date[kGridMaxMat] = {};
+    double strikes[kGridMaxStrikes] = {};
+};
 // sums of P - CV per replicate (mode 2)
 struct CvSums
 {
     double sum = 0, sum2 = 0;
 };
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv = nullptr );
+                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
+// grid sums of every book, [nm][2 nk][2] each, from the last sweep (device reduction + copy)
+int ReduceGrids( const DeviceBatch& B, hipStream_t s );
+// re-stage the leverage layers G per barrier (ring of 2G): updates the device
+// books and the dynamic-LDS size
+int SetLevGroup( DeviceBatch& B, int G );
+int CopyGrids( const DeviceBatch& B, std::vector<std::vector<double>>& out );
 int SweepBooks( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg,
                 thoth_mcl_book_out* out );
 

@@ -59,5 +59,13 @@ hipError_t cv_train_init();  // once per process before the first step (LDS limi
 bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd, float* w,
                            float* m, float* v, const uint16_t* feat, const float* dm, const float* pay, int64_t N,
                            int* step, float* cvpart, float* gpart, float* loss_part, float lr );
+bool launch_persistent_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                           const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
+                           int n_items, uint32_t ntiles, int* counter );
+bool launch_persistent_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                           const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
+                           int n_items, uint32_t ntiles, int* counter );
+void launch_counter_reset( hipStream_t s, int* counter );
+void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slot, double* out );
 bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t first, uint32_t last, uint32_t* out );
 }  // namespace thoth

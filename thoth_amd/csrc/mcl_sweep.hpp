@@ -345,12 +345,13 @@ static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 
 // CV: 0 plain sweep; 1 also writes the control-variate training samples;
 // 2 evaluates the hedge network at every hedge date inside the stepping loop
 // and accumulates P - CV in an extra result slot after the contracts.
-template <class R, int NA, int DEPTH, int CV = 0>
-__global__ void __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( CV == 2 ? 2 : 1 ) ) )
-sweep_kernel( const DevBook* __restrict__ books,
-                                                         const uint32_t* __restrict__ V, uint32_t bq0,
-                                                         uint32_t first_idx, uint32_t last_idx,
-                                                         const uint32_t* __restrict__ shifts )
+// One block of 256 consecutive Sobol indices (Sobol block bq) of book B,
+// replicate z (RQMC), partial-sum slot (z, tile) of ntiles per replicate.
+template <class R, int NA, int DEPTH, int CV>
+__device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* __restrict__ V, uint32_t bq,
+                                             uint32_t first_idx, uint32_t last_idx,
+                                             const uint32_t* __restrict__ shifts, int z, uint32_t tile,
+                                             uint32_t ntiles )
 {
     constexpr bool F64 = std::is_same_v<R, double>;
     constexpr int NF = 2 * NA;
@@ -360,13 +361,11 @@ sweep_kernel( const DevBook* __restrict__ books,
     typedef R facv __attribute__( ( ext_vector_type( NFV ) ) );
     typedef R assetv __attribute__( ( ext_vector_type( NAV ) ) );
 
-    const DevBook& B = books[blockIdx.y];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
-    const uint32_t bq = bq0 + blockIdx.x;  // block of 256 Sobol indices
     const uint32_t idx = bq * 256u + (uint32_t)threadIdx.x;
     const bool valid = idx >= first_idx && idx <= last_idx;
-    const uint32_t* sh = shifts ? shifts + (size_t)blockIdx.z * kSobolDimsDev : nullptr;
+    const uint32_t* sh = shifts ? shifts + (size_t)z * kSobolDimsDev : nullptr;
 
     const int F = B.factors;
     const int m = B.m;
@@ -379,6 +378,14 @@ sweep_kernel( const DevBook* __restrict__ books,
     extern __shared__ uint32_t lds[];
     uint32_t* tab = lds;
     float* levs = reinterpret_cast<float*>( lds + m * F );
+    // vanilla-grid area (never together with the control variate)
+    [[maybe_unused]] char* gl = nullptr;
+    if constexpr ( CV == 0 )
+        if ( B.grid_nm > 0 )
+        {
+            const int lev_words = ( !F64 && B.lev_points > 0 ) ? 2 * B.lev_group * NA * B.lev_points : 0;
+            gl = reinterpret_cast<char*>( lds + ( ( m * F + lev_words + 3 ) & ~3 ) );
+        }
     constexpr bool cv_on = CV != 0 && !F64;
     constexpr bool cv_samples = CV == 1 && !F64;
     constexpr bool cv_infer = CV == 2 && !F64;
@@ -515,7 +522,7 @@ sweep_kernel( const DevBook* __restrict__ books,
     {
         uint32_t w[8];
         cv_feature_words( i, w );
-        uint4* dst = reinterpret_cast<uint4*>( B.cv_feat + ( ( (size_t)blockIdx.z * B.cv_J + j ) * B.cv_N + pidx ) * 16 );
+        uint4* dst = reinterpret_cast<uint4*>( B.cv_feat + ( ( (size_t)z * B.cv_J + j ) * B.cv_N + pidx ) * 16 );
         dst[0] = uint4{ w[0], w[1], w[2], w[3] };
         dst[1] = uint4{ w[4], w[5], w[6], w[7] };
     };
@@ -570,7 +577,7 @@ sweep_kernel( const DevBook* __restrict__ books,
                 for ( int a = 0; a < NA; a++ )
                     if ( a < 8 )
                         o[a] = dmacc[a];
-                float4* dst = reinterpret_cast<float4*>( B.cv_dm + ( ( (size_t)blockIdx.z * B.cv_J + je ) * B.cv_N + pidx ) * 8 );
+                float4* dst = reinterpret_cast<float4*>( B.cv_dm + ( ( (size_t)z * B.cv_J + je ) * B.cv_N + pidx ) * 8 );
                 dst[0] = float4{ o[0], o[1], o[2], o[3] };
                 dst[1] = float4{ o[4], o[5], o[6], o[7] };
             }
@@ -640,6 +647,70 @@ sweep_kernel( const DevBook* __restrict__ books,
                 acc[wib][1 + c][1] += s2;
             }
         }
+    };
+
+    // vanilla grid at maturity j: call and put payoffs of every strike, summed
+    // over the block's 256 paths (FP64 from the first addition), one partial
+    // per (replicate, Sobol block, maturity, cell).  Per wave, 16 cells at a
+    // time go lane -> LDS [cell][lane]; lane l then adds a quarter row
+    // (cell l/4, lanes 16 (l%4) ..) and two shuffles finish the row.
+    auto grid_observe = [&]( int j ) __attribute__( ( always_inline ) )
+    {
+        const R s = udl_value( B.grid_udl );
+        const int ncell = 2 * B.grid_nk;
+        R* tr = reinterpret_cast<R*>( gl ) + wib * kGridRows * 64;
+        double* stage = reinterpret_cast<double*>( gl + kWavesPerBlock * kGridRows * 64 * sizeof( R ) );
+        for ( int c0 = 0; c0 < ncell; c0 += kGridRows )
+        {
+#pragma unroll
+            for ( int r = 0; r < kGridRows; r++ )
+            {
+                const int c = c0 + r;
+                R x = R( 0 );
+                if ( c < ncell )
+                {
+                    R K;
+                    if constexpr ( F64 )
+                        K = B.grid_kd[c >> 1];
+                    else
+                        K = B.grid_kf[c >> 1];
+                    x = ( c & 1 ) ? K - s : s - K;  // finance.cpp:9-32
+                    x = x < R( 0 ) ? R( 0 ) : x;
+                }
+                tr[r * 64 + lane] = valid ? x : R( 0 );
+            }
+            wave_lds_sync();
+            const int row = lane >> 2, q = lane & 3;
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for ( int t = 0; t < 16; t++ )
+            {
+                const double v = (double)tr[row * 64 + q * 16 + t];
+                s1 += v;
+                s2 = fma( v, v, s2 );
+            }
+            s1 += __shfl_xor( s1, 1, 64 );
+            s2 += __shfl_xor( s2, 1, 64 );
+            s1 += __shfl_xor( s1, 2, 64 );
+            s2 += __shfl_xor( s2, 2, 64 );
+            if ( q == 0 && c0 + row < ncell )
+            {
+                stage[( wib * kGridMaxCells + c0 + row ) * 2] = s1;
+                stage[( wib * kGridMaxCells + c0 + row ) * 2 + 1] = s2;
+            }
+            wave_lds_sync();
+        }
+        __syncthreads();
+        double* out = B.grid_part + ( ( (size_t)z * ntiles + tile ) * B.grid_nm + j ) * ncell * 2;
+        for ( int t = threadIdx.x; t < 2 * ncell; t += kBlock )
+        {
+            const int c = t >> 1, mom = t & 1;
+            double v = stage[c * 2 + mom];
+            for ( int w = 1; w < kWavesPerBlock; w++ )
+                v += stage[( w * kGridMaxCells + c ) * 2 + mom];
+            out[t] = v;
+        }
+        __syncthreads();
     };
 
     observe( 0 );
@@ -913,6 +984,9 @@ sweep_kernel( const DevBook* __restrict__ books,
             }
         }
         observe( i );
+        if constexpr ( CV == 0 )
+            if ( B.grid_nm > 0 && B.grid_mat_at[i] >= 0 )
+                grid_observe( B.grid_mat_at[i] );
         if constexpr ( cv_on )
             cv_date( i );
         if ( i < m )
@@ -920,7 +994,7 @@ sweep_kernel( const DevBook* __restrict__ books,
     }
     if constexpr ( cv_samples )
         if ( cv_store )
-            B.cv_pay[(size_t)blockIdx.z * B.cv_N + pidx] = (float)book;
+            B.cv_pay[(size_t)z * B.cv_N + pidx] = (float)book;
     if constexpr ( cv_infer )
     {
         const double y = valid ? book - (double)cvacc : 0.0;
@@ -953,7 +1027,44 @@ sweep_kernel( const DevBook* __restrict__ books,
         double s = 0.0;
         for ( int w = 0; w < kWavesPerBlock; w++ )
             s += acc[w][e][mom];
-        B.partial[( (size_t)blockIdx.z * gridDim.x + blockIdx.x ) * ne + threadIdx.x] = s;
+        B.partial[( (size_t)z * ntiles + tile ) * ne + threadIdx.x] = s;
+    }
+}
+
+// one Sobol block per workgroup: grid (blocks, books, replicates)
+template <class R, int NA, int DEPTH, int CV = 0>
+__global__ void __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( CV == 2 ? 2 : 1 ) ) )
+sweep_kernel( const DevBook* __restrict__ books, const uint32_t* __restrict__ V, uint32_t bq0, uint32_t first_idx,
+              uint32_t last_idx, const uint32_t* __restrict__ shifts )
+{
+    sweep_block<R, NA, DEPTH, CV>( books[blockIdx.y], V, bq0 + blockIdx.x, first_idx, last_idx, shifts, blockIdx.z,
+                                   blockIdx.x, gridDim.x );
+}
+
+// Persistent form: a fixed grid of workgroups pulls (book, Sobol block) work
+// items from a queue (items sorted longest book first by the host; the
+// counter is zeroed by a kernel of the same graph).  Every item writes its
+// own partial-sum slot, so the result does not depend on which workgroup
+// took which item.
+template <class R, int NA, int DEPTH>
+__global__ void __launch_bounds__( kBlock )
+sweep_persistent( const DevBook* __restrict__ books, const uint32_t* __restrict__ V, uint32_t bq0, uint32_t first_idx,
+                  uint32_t last_idx, const int2* __restrict__ items, int n_items, uint32_t ntiles,
+                  int* __restrict__ counter )
+{
+    __shared__ int item_s;
+    for ( ;; )
+    {
+        if ( threadIdx.x == 0 )
+            item_s = atomicAdd( counter, 1 );
+        __syncthreads();
+        const int it = item_s;
+        __syncthreads();  // every thread has its item before thread 0 takes the next
+        if ( it >= n_items )
+            break;
+        const int2 w = items[it];
+        sweep_block<R, NA, DEPTH, 0>( books[w.x], V, bq0 + (uint32_t)w.y, first_idx, last_idx, nullptr, 0,
+                                      (uint32_t)w.y, ntiles );
     }
 }
 

@@ -39,4 +39,20 @@ bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, 
         THOTH_DISPATCH_SWEEP( CvLaunch<2>::one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
 }
+template <int NA, int D>
+static void launch_persistent_one( dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,
+                                   uint32_t bq0, uint32_t first, uint32_t last, const int2* items, int n_items,
+                                   uint32_t ntiles, int* counter )
+{
+    hipLaunchKernelGGL( ( sweep_persistent<float, NA, D> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first, last,
+                        items, n_items, ntiles, counter );
+}
+
+bool launch_persistent_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                           const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
+                           int n_items, uint32_t ntiles, int* counter )
+{
+    THOTH_DISPATCH_SWEEP( launch_persistent_one, na, depth, grid, s, lds, books, V, bq0, first, last, items, n_items,
+                          ntiles, counter );
+}
 }  // namespace thoth

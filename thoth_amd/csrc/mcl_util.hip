@@ -43,4 +43,36 @@ bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t fi
     return hipGetLastError() == hipSuccess;
 }
 
+// zero the persistent sweep's work counter (a kernel, so that the graph that
+// replays the sweep holds no memset node)
+__global__ void counter_reset_kernel( int* counter ) { *counter = 0; }
+
+void launch_counter_reset( hipStream_t s, int* counter )
+{
+    hipLaunchKernelGGL( counter_reset_kernel, dim3( 1 ), dim3( 1 ), 0, s, counter );
+}
+
+// out[e] = Kahan sum over the slots, in slot order, of part[slot * per_slot + e]
+__global__ void slot_sum_kernel( const double* __restrict__ part, int nslots, int per_slot, double* __restrict__ out )
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( e >= per_slot )
+        return;
+    double s = 0.0, c = 0.0;
+    for ( int k = 0; k < nslots; k++ )
+    {
+        const double y = part[(size_t)k * per_slot + e] - c;
+        const double t = s + y;
+        c = ( t - s ) - y;
+        s = t;
+    }
+    out[e] = s;
+}
+
+void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slot, double* out )
+{
+    hipLaunchKernelGGL( slot_sum_kernel, dim3( ( per_slot + 255 ) / 256 ), dim3( 256 ), 0, s, part, nslots, per_slot,
+                        out );
+}
+
 }  // namespace thoth

@@ -50,6 +50,11 @@ def lib():
         L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
                                                    P(C.c_double), P(C.c_int64), P(C.c_double)]
         L.thoth_mcl_price_rqmc.argtypes = [P(abi.Request), C.c_int32, C.c_uint64, P(abi.RqmcResult)]
+        L.thoth_grid_prepare.argtypes = [C.c_int32, P(abi.Request), P(abi.VanillaGrid), C.c_int32, P(C.c_void_p)]
+        L.thoth_grid_run.argtypes = [C.c_void_p, P(abi.GridResult), P(C.c_double)]
+        L.thoth_grid_get_tuning.argtypes = [C.c_void_p, P(abi.GridTuning)]
+        L.thoth_grid_release.argtypes = [C.c_void_p]
+        L.thoth_grid_release.restype = None
         L.thoth_cv_train.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), P(abi.CvTrainReport)]
         L.thoth_cv_price.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), C.c_int32, C.c_uint64,
                                      P(abi.CvResult)]
@@ -299,3 +304,62 @@ def cv_weights_from_array(a):
 
 def cv_weights_array(w):
     return np.ctypeslib.as_array(w.w).copy()
+
+
+# ---- vanilla grids (include/thoth_mcl_grid.h) ------------------------------
+
+def vanilla_grid(underlying, maturity_days, strikes, absolute=False):
+    g = abi.VanillaGrid()
+    g.underlying = underlying
+    g.n_maturities = len(maturity_days)
+    for j, d in enumerate(maturity_days):
+        g.maturity_days[j] = int(d)
+    g.n_strikes = len(strikes)
+    for k, x in enumerate(strikes):
+        g.strikes[k] = float(x)
+    g.is_absolute_strike = 1 if absolute else 0
+    return g
+
+
+class GridPlan:
+    """All books (request x bump scenario) of a set of vanilla grids, swept by
+    one persistent launch replayed from a hipGraph (thoth_grid_*)."""
+
+    def __init__(self, builts, grids, autotune=True):
+        self.builts = list(builts)
+        self.grids = list(grids)
+        reqs = (abi.Request * len(self.builts))(*[b.req for b in self.builts])
+        gr = (abi.VanillaGrid * len(self.grids))(*self.grids)
+        self._keep = (reqs, gr)
+        self.handle = C.c_void_p()
+        _check(lib().thoth_grid_prepare(len(self.builts), reqs, gr, 1 if autotune else 0, C.byref(self.handle)))
+
+    def run(self):
+        """Returns (list of dicts of [n_maturities, n_strikes] arrays, device ms)."""
+        out, res = [], (abi.GridResult * len(self.grids))()
+        for g, r in zip(self.grids, res):
+            shape = (g.n_maturities, g.n_strikes)
+            d = {k: np.zeros(shape) for k in ("call", "put", "call_trust", "put_trust", "call_delta", "put_delta",
+                                               "call_vega", "put_vega")}
+            for k, a in d.items():
+                setattr(r, k, a.ctypes.data_as(C.POINTER(C.c_double)))
+            out.append(d)
+        ms = C.c_double()
+        _check(lib().thoth_grid_run(self.handle, res, C.byref(ms)))
+        return out, ms.value
+
+    def tuning(self):
+        t = abi.GridTuning()
+        _check(lib().thoth_grid_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def close(self):
+        if self.handle:
+            lib().thoth_grid_release(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
