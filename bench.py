@@ -103,6 +103,34 @@ def time_to_1bp(replicates, paths, seed=101, target=0.01):
     return out
 
 
+def vanilla_grids(paths, replays=5):
+    """The 5 LSV underlyings' vanilla grids (41 strikes x 12 monthly maturities,
+    calls and puts, base + spot-bump + vol-bump books = 15 books of `paths`
+    paths) in one persistent launch replayed from a hipGraph; setup includes
+    the 15 leverage calibrations and the autotuner."""
+    books = configs.lsv_vanilla_grids(paths=paths, precision=abi.FP32)
+    t0 = time.perf_counter()
+    plan = mcl.GridPlan([BuiltRequest(s) for s, _ in books], [mcl.vanilla_grid(**g) for _, g in books])
+    setup = time.perf_counter() - t0
+    plan.run()
+    ms = [plan.run()[1] for _ in range(replays)]
+    t = plan.tuning()
+    dev = sum(ms) / len(ms)
+    n_books = 3 * len(books)
+    # per book-path: 365 LSV steps x 128 FLOP + 12 maturities x 328 (82 cells)
+    flops = n_books * paths * (365 * 128 + 12 * 328)
+    achieved = flops / (dev * 1e-3) / 1e12
+    return {"workload": "lsv_vanilla_grids", "underlyings": len(books), "books": n_books,
+            "cells_per_underlying": 2 * len(configs.GRID_STRIKES) * len(configs.GRID_MATURITIES),
+            "paths_per_book": paths, "replays": replays, "device_ms": dev,
+            "book_paths_per_s": n_books * paths / (dev * 1e-3),
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS},
+            "autotuned": {"workgroups_per_cu": t.blocks_per_cu, "leverage_layers_per_barrier": t.lev_group,
+                          "candidates": t.candidates, "best_ms": t.device_ms},
+            "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +143,8 @@ def main():
     ap.add_argument("--no-t1bp", action="store_true", help="skip the time-to-1bp comparison")
     ap.add_argument("--t1bp-replicates", type=int, default=32)
     ap.add_argument("--t1bp-paths", type=int, default=1 << 14)
+    ap.add_argument("--no-grid", action="store_true", help="skip the vanilla-grid batch")
+    ap.add_argument("--grid-paths", type=int, default=1 << 18)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -179,6 +209,8 @@ def main():
                      "frac": achieved / peak, "traffic": None},
         "device": mcl.device_info(),
     }
+    if not args.no_grid:
+        line["vanilla_grids"] = vanilla_grids(args.grid_paths)
     if not args.no_t1bp and args.workload == "lsv_worst_of_basket":
         line["time_to_1bp"] = time_to_1bp(args.t1bp_replicates, args.t1bp_paths)
     if not args.no_cpu:

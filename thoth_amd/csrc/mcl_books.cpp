@@ -494,6 +494,14 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
     }
     const size_t shift_off = ar.put( shifts );
+    std::vector<long long> gtab;
+    for ( int b = 0; b < n_books; b++ )
+    {
+        gtab.push_back( (long long)B.grid_off[b] );
+        gtab.push_back( (long long)B.grid_out_off[b] );
+        gtab.push_back( (long long)B.grid_cells[b] * 2 );
+    }
+    const size_t gtab_off = ar.put( gtab );
     const size_t arena_bytes = ( ar.data.size() + 255 ) & ~size_t( 255 );
     const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
     const size_t total =
@@ -548,6 +556,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
     }
     B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
+    B.dgrid_table = At<long long>( dbase, gtab_off );
     B.dshifts = rqmc ? At<uint32_t>( dbase, shift_off ) : nullptr;
     hipStream_t s = Stream();
 #define HIP_OK( expr )                                                                                      \
@@ -643,10 +652,12 @@ int SetLevGroup( DeviceBatch& B, int G )
 
 int ReduceGrids( const DeviceBatch& B, hipStream_t s )
 {
+    int max_cells = 0;
     for ( int b = 0; b < B.n_books; b++ )
-        if ( B.grid_cells[b] > 0 )
-            launch_slot_sum( s, B.dgrid + B.grid_off[b], (int)( B.replicates * B.nblocks ), B.grid_cells[b] * 2,
-                             B.dgrid_out + B.grid_out_off[b] );
+        max_cells = std::max( max_cells, B.grid_cells[b] * 2 );
+    if ( max_cells > 0 )
+        launch_grid_sum( s, B.dgrid, B.dgrid_table, B.n_books, max_cells, (int)( B.replicates * B.nblocks ),
+                         B.dgrid_out );
     HIP_OK( hipGetLastError() );
     return THOTH_OK;
 }

@@ -103,7 +103,7 @@ int TimeOnce( const GridPlan& P, double& ms )
 // fastest is kept.
 int Autotune( GridPlan& P )
 {
-    const int bpc[] = { 1, 2, 3, 4 };
+    const int bpc[] = { 1, 2, 4, 6, 8 };
     const int lg[] = { 1, 2, 4, 8 };
     const bool has_lev = P.batch.lds_lev_words > 0;
     double best = 1e300;

@@ -55,6 +55,7 @@ struct DeviceBatch
     double* dgrid_out = nullptr;
     std::vector<size_t> grid_off, grid_out_off;
     std::vector<int> grid_cells;
+    const long long* dgrid_table = nullptr;  // [books][3]: partial offset, output offset, doubles per slot
     size_t grid_total = 0, grid_out_total = 0;
     // dynamic-LDS recipe: Sobol table words, leverage words per staged layer,
     // fixed tail bytes (hedge-network or grid area)

@@ -67,5 +67,7 @@ bool launch_persistent_f32( int na, int depth, dim3 grid, hipStream_t s, size_t 
                            int n_items, uint32_t ntiles, int* counter );
 void launch_counter_reset( hipStream_t s, int* counter );
 void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slot, double* out );
+void launch_grid_sum( hipStream_t s, const double* part, const long long* table, int n_books, int max_per_slot,
+                      int nslots, double* out );
 bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t first, uint32_t last, uint32_t* out );
 }  // namespace thoth

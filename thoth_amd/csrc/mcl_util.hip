@@ -69,6 +69,54 @@ __global__ void slot_sum_kernel( const double* __restrict__ part, int nslots, in
     out[e] = s;
 }
 
+// Grid sums of every book in one launch: block (x, book) owns 32 consecutive
+// elements; its 8 thread groups Kahan-sum the slots g, g + 8, ... and the 8
+// group sums are combined in group order (fixed partition: deterministic).
+// table[b] = { partial offset, output offset, elements per slot } (doubles).
+__global__ void __launch_bounds__( 256 ) grid_sum_kernel( const double* __restrict__ part,
+                                                         const long long* __restrict__ table, int nslots,
+                                                         double* __restrict__ out )
+{
+    __shared__ double ps[8][32], pc[8][32];
+    const long long* tb = table + 3 * blockIdx.y;
+    const int per_slot = (int)tb[2];
+    const int e = blockIdx.x * 32 + ( threadIdx.x & 31 );
+    const int g = threadIdx.x >> 5;
+    double sum = 0.0, c = 0.0;
+    if ( e < per_slot )
+    {
+        const double* p = part + tb[0] + e;
+        for ( int k = g; k < nslots; k += 8 )
+        {
+            const double y = p[(size_t)k * per_slot] - c;
+            const double t = sum + y;
+            c = ( t - sum ) - y;
+            sum = t;
+        }
+    }
+    ps[g][threadIdx.x & 31] = sum;
+    pc[g][threadIdx.x & 31] = c;
+    __syncthreads();
+    if ( g == 0 && e < per_slot )
+    {
+        for ( int h = 1; h < 8; h++ )
+        {
+            const double y = ps[h][threadIdx.x] - ( c + pc[h][threadIdx.x] );
+            const double t = sum + y;
+            c = ( t - sum ) - y;
+            sum = t;
+        }
+        out[tb[1] + e] = sum;
+    }
+}
+
+void launch_grid_sum( hipStream_t s, const double* part, const long long* table, int n_books, int max_per_slot,
+                      int nslots, double* out )
+{
+    hipLaunchKernelGGL( grid_sum_kernel, dim3( ( max_per_slot + 31 ) / 32, n_books ), dim3( 256 ), 0, s, part, table,
+                        nslots, out );
+}
+
 void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slot, double* out )
 {
     hipLaunchKernelGGL( slot_sum_kernel, dim3( ( per_slot + 255 ) / 256 ), dim3( 256 ), 0, s, part, nslots, per_slot,
