@@ -173,6 +173,27 @@ typedef struct thoth_rqmc_result {
 } thoth_rqmc_result;
 int thoth_mcl_price_rqmc(const thoth_mcl_request* req, int32_t replicates, uint64_t rqmc_seed, thoth_rqmc_result* res);
 
+/* Pathwise Greeks (no reference counterpart; the reference computes bump
+ * Greeks only, pricer_mcl_greeks.cpp:43-299) of a single-equity Black-Scholes
+ * or Heston book holding one vanilla or one knock-out barrier contract, on
+ * the same Sobol points.  Knock-out books are simulated with one-step
+ * survival conditioning (each monitored step draws the spot conditioned on
+ * survival and carries the survival probability as a weight): an unbiased
+ * estimator of the same discretely monitored price whose pathwise derivative
+ * exists.  delta = dP/dS0, vega per vol point in the reference's bump
+ * parametrisation (Black-Scholes vol + 1pt; Heston sqrt(v0) and sqrt(theta)
+ * + 0.01), gamma = (delta(S0 (1 + 10%)) - delta(S0 (1 - 10%))) / (20% S0)
+ * when THOTH_GREEK_GAMMA is requested (the reference's gamma bump applied to
+ * the pathwise delta). */
+typedef struct thoth_pathwise_result {
+    double premium, premium_trust;   /* the survival-weighted estimator */
+    double delta, delta_trust;
+    double gamma;
+    double vega, vega_trust;
+    double sweep_ms;
+} thoth_pathwise_result;
+int thoth_mcl_price_pathwise(const thoth_mcl_request* req, thoth_pathwise_result* res);
+
 /* Diffusion grid of a request (pricer_mcl.cpp:71-117): writes up to cap dates
  * (days from today, year fractions t and step dt) and returns the date count,
  * or a negative THOTH_* status. */

@@ -153,3 +153,8 @@ class GridResult(C.Structure):
 
 class GridTuning(C.Structure):
     _fields_ = [("blocks_per_cu", _i32), ("lev_group", _i32), ("device_ms", _d), ("candidates", _i32)]
+
+
+class PathwiseResult(C.Structure):
+    _fields_ = [("premium", _d), ("premium_trust", _d), ("delta", _d), ("delta_trust", _d), ("gamma", _d),
+                ("vega", _d), ("vega_trust", _d), ("sweep_ms", _d)]

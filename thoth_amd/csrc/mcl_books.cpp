@@ -356,7 +356,12 @@ static size_t LdsBytes( const DeviceBatch& B, int G )
 
 // result slots per block of book b: the book, its contracts, and P - CV when
 // the hedge network runs in the loop
-static int SlotsOf( const DeviceBatch& B, int b ) { return 1 + B.db[b].n_contracts + ( B.cv_mode == 2 ? 1 : 0 ); }
+static int SlotsOf( const DeviceBatch& B, int b )
+{
+    if ( B.pathwise )
+        return 3;  // survival-weighted payoff, pathwise delta, pathwise vega
+    return 1 + B.db[b].n_contracts + ( B.cv_mode == 2 ? 1 : 0 );
+}
 
 DeviceBatch::~DeviceBatch()
 {
@@ -365,8 +370,9 @@ DeviceBatch::~DeviceBatch()
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv, const GridSpec* grids )
+                  const CvSpec* cv, const GridSpec* grids, bool pathwise )
 {
+    B.pathwise = pathwise;
     int rc = EnsureDevice();
     if ( rc != THOTH_OK )
         return rc;

@@ -9,6 +9,7 @@
 // launch on common Sobol points, and finite-differences the results exactly as
 // the reference does.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <mutex>
 #include <cmath>
@@ -19,6 +20,7 @@
 #include <vector>
 
 #include "mcl_hostlib.hpp"
+#include "mcl_launch.hpp"
 #include "mcl_internal.hpp"
 #include "thoth_mcl.h"
 
@@ -621,6 +623,130 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_rqmc
         res->var = ( s2 - s1 * s1 / n ) / ( n - 1 );
         res->sweep_ms = out[0].device_ms;
         return THOTH_OK;
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+namespace thoth::host
+{
+
+// single-equity BS / Heston book with one vanilla or knock-out contract:
+// base and (gamma) +-10 % spot books swept by the pathwise kernel
+int PricePathwise( const thoth_mcl_request& req, thoth_pathwise_result* res )
+{
+    if ( req.n_assets != 1 || !req.assets || req.n_contracts != 1 || !req.contracts )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: one equity and one contract" );
+    const thoth_asset& a0 = req.assets[0];
+    if ( a0.vol_kind == THOTH_VOL_LSV )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: Black-Scholes or Heston equities" );
+    const thoth_contract& k0 = req.contracts[0];
+    if ( k0.kind == THOTH_BARRIER && ( k0.barrier_type == THOTH_UP_IN || k0.barrier_type == THOTH_DOWN_IN ) )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: knock-out barriers (knock-in = vanilla - knock-out)" );
+    if ( k0.underlying != 0 )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: the contract's underlying is the equity" );
+    if ( req.mcl.paths <= 1 || req.mcl.paths > 200000000 || !req.mcl.use_sobol )
+        throw Invalid( THOTH_ERR_INVALID, "paths in (1, 2e8], Sobol mode" );
+    const Market base = BuildMarket( req );
+    const Grid g = BuildGrid( req, base );
+    std::vector<Market> mks{ base };
+    const double spot = base.eq[0].spot, h = kGammaBump * spot;
+    const bool gamma = ( req.request_greeks & THOTH_GREEK_GAMMA ) != 0;
+    if ( gamma )
+    {
+        Market u = base, d = base;
+        u.eq[0].spot = spot + h;
+        d.eq[0].spot = spot - h;
+        mks.push_back( u );
+        mks.push_back( d );
+    }
+    std::vector<BookArrays> arrays( mks.size() );
+    std::vector<thoth_mcl_book> books;
+    for ( size_t k = 0; k < mks.size(); k++ )
+    {
+        Resolve( req, mks[k], g, {}, std::vector<LeverageGrid>( 1 ), arrays[k] );
+        books.push_back( arrays[k].book );
+    }
+    thoth_mcl_sweep_cfg cfg{};
+    cfg.paths = req.mcl.paths;
+    cfg.sobol_skip = req.mcl.sobol_skip;
+    cfg.precision = req.precision;
+    DeviceBatch B;
+    int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, B, nullptr, nullptr, true );
+    if ( rc != THOTH_OK )
+        return rc;
+    hipStream_t s = Stream();
+    hipEvent_t e0, e1;
+    if ( hipEventCreate( &e0 ) != hipSuccess || hipEventCreate( &e1 ) != hipSuccess )
+        return SetError( THOTH_ERR_DEVICE, "hipEventCreate" );
+    (void)hipEventRecord( e0, s );
+    if ( !launch_pathwise( B.precision, B.depth, dim3( B.nblocks, B.n_books ), s, B.lds_bytes, B.dbooks,
+                           DeviceDirections(), B.q0, B.first, B.last ) )
+        return SetError( THOTH_ERR_UNSUPPORTED, "bridge deeper than the pathwise kernels support" );
+    if ( hipGetLastError() != hipSuccess )
+        return SetError( THOTH_ERR_DEVICE, "pathwise launch failed" );
+    (void)hipEventRecord( e1, s );
+    std::vector<double> part( B.part_total );
+    if ( hipMemcpyAsync( part.data(), B.dpart, part.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) !=
+             hipSuccess ||
+         hipStreamSynchronize( s ) != hipSuccess )
+        return SetError( THOTH_ERR_DEVICE, "pathwise copy failed" );
+    float ms = 0;
+    (void)hipEventElapsedTime( &ms, e0, e1 );
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    // Kahan over blocks in block order, per book: [Y, delta, vega] x [sum, sum2]
+    const double n = (double)req.mcl.paths;
+    std::vector<std::array<double, 6>> sums( books.size() );
+    for ( size_t b = 0; b < books.size(); b++ )
+    {
+        std::array<double, 6> sm{}, cp{};
+        for ( uint32_t k = 0; k < B.nblocks; k++ )
+            for ( int j = 0; j < 6; j++ )
+            {
+                const double y = part[B.part_off[b] + (size_t)k * 6 + j] - cp[j];
+                const double t = sm[j] + y;
+                cp[j] = ( t - sm[j] ) - y;
+                sm[j] = t;
+            }
+        sums[b] = sm;
+    }
+    auto mean = [&]( int b, int e ) { return sums[b][2 * e] / n; };
+    auto trust = [&]( int b, int e ) {
+        const double s1 = sums[b][2 * e], s2 = sums[b][2 * e + 1];
+        return std::sqrt( std::max( 0.0, ( s2 - s1 * s1 / n ) / ( n - 1 ) / n ) );
+    };
+    res->premium = mean( 0, 0 );
+    res->premium_trust = trust( 0, 0 );
+    res->delta = mean( 0, 1 );
+    res->delta_trust = trust( 0, 1 );
+    res->vega = mean( 0, 2 );
+    res->vega_trust = trust( 0, 2 );
+    res->gamma = gamma ? ( mean( 1, 1 ) - mean( 2, 1 ) ) / ( 2 * h ) : 0.0;
+    res->sweep_ms = ms;
+    return THOTH_OK;
+}
+
+}  // namespace thoth::host
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_pathwise( const thoth_mcl_request* req,
+                                                                                  thoth_pathwise_result* res )
+{
+    if ( !req || !res )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    try
+    {
+        int rc = thoth::EnsureDevice();
+        if ( rc != THOTH_OK )
+            return rc;
+        return thoth::host::PricePathwise( *req, res );
     }
     catch ( const thoth::host::Invalid& e )
     {

@@ -49,6 +49,7 @@ struct DeviceBatch
     size_t lds_bytes = 0;
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
     int cv_mode = 0;  // 0 none, 1 control-variate samples, 2 in-loop hedge inference
+    bool pathwise = false;  // pathwise-Greek sweep: three estimators per block
     // vanilla grids: per-(replicate, block) partials and their per-book sums
     // [nm][2 nk][2] (device), offsets per book; grid_cells[b] = nm * 2 nk
     double* dgrid = nullptr;
@@ -97,7 +98,7 @@ struct CvSums
     double sum = 0, sum2 = 0;
 };
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr );
+                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, bool pathwise = false );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
 // grid sums of every book, [nm][2 nk][2] each, from the last sweep (device reduction + copy)
 int ReduceGrids( const DeviceBatch& B, hipStream_t s );

@@ -336,6 +336,103 @@ __device__ __forceinline__ bool qe_mgf_m1( const QeDraw& d, float A, float& out 
     return true;
 }
 
+// Brownian bridge of the reference schedule (path_generator.cpp:49-198)
+// replayed as an in-order tree walk: the endpoint first, then per step the
+// pending points of the step are built (their right anchors pushed on a
+// per-factor stack held in VGPRs, indexed by a wave-uniform position) and
+// W(t_i) is popped as the normalised increment.  The walk depends on the
+// Sobol points only, never on the path state.
+template <class R, int NF, int DEPTH>
+struct BridgeWalk
+{
+    typedef R stackv __attribute__( ( ext_vector_type( DEPTH ) ) );
+    stackv stk[NF];
+    R top[NF], prev[NF];
+    int sp;
+
+    // endpoint first (build index 0, path_generator.cpp:166-167)
+    __device__ __forceinline__ void start( const DevBook& B, const SobolLane& sob )
+    {
+        const int F = B.factors;
+#pragma unroll
+        for ( int f = 0; f < NF; f++ )
+        {
+            stk[f] = (stackv)( R( 0 ) );
+            prev[f] = R( 0 );
+            top[f] = R( 0 );
+            if ( f < F )
+            {
+                const R z = normal_of<R>( sob( f ) );
+                if constexpr ( std::is_same_v<R, double> )
+                    top[f] = B.sd0 * z;
+                else
+                    top[f] = B.sd0f * z;
+            }
+        }
+        sp = 1;
+    }
+
+    // bridge step p: build the pending points, then pop W(t_{p+1}) into nz
+    // (normalised increment per factor, path_generator.cpp:175-196)
+    template <class V>
+    __device__ __forceinline__ void step( const DevBook& B, const SobolLane& sob, int p, V& nz )
+    {
+        constexpr bool F64 = std::is_same_v<R, double>;
+        const int F = B.factors;
+        const int i = p + 1;
+        const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
+        for ( int o = o0; o < o1; o++ )
+        {
+            const int bidx = B.op_b[o];
+            const bool left = B.op_left[o] != 0;
+            const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
+            R lw, rw, sd;
+            if constexpr ( F64 )
+            {
+                lw = B.op_w[3 * o];
+                rw = B.op_w[3 * o + 1];
+                sd = B.op_w[3 * o + 2];
+            }
+            else
+            {
+                lw = B.op_wf[3 * o];
+                rw = B.op_wf[3 * o + 1];
+                sd = B.op_wf[3 * o + 2];
+            }
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+            {
+                if ( f < F )
+                {
+                    const R z = normal_of<R>( sob( bidx * F + f ) );
+                    const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
+                    stk[f][pos] = top[f];
+                    top[f] = nv;
+                }
+            }
+            sp++;
+        }
+        const int pos = __builtin_amdgcn_readfirstlane( sp >= 2 ? sp - 2 : 0 );
+#pragma unroll
+        for ( int f = 0; f < NF; f++ )
+        {
+            nz[f] = R( 0 );
+            if ( f < F )
+            {
+                const R w = top[f];
+                if constexpr ( F64 )
+                    nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
+                else
+                    nz[f] = ( w - prev[f] ) * B.binv[i];
+                prev[f] = w;
+                if ( sp >= 2 )
+                    top[f] = stk[f][pos];
+            }
+        }
+        sp--;
+    }
+};
+
 // LDS of the in-loop hedge inference (after the Sobol table and leverage
 // layers, 16-byte aligned): the BF16 A-operand image of the network, its
 // biases, and per wave a [64 paths][16] BF16 feature block and a [64][8]

@@ -49,6 +49,7 @@ def lib():
         L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
                                                    P(C.c_double), P(C.c_int64), P(C.c_double)]
+        L.thoth_mcl_price_pathwise.argtypes = [P(abi.Request), P(abi.PathwiseResult)]
         L.thoth_mcl_price_rqmc.argtypes = [P(abi.Request), C.c_int32, C.c_uint64, P(abi.RqmcResult)]
         L.thoth_grid_prepare.argtypes = [C.c_int32, P(abi.Request), P(abi.VanillaGrid), C.c_int32, P(C.c_void_p)]
         L.thoth_grid_run.argtypes = [C.c_void_p, P(abi.GridResult), P(C.c_double)]
@@ -108,6 +109,14 @@ def price_rqmc(built, replicates=16, rqmc_seed=1):
     (built.req.mcl.paths paths per replicate); returns abi.RqmcResult."""
     res = abi.RqmcResult()
     _check(lib().thoth_mcl_price_rqmc(C.byref(built.req), replicates, rqmc_seed, C.byref(res)))
+    return res
+
+
+def price_pathwise(built):
+    """Pathwise delta / vega (and gamma from pathwise deltas) of a single-equity
+    Black-Scholes or Heston book with one vanilla or knock-out contract."""
+    res = abi.PathwiseResult()
+    _check(lib().thoth_mcl_price_pathwise(C.byref(built.req), C.byref(res)))
     return res
 
 
