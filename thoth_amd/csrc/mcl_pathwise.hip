@@ -184,13 +184,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
     SobolLane sob;
     sob.V = V;
     sob.tab = lds;
-    sob.gw = __builtin_amdgcn_readfirstlane( gray32( 64u * (uint32_t)wib ) );
-    {
-        const uint32_t g = gray32( (uint32_t)lane );
-#pragma unroll
-        for ( int k = 0; k < 6; k++ )
-            sob.m[k] = ( ( g >> k ) & 1u ) ? 0xffffffffu : 0u;
-    }
+    sob.set_lane( wib, lane );
     BridgeWalk<R, 2, DEPTH> bw;
     bw.start( B, sob );
 

@@ -55,27 +55,35 @@ __device__ __forceinline__ float erfc_fast( float x )
 // n = 256B + 64w + j (w < 4, j < 64) gray(n) = gray(256B) ^ gray(64w) ^ gray(j)
 // (the summands occupy disjoint bits), so
 //   x_d = T_B[d] ^ XOR_{k in gray(64w)} V[d][k] ^ XOR_{k in gray(j)} V[d][k]
-// with T_B[d] (block part, bits >= 7, plus the RQMC digital shift) tabulated in
-// LDS once per block, the wave part (bits 5..7) selected in scalar registers
-// and the lane part (bits 0..5) by six per-lane masks.
+// with T_B[d] (block part, plus the RQMC digital shift) tabulated in LDS once
+// per block and the intra-block part gray(64 w + j) (bits 0..7) applied by
+// eight per-lane masks.
 struct SobolLane
 {
     const uint32_t* __restrict__ V;
     const uint32_t* tab;  // LDS, block part per dimension
-    uint32_t gw;          // gray(64 w), wave-uniform
-    uint32_t m[6];        // per-lane masks of gray(lane)
+    uint32_t m[8];        // per-lane masks of the bits of gray(64 w + lane)
+
+    // lane 'lane' of wave w: m[k] = all ones iff bit k of gray(64 w + lane) is set
+    __device__ __forceinline__ void set_lane( int w, int lane )
+    {
+        const uint32_t g = gray32( 64u * (uint32_t)w + (uint32_t)lane );
+#pragma unroll
+        for ( int k = 0; k < 8; k++ )
+            m[k] = ( ( g >> k ) & 1u ) ? 0xffffffffu : 0u;
+    }
 
     __device__ __forceinline__ uint32_t operator()( int d ) const
     {
         // d is wave-uniform: the eight direction numbers come in through the
         // scalar cache (one s_load_dwordx8) instead of per-lane vector loads
         const uint32_t* v = V + (size_t)__builtin_amdgcn_readfirstlane( d ) * 32;
+        // x ^= v[k] & m[k] as one v_bitop3_b32 each (truth table 0x6c of
+        // f(S0, S1, S2) = S1 ^ (S0 & S2), table bit index 4 S0 + 2 S1 + S2)
         uint32_t x = tab[d];
-        x ^= ( gw & 0x20u ) ? v[5] : 0u;
-        x ^= ( gw & 0x40u ) ? v[6] : 0u;
-        x ^= ( gw & 0x80u ) ? v[7] : 0u;
-        x ^= ( v[0] & m[0] ) ^ ( v[1] & m[1] ) ^ ( v[2] & m[2] );
-        x ^= ( v[3] & m[3] ) ^ ( v[4] & m[4] ) ^ ( v[5] & m[5] );
+#pragma unroll
+        for ( int k = 0; k < 8; k++ )
+            x = __builtin_amdgcn_bitop3_b32( v[k], x, m[k], 0x6c );
         return x;
     }
 };
@@ -506,13 +514,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     SobolLane sob;
     sob.V = V;
     sob.tab = tab;
-    sob.gw = __builtin_amdgcn_readfirstlane( gray32( 64u * (uint32_t)wib ) );
-    {
-        const uint32_t g = gray32( (uint32_t)lane );
-#pragma unroll
-        for ( int k = 0; k < 6; k++ )
-            sob.m[k] = ( ( g >> k ) & 1u ) ? 0xffffffffu : 0u;
-    }
+    sob.set_lane( wib, lane );
     // FP32 LSV books stage the leverage layers in LDS, G layers at a time into
     // a ring of 2G slots (layer i in slot (i - 1) mod 2G): one barrier per G
     // steps publishes the current group and frees the previous group's slots
@@ -822,6 +824,46 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     {
         const int i = p + 1;
         const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
+        if ( o1 - o0 == 1 && sp >= 1 )
+        {
+            // one build then one pop: the pushed right anchor would be popped
+            // straight back, so the new point is consumed without touching the
+            // stack (same arithmetic as the general walk below)
+            const int o = o0;
+            const int bidx = B.op_b[o];
+            const bool left = B.op_left[o] != 0;
+            R lw, rw, sd;
+            if constexpr ( F64 )
+            {
+                lw = B.op_w[3 * o];
+                rw = B.op_w[3 * o + 1];
+                sd = B.op_w[3 * o + 2];
+            }
+            else
+            {
+                lw = B.op_wf[3 * o];
+                rw = B.op_wf[3 * o + 1];
+                sd = B.op_wf[3 * o + 2];
+            }
+#pragma unroll
+            for ( int f = 0; f < NFV; f++ )
+                nz[f] = R( 0 );
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+            {
+                if ( f < F )
+                {
+                    const R z = normal_of<R>( sob( bidx * F + f ) );
+                    const R w = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
+                    if constexpr ( F64 )
+                        nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
+                    else
+                        nz[f] = ( w - prev[f] ) * B.binv[i];
+                    prev[f] = w;
+                }
+            }
+            return;
+        }
         for ( int o = o0; o < o1; o++ )
         {
             const int bidx = B.op_b[o];

@@ -21,10 +21,7 @@ __global__ void __launch_bounds__( kBlock ) sobol_points_kernel( const uint32_t*
     SobolLane sob;
     sob.V = V;
     sob.tab = tab;
-    sob.gw = __builtin_amdgcn_readfirstlane( gray32( 64u * ( threadIdx.x >> 6 ) ) );
-    const uint32_t g = gray32( threadIdx.x & 63u );
-    for ( int k = 0; k < 6; k++ )
-        sob.m[k] = ( ( g >> k ) & 1u ) ? 0xffffffffu : 0u;
+    sob.set_lane( threadIdx.x >> 6, threadIdx.x & 63 );
     const bool valid = idx >= first && idx <= last;
     for ( int d = 0; d < dim; d++ )
     {
