@@ -127,6 +127,15 @@ typedef struct thoth_mcl_request {
     int32_t correlate_stochastic;  /* 1: Cholesky-correlate Heston/LSV spots too (the reference
                                       leaves them uncorrelated, pricer_mcl.cpp:280-284) */
     int32_t uniform_steps;         /* > 0: replace the calendar grid by uniform steps t_i = i*T/n */
+    /* LSV leverage layout.  0, 0: one 201-point layer per diffusion date, each on
+     * its own +-8 sigma sqrt(t) log-spot axis (single.cpp:175-181, the reference's).
+     * spot_points > 1, time_layers > 0: a surface of min(time_layers, dates)
+     * layers calibrated on diffusion dates spread evenly in sqrt(t), all on one
+     * log-spot axis of spot_points points (+-4 sigma_ATM sqrt(T) at the last
+     * date), read between layers linearly in time as LeverageSurface::GetLeverage
+     * does (leverage_surface.cpp:43-66). */
+    int32_t leverage_spot_points;
+    int32_t leverage_time_layers;
 } thoth_mcl_request;
 
 typedef struct thoth_mcl_result {
@@ -201,11 +210,13 @@ int thoth_mcl_grid(const thoth_mcl_request* req, int32_t cap, double* days, doub
 
 /* FP64 LSV leverage calibration of equity `asset` on the request's grid, with
  * the equity spot replaced by `spot` (reference: Single::CalibrateLeverage,
- * marketdata/single.cpp:400-473).  Fills cap_dates layers of 201 points:
- * ln_step[k], offset[k], levels[k*201 + i].  Returns the layer count or a
- * negative THOTH_* status. */
-int thoth_mcl_calibrate_leverage(const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_dates,
-                                 double* ln_step, int64_t* offset, double* levels);
+ * marketdata/single.cpp:400-473).  Fills up to cap_layers calibrated layers of
+ * P points, P = 201 for the per-date layout and leverage_spot_points for the
+ * surface layout: layer time t[k] (years), ln_step[k], offset[k] and
+ * levels[k*P + i] at log-spot (offset[k] + i) * ln_step[k].  Returns the layer
+ * count or a negative THOTH_* status. */
+int thoth_mcl_calibrate_leverage(const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_layers,
+                                 double* t, double* ln_step, int64_t* offset, double* levels);
 
 #ifdef __cplusplus
 }

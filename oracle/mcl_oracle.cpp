@@ -612,6 +612,7 @@ namespace
 {
 constexpr int LOCAL_VOL_GRID_POINTS = 201;
 constexpr double LOCAL_VOL_GRID_SIGMA_FACTOR = 8.0;
+constexpr double SURFACE_SIGMA_FACTOR = 4.0;  // the engine's surface layout: one +-4 sigma sqrt(T) axis
 constexpr int LSV_CALIB_PATHS = 16384;
 constexpr int LSV_CALIB_BINS = 30;
 constexpr int LSV_CALIB_MIN_BIN_COUNT = 20;
@@ -744,8 +745,108 @@ double LeverageLayers::LayerAt( size_t K, double S ) const
     return ( 1.0 - w ) * lev[i] + w * lev[i + 1];
 }
 
+double LeverageLayers::GetLeverage( double S, double T ) const
+{
+    if ( t.empty() )
+        return 1.0;
+    if ( T <= t.front() )
+        return LayerAt( 0, S );
+    if ( T >= t.back() )
+        return LayerAt( t.size() - 1, S );
+    size_t hi = 1;
+    while ( t[hi] < T )
+        hi++;
+    const size_t lo = hi - 1;
+    const double w = ( T - t[lo] ) / ( t[hi] - t[lo] );
+    return ( 1.0 - w ) * LayerAt( lo, S ) + w * LayerAt( hi, S );
+}
+
+namespace
+{
+// The engine's production layout (no reference counterpart): min(time_layers,
+// dates) layers calibrated on diffusion dates spread evenly in sqrt(t), all on
+// one log-spot axis of spot_points points, +-4 sigma_ATM(T) sqrt(T) around
+// log S0 at the last date T.  The particles step through every
+// diffusion date exactly as in Single::CalibrateLeverage (single.cpp:400-473),
+// each step on the latest calibrated layer; layer k is built from the ensemble
+// entering date at[k].
+LeverageLayers CalibrateLeverageSurface( const Market& mk, int a, const std::vector<double>& Dates )
+{
+    const StochParams h = mk.Stoch( a );
+    const double today = Dates.front();
+    const double s0 = mk.eq[a].spot - 0.0;
+    const int P = mk.lev_spot_points;
+    const size_t nd = Dates.size();
+    const size_t nt = std::min<size_t>( (size_t)mk.lev_time_layers, nd );
+    auto cum_carry = [&]( double d )
+    {
+        const double r = mk.rate.Value( d );
+        return ( r - mk.DivRepo( a, d ) ) * YearFraction( today, d );
+    };
+    double T = YearFraction( today, Dates.back() );
+    if ( T <= 0 )
+        T = 1.0 / NB_OF_DAYS_A_YEAR;
+    const double v_atm = mk.ImplicitVol( a, 0, Dates.back() );
+    const double half_width = SURFACE_SIGMA_FACTOR * v_atm * std::sqrt( T );
+    double ln_step = 2.0 * half_width / ( P - 1 );
+    if ( ln_step <= 0 )
+        ln_step = 1e-4;
+    const long offset = (long)std::floor( ( std::log( s0 ) - half_width ) / ln_step );
+    Rng rng( LSV_CALIB_SEED );
+    std::vector<double> spot( LSV_CALIB_PATHS, s0 );
+    std::vector<double> var( LSV_CALIB_PATHS, h.v0 );
+    std::vector<double> ln_spot( LSV_CALIB_PATHS, std::log( s0 ) );
+    // calibration dates evenly spread in sqrt(t): at[k] = round(k^2 (nd-1) / (nt-1)^2),
+    // at least one date after the previous one
+    std::vector<size_t> at( nt, 0 );
+    for ( size_t k = 1; k < nt; k++ )
+        at[k] = std::max( at[k - 1] + 1, ( 2 * k * k * ( nd - 1 ) + ( nt - 1 ) * ( nt - 1 ) ) / ( 2 * ( nt - 1 ) * ( nt - 1 ) ) );
+    LeverageLayers lev;
+    lev.surface = true;
+    size_t next = 0;
+    for ( size_t k = 0; k < nd; k++ )
+    {
+        if ( next < nt && at[next] == k )
+        {
+            const ConditionalVariance cond_var( ln_spot, var );
+            std::vector<double> layer( P );
+            for ( int i = 0; i < P; i++ )
+            {
+                const double x = ( offset + i ) * ln_step;
+                const double sig = mk.LocalVol( a, std::exp( x ), Dates[k] );
+                const double ev = std::max( cond_var( x ), LSV_MIN_COND_VARIANCE );
+                layer[i] = std::clamp( sig / std::sqrt( ev ), LSV_LEVERAGE_MIN, LSV_LEVERAGE_MAX );
+            }
+            lev.t.push_back( YearFraction( today, Dates[k] ) );
+            lev.ln_step.push_back( ln_step );
+            lev.offset.push_back( offset );
+            lev.levels.push_back( std::move( layer ) );
+            next++;
+        }
+        if ( k == 0 )
+            continue;
+        const size_t cur = next - 1;
+        const double dt = YearFraction( Dates[k - 1], Dates[k] );
+        const double carry = cum_carry( Dates[k] ) - cum_carry( Dates[k - 1] );
+        for ( int p = 0; p < LSV_CALIB_PATHS; p++ )
+        {
+            const double lp = lev.LayerAt( cur, spot[p] );
+            const double vp = var[p];
+            const double vc = QeVarianceStep( vp, dt, rng.Gaussian(), h );
+            const double step = LeveragedLogSpotStep( vp, vc, dt, carry, lp, rng.Gaussian(), h );
+            var[p] = vc;
+            spot[p] *= std::exp( step );
+            ln_spot[p] += step;
+        }
+    }
+    return lev;
+}
+}  // namespace
+
 LeverageLayers CalibrateLeverage( const Market& mk, int a, const std::vector<double>& Dates )
 {
+    if ( mk.lev_time_layers > 0 )
+        return CalibrateLeverageSurface( mk, a, Dates );
     const StochParams h = mk.Stoch( a );
     const double today = Dates.front();
     const double s0 = mk.eq[a].spot - 0.0;
@@ -839,6 +940,8 @@ Market BuildMarket( const thoth_mcl_request& req )
     }
     if ( req.correlation )
         mk.correlation.assign( req.correlation, req.correlation + req.n_assets * req.n_assets );
+    mk.lev_spot_points = req.leverage_spot_points;
+    mk.lev_time_layers = req.leverage_time_layers;
     return mk;
 }
 
@@ -1165,19 +1268,25 @@ BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, i
                 if ( b.model[a] == THOTH_MODEL_LSV )
                 {
                     // local_volatility_node.cpp:19-47: layer i at the previous spot
+                    // (a surface: LeverageSurface::GetLeverage at the date's time)
                     const LeverageLayers& lv = b.lev[a];
-                    const std::vector<double>& vv = lv.levels[i];
-                    const double f = std::log( S[a][i - 1] ) / lv.ln_step[i] - (double)lv.offset[i];
-                    const size_t last = vv.size() - 1;
-                    if ( f <= 0.0 )
-                        L = vv[0];
-                    else if ( f >= (double)last )
-                        L = vv[last];
+                    if ( lv.surface )
+                        L = lv.GetLeverage( S[a][i - 1], b.grid.t[i] );
                     else
                     {
-                        const size_t k = (size_t)f;
-                        const double w = f - (double)k;
-                        L = ( 1.0 - w ) * vv[k] + w * vv[k + 1];
+                        const std::vector<double>& vv = lv.levels[i];
+                        const double f = std::log( S[a][i - 1] ) / lv.ln_step[i] - (double)lv.offset[i];
+                        const size_t last = vv.size() - 1;
+                        if ( f <= 0.0 )
+                            L = vv[0];
+                        else if ( f >= (double)last )
+                            L = vv[last];
+                        else
+                        {
+                            const size_t k = (size_t)f;
+                            const double w = f - (double)k;
+                            L = ( 1.0 - w ) * vv[k] + w * vv[k + 1];
+                        }
                     }
                 }
                 const double z = corr[a];

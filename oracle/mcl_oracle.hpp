@@ -130,6 +130,7 @@ struct Market
     std::vector<Equity> eq;
     Curve rate;
     std::vector<double> correlation;
+    int lev_spot_points = 0, lev_time_layers = 0;    // 0, 0: one 201-point layer per diffusion date
     double today = 0;                                // days
     StochParams Stoch( int a ) const;                // StochasticParams()
     double ImplicitVol( int a, double strike, double day ) const;   // Single::GetImplicitVol
@@ -143,7 +144,9 @@ struct LeverageLayers                                // leverage_surface.hpp
     std::vector<double> t, ln_step;
     std::vector<long> offset;
     std::vector<std::vector<double>> levels;
+    bool surface = false;                            // read by time (GetLeverage), not by date index
     double LayerAt( size_t k, double s ) const;      // leverage_surface.cpp:19-40
+    double GetLeverage( double s, double T ) const;  // leverage_surface.cpp:43-66
 };
 LeverageLayers CalibrateLeverage( const Market& mk, int a, const std::vector<double>& date_days );
 

@@ -130,19 +130,23 @@ def grid(built, cap=100000):
     return days[:n].copy(), t[:n].copy(), dt[:n].copy()
 
 
-def calibrate(built, asset, spot=None, vol_shift=0.0, rate_shift=0.0, points=201):
+def calibrate(built, asset, spot=None, vol_shift=0.0, rate_shift=0.0):
+    """Calibrated leverage layers: one per diffusion date (201 points), or the
+    surface's time layers on their shared axis."""
     days, _, _ = grid(built)
     n = days.size
+    r = built.req
+    points = r.leverage_spot_points if r.leverage_time_layers > 0 else 201
     t, ln_step = np.zeros(n), np.zeros(n)
     offset = np.zeros(n, np.int64)
     levels = np.zeros((n, points))
     if spot is None:
         spot = built.spec.equities[asset].spot
-    k = lib().oracle_calibrate(C.byref(built.req), asset, spot, vol_shift, rate_shift, n, _p(t), _p(ln_step),
+    k = lib().oracle_calibrate(C.byref(r), asset, spot, vol_shift, rate_shift, n, _p(t), _p(ln_step),
                                _p(offset, C.c_int64), _p(levels))
     if k < 0:
         raise RuntimeError(lib().oracle_last_error().decode())
-    return dict(t=t, ln_step=ln_step, offset=offset, levels=levels)
+    return dict(t=t[:k], ln_step=ln_step[:k], offset=offset[:k], levels=levels[:k])
 
 
 def sweep(built):

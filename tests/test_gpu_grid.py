@@ -17,7 +17,7 @@ GREEKS = abi.GREEK_DELTA | abi.GREEK_VEGA
 
 def grid_book(eq, paths, precision):
     return BookSpec(mcl=McConfig(paths=paths, max_day_step=1), equities=[eq], contracts=[], rate_days=[0],
-                    rate_values=[3.0], greeks=GREEKS, precision=precision)
+                    rate_values=[3.0], greeks=GREEKS, precision=precision, leverage_surface=configs.LEVERAGE_SURFACE)
 
 
 def test_grid_fp64_matches_oracle_contracts(gpu, oracle):
@@ -33,7 +33,7 @@ def test_grid_fp64_matches_oracle_contracts(gpu, oracle):
             k = p * eq.spot / 100.0
             contracts += [Vanilla(0, d, k, is_call=True), Vanilla(0, d, k, is_call=False)]
     ref = BookSpec(mcl=McConfig(paths=1 << 12, max_day_step=1), equities=[eq], contracts=contracts, rate_days=[0],
-                   rate_values=[3.0], greeks=GREEKS, precision=abi.FP64)
+                   rate_values=[3.0], greeks=GREEKS, precision=abi.FP64, leverage_surface=configs.LEVERAGE_SURFACE)
     o = oracle.price(BuiltRequest(ref))
     worst = 0.0
     for j in range(len(MATS)):

@@ -95,16 +95,23 @@ def test_multi_asset_bs_heston_correlated_fp64(gpu, oracle):
     compare(gpu, oracle, spec)
 
 
-def _lsv_basket(n=2, days=40, paths=1 << 12, precision=abi.FP64, corr_stoch=False):
+def _lsv_basket(n=2, days=40, paths=1 << 12, precision=abi.FP64, corr_stoch=False, surface=None, greeks=0):
     eqs, corr = configs.lsv_assets(n)
     return BookSpec(mcl=McConfig(paths=paths, max_day_step=1), equities=eqs,
                     contracts=[Vanilla(-1, days, 100.0)], correlation=corr.tolist(), rate_values=[3.0],
                     rainbows=[RainbowUdl(list(range(n)), best=False)], precision=precision,
-                    correlate_stochastic=corr_stoch)
+                    correlate_stochastic=corr_stoch, leverage_surface=surface, greeks=greeks)
 
 
 def test_lsv_worst_of_fp64(gpu, oracle):
     compare(gpu, oracle, _lsv_basket())
+
+
+@pytest.mark.parametrize("surface", [(100, 12), (64, 50)])
+def test_lsv_leverage_surface_fp64(gpu, oracle, surface):
+    # the device reads per-date layers folded from the surface on the host; the
+    # oracle reads the surface itself with LeverageSurface::GetLeverage
+    compare(gpu, oracle, _lsv_basket(n=2, days=40, surface=surface, greeks=abi.GREEK_DELTA | abi.GREEK_VEGA))
 
 
 def test_lsv_correlated_mode_fp64(gpu, oracle):

@@ -1,0 +1,31 @@
+"""Host-side logic of the product library against the oracle, without a
+device: the FP64 LSV leverage calibration (both layouts) and request checks."""
+import numpy as np
+import pytest
+
+from thoth_amd import abi, mcl
+from thoth_amd.book import BookSpec, BuiltRequest, Equity, McConfig, Sabr, Vanilla
+
+
+def _lsv(days=40, surface=None):
+    eq = Equity(spot=100, vol_kind=abi.VOL_LSV, init_vol=18, long_vol=26, kappa=2.0, vol_of_vol=0.5,
+                rho_spot_var=-0.6, surface=Sabr([0.5, 2.0], [0.22, 0.24], [1, 1], [-0.5, -0.5], [0.6, 0.5]))
+    return BookSpec(mcl=McConfig(paths=1 << 10, max_day_step=1), equities=[eq], contracts=[Vanilla(0, days, 100.0)],
+                    rate_values=[3.0], leverage_surface=surface)
+
+
+@pytest.mark.parametrize("surface", [None, (100, 50), (64, 7)])
+def test_host_calibration_equals_oracle(oracle, surface):
+    b = BuiltRequest(_lsv(surface=surface))
+    for spot in (100.0, 101.0):
+        h = mcl.calibrate_leverage(b, 0, spot=spot)
+        o = oracle.calibrate(b, 0, spot=spot)
+        for key in ("t", "ln_step", "offset", "levels"):
+            assert h[key].shape == o[key].shape, key
+            assert np.array_equal(h[key], o[key]), key
+
+
+def test_surface_layout_is_checked():
+    for bad in ((1, 50), (513, 50), (100, 0), (0, 50)):
+        with pytest.raises(RuntimeError):
+            mcl.calibrate_leverage(BuiltRequest(_lsv(surface=bad)), 0)

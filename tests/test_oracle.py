@@ -316,3 +316,48 @@ def test_leverage_layers_shape(oracle):
     assert lev["levels"].shape == (21, 201)
     assert (lev["levels"] >= 0.05).all() and (lev["levels"] <= 10).all()
     assert lev["t"][0] == 0.0 and abs(lev["t"][-1] - 20 / 365) < 1e-15
+
+
+def test_leverage_surface_shape(oracle):
+    # production layout: 50 time layers on diffusion dates spread evenly in
+    # sqrt(t), one shared 100-point log-spot axis of +-4 sigma_ATM sqrt(T) around log S0
+    spec = _lsv_spec(days=200)
+    spec.leverage_surface = (100, 50)
+    b = BuiltRequest(spec)
+    days, t, _ = oracle.grid(b)
+    lev = oracle.calibrate(b, 0)
+    nd = days.size
+    assert lev["levels"].shape == (50, 100)
+    at = [0]
+    for k in range(1, 50):
+        at.append(max(at[-1] + 1, (2 * k * k * (nd - 1) + 49 ** 2) // (2 * 49 ** 2)))
+    assert at[-1] == nd - 1 and at[1] == 1
+    assert np.array_equal(lev["t"], t[at]) and lev["t"][0] == 0.0 and lev["t"][-1] == t[-1]
+    assert (lev["ln_step"] == lev["ln_step"][0]).all() and (lev["offset"] == lev["offset"][0]).all()
+    hw = 4.0 * oracle.sabr_vol(b.assets[0].surface, 0.0, 100.0, days[-1]) * math.sqrt(t[-1])
+    assert abs(lev["ln_step"][0] - 2 * hw / 99) < 1e-15
+    assert (lev["offset"][0] + np.arange(100))[[0, -1]] @ [1, 0] * lev["ln_step"][0] <= math.log(100) - hw
+    assert (lev["levels"] >= 0.05).all() and (lev["levels"] <= 10).all()
+    # fewer dates than layers: every date is a layer
+    spec.contracts = [Vanilla(0, 20, 100.0)]
+    short = oracle.calibrate(BuiltRequest(spec), 0)
+    assert short["levels"].shape == (21, 100)
+
+
+def test_lsv_surface_reprices_its_target(oracle):
+    # the same acceptance as pricer/tests/test_lsv.cpp:45-58 for the 100 x 50 surface
+    spec = _lsv_spec(paths=1 << 15, days=365)
+    spec.leverage_surface = (100, 50)
+    b = BuiltRequest(spec)
+    r = oracle.price(b)
+    cp = b.contract_values(r, "contract_premium", 3)
+    ct = b.contract_values(r, "contract_trust", 3)
+    sabr = b.assets[0].surface
+    T = 365 / 365
+    F = 100 * math.exp(0.02 * T)
+    for K, p, t in zip((90.0, 100.0, 110.0), cp, ct):
+        v = oracle.sabr_vol(sabr, K, F, 365)
+        d1 = (math.log(F / K) + 0.5 * v * v * T) / (v * math.sqrt(T))
+        ref = math.exp(-0.02 * T) * (F * norm.cdf(d1) - K * norm.cdf(d1 - v * math.sqrt(T)))
+        print("K %.0f surface %.4f +- %.4f target %.4f" % (K, p, t, ref))
+        assert abs(p - ref) < 6 * t + 0.10

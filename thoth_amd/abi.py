@@ -68,7 +68,8 @@ class Request(C.Structure):
                 ("correlation", _pd), ("rate", Curve), ("n_rainbows", _i32),
                 ("rainbows", C.POINTER(Rainbow)), ("n_contracts", _i32),
                 ("contracts", C.POINTER(Contract)), ("request_greeks", _i32),
-                ("precision", _i32), ("correlate_stochastic", _i32), ("uniform_steps", _i32)]
+                ("precision", _i32), ("correlate_stochastic", _i32), ("uniform_steps", _i32),
+                ("leverage_spot_points", _i32), ("leverage_time_layers", _i32)]
 
 
 class Result(C.Structure):

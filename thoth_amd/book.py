@@ -8,7 +8,7 @@ percent, dates in days from today).  A built request keeps every ctypes array
 it points to alive.
 """
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 import ctypes as C
 
 from . import abi
@@ -92,6 +92,9 @@ class BookSpec:
     precision: int = abi.FP64
     correlate_stochastic: bool = False
     uniform_steps: int = 0
+    # LSV leverage layout: (spot points, time layers) of one shared log-spot axis,
+    # or None for one 201-point layer per diffusion date
+    leverage_surface: Optional[Tuple[int, int]] = None
 
 
 class BuiltRequest:
@@ -167,6 +170,8 @@ class BuiltRequest:
         r.precision = spec.precision
         r.correlate_stochastic = int(spec.correlate_stochastic)
         r.uniform_steps = spec.uniform_steps
+        if spec.leverage_surface is not None:
+            r.leverage_spot_points, r.leverage_time_layers = spec.leverage_surface
         self.req = r
 
     def result(self):

@@ -12,7 +12,8 @@ with a fixed draw order, so every run rebuilds the same books.
 * lsv_worst_of_basket    -- 5-asset LSV worst-of call (rebased strike 100), 2y in
                             365 two-day steps, seeded Heston parameters, seeded
                             SABR target surfaces, seeded correlation repaired by
-                            eigenvalue clipping, 2^20 paths, FP32.
+                            eigenvalue clipping, leverage on a 100 spot x 50 time
+                            surface, 2^20 paths, FP32.
 """
 import math
 
@@ -116,14 +117,19 @@ def lsv_assets(n=5, seed=SEED):
     return eqs, repair_correlation(c)
 
 
+# the LSV leverage function of the basket workloads: 100 log-spot points x 50 time layers
+LEVERAGE_SURFACE = (100, 50)
+
+
 def lsv_worst_of_basket(paths=1 << 20, precision=abi.FP32, n_assets=5, maturity_days=730, max_day_step=2,
-                        correlate_stochastic=False, greeks=0):
+                        correlate_stochastic=False, greeks=0, leverage_surface=LEVERAGE_SURFACE):
     eqs, corr = lsv_assets(n_assets)
     return BookSpec(mcl=McConfig(paths=paths, max_day_step=max_day_step), equities=eqs,
                     contracts=[Vanilla(-1, maturity_days, 100.0, is_call=True)],
                     correlation=corr.tolist(), rate_days=[0], rate_values=[3.0],
                     rainbows=[RainbowUdl(members=list(range(n_assets)), best=False)],
-                    greeks=greeks, precision=precision, correlate_stochastic=correlate_stochastic)
+                    greeks=greeks, precision=precision, correlate_stochastic=correlate_stochastic,
+                    leverage_surface=leverage_surface)
 
 
 # Vanilla grids on the LSV underlyings: 41 strikes 60..140 % of spot, 12 monthly
@@ -133,14 +139,15 @@ GRID_MATURITIES = [round(365 * k / 12) for k in range(1, 13)]
 
 
 def lsv_vanilla_grids(paths=1 << 18, precision=abi.FP32, greeks=abi.GREEK_DELTA | abi.GREEK_VEGA, n_underlyings=5,
-                      strikes=None, maturities=None):
+                      strikes=None, maturities=None, leverage_surface=LEVERAGE_SURFACE):
     """(BookSpec, grid arguments) per underlying: single-equity LSV books of
     the basket's underlyings, no contracts beyond the grid."""
     eqs, _ = lsv_assets(5)
     out = []
     for a in range(n_underlyings):
         spec = BookSpec(mcl=McConfig(paths=paths, max_day_step=1), equities=[eqs[a]], contracts=[],
-                        rate_days=[0], rate_values=[3.0], greeks=greeks, precision=precision)
+                        rate_days=[0], rate_values=[3.0], greeks=greeks, precision=precision,
+                        leverage_surface=leverage_surface)
         out.append((spec, dict(underlying=0, maturity_days=maturities or GRID_MATURITIES,
                                strikes=strikes or GRID_STRIKES)))
     return out

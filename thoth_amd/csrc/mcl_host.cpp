@@ -832,8 +832,8 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_grid( cons
 }
 
 extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_calibrate_leverage(
-    const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_dates, double* ln_step, int64_t* offset,
-    double* levels )
+    const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_layers, double* t, double* ln_step,
+    int64_t* offset, double* levels )
 {
     if ( !req || asset < 0 || asset >= req->n_assets )
         return -thoth::SetError( THOTH_ERR_INVALID, "asset index" );
@@ -848,12 +848,15 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_calibrate_
         int rc = thoth::host::CalibrateLeverage( mk, asset, g, L );
         if ( rc != THOTH_OK )
             return -rc;
-        const int n = (int)L.t.size();
-        for ( int k = 0; k < n && k < cap_dates; k++ )
+        const bool surf = !L.surface.t.empty();
+        const int n = surf ? (int)L.surface.t.size() : (int)L.t.size();
+        const std::vector<double>& v = surf ? L.surface.values : L.values;
+        for ( int k = 0; k < n && k < cap_layers; k++ )
         {
-            ln_step[k] = L.ln_step[k];
-            offset[k] = L.offset[k];
-            std::copy( L.values.begin() + (size_t)k * L.points, L.values.begin() + (size_t)( k + 1 ) * L.points,
+            t[k] = surf ? L.surface.t[k] : L.t[k];
+            ln_step[k] = surf ? L.surface.ln_step : L.ln_step[k];
+            offset[k] = surf ? L.surface.offset : L.offset[k];
+            std::copy( v.begin() + (size_t)k * L.points, v.begin() + (size_t)( k + 1 ) * L.points,
                        levels + (size_t)k * L.points );
         }
         return n;

@@ -17,6 +17,7 @@ namespace thoth::host
 {
 
 constexpr double kDaysPerYear = 365.0;  // helpers/constants.hpp:7 (ACT/365)
+constexpr int kMaxLevPoints = 512;      // spot points of a leverage surface (FP32 LDS ring bound)
 
 inline double YearFraction( double from, double to ) { return ( to - from ) / kDaysPerYear; }  // misc.hpp:54-57
 
@@ -58,6 +59,7 @@ struct Market
 {
     std::vector<Equity> eq;
     Curve rate;
+    int lev_spot_points = 0, lev_time_layers = 0;  // 0, 0: one 201-point layer per diffusion date
     StochParams Stoch( int a ) const;                       // StochasticParams()
     double DivRepo( int a, double day ) const;              // equity.cpp:149-154
     double Drift( int a, double day ) const;                // drift_node.cpp:20-27
@@ -72,13 +74,23 @@ struct Grid
     bool uniform = false;
 };
 
-// leverage layers of one equity on the diffusion grid (leverage_surface.hpp)
+// leverage layers of one equity on the diffusion grid (leverage_surface.hpp):
+// layer i is read by the step into diffusion date i.  A surface calibration
+// keeps its calibrated layers in `surface` and fills the per-date layers by
+// interpolating them linearly in time (leverage_surface.cpp:43-66).
 struct LeverageGrid
 {
     int points = 0;
     std::vector<double> t, ln_step;
     std::vector<int64_t> offset;
     std::vector<double> values;  // [dates * points]
+    struct Surface
+    {
+        std::vector<double> t;       // calibration dates, years
+        double ln_step = 0;
+        int64_t offset = 0;
+        std::vector<double> values;  // [layers * points]
+    } surface;
 };
 
 // the arrays one thoth_mcl_book points to (the book keeps pointers into them)

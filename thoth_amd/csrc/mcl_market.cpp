@@ -266,6 +266,14 @@ Market BuildMarket( const thoth_mcl_request& req )
         mk.rate.days.push_back( 0 );
         mk.rate.values.push_back( 0 );
     }
+    const int sp = req.leverage_spot_points, tl = req.leverage_time_layers;
+    if ( sp != 0 || tl != 0 )
+    {
+        if ( sp < 2 || sp > kMaxLevPoints || tl < 1 )
+            throw Invalid( THOTH_ERR_INVALID, "leverage surface needs 2..512 spot points and at least one time layer" );
+        mk.lev_spot_points = sp;
+        mk.lev_time_layers = tl;
+    }
     return mk;
 }
 
@@ -294,6 +302,7 @@ namespace
 {
 constexpr int kGridPoints = 201;          // single.cpp:179
 constexpr double kGridSigmas = 8.0;       // single.cpp:177
+constexpr double kSurfaceSigmas = 4.0;    // half-width of a surface's shared log-spot axis
 constexpr int kParticles = 16384;         // single.cpp:233
 constexpr int kBins = 30;
 constexpr int kMinBinCount = 20;
@@ -453,112 +462,132 @@ int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& lev
     const double s0 = mk.eq[a].spot - 0.0;
     const double ls0 = std::log( s0 );
     auto cum_carry = [&]( double d ) { return ( mk.rate.Value( d ) - mk.DivRepo( a, d ) ) * YearFraction( 0.0, d ); };
-    // layer geometry and the Dupire target on every layer are particle-free:
-    // evaluated first, in parallel
-    lev.points = kGridPoints;
-    lev.t.assign( nd, 0 );
-    lev.ln_step.assign( nd, 0 );
-    lev.offset.assign( nd, 0 );
-    lev.values.assign( nd * kGridPoints, 0 );
-    std::vector<double> sig( nd * kGridPoints );
-    for ( size_t k = 0; k < nd; k++ )
+    // calibration dates: every diffusion date (the reference), or for a surface
+    // nt of them spread evenly in sqrt(t), at[k] = round(k^2 (nd-1) / (nt-1)^2)
+    // and at least one date after the previous one (the leverage moves fastest
+    // near t = 0, where linear-in-time reads of evenly spread layers bias LSV
+    // vanillas by several trusts at one year)
+    const bool surf = mk.lev_time_layers > 0;
+    const int P = surf ? mk.lev_spot_points : kGridPoints;
+    const size_t nt = surf ? std::min<size_t>( (size_t)mk.lev_time_layers, nd ) : nd;
+    std::vector<size_t> at( nt, 0 );
+    for ( size_t k = 1; k < nt; k++ )
+        at[k] = !surf ? k
+                      : std::max( at[k - 1] + 1, ( 2 * k * k * ( nd - 1 ) + ( nt - 1 ) * ( nt - 1 ) ) /
+                                                     ( 2 * ( nt - 1 ) * ( nt - 1 ) ) );
+    // layer geometry: its own +-8 sigma sqrt(t) axis per date (single.cpp:175-181), or
+    // one +-4 sigma sqrt(T) axis at the last date for the whole surface
+    std::vector<double> lt( nt ), lstep( nt );
+    std::vector<int64_t> loff( nt );
+    for ( size_t k = 0; k < nt; k++ )
     {
-        double T = YearFraction( 0.0, D[k] );
-        lev.t[k] = T;
+        const double ax_day = surf ? D[nd - 1] : D[at[k]];
+        lt[k] = YearFraction( 0.0, D[at[k]] );
+        double T = YearFraction( 0.0, ax_day );
         if ( T <= 0 )
             T = 1.0 / kDaysPerYear;
-        const double v_atm = mk.ImplicitVol( a, 0, D[k] );
-        const double hw = kGridSigmas * v_atm * std::sqrt( T );
-        double ln_step = 2.0 * hw / ( kGridPoints - 1 );
+        const double v_atm = mk.ImplicitVol( a, 0, ax_day );
+        const double hw = ( surf ? kSurfaceSigmas : kGridSigmas ) * v_atm * std::sqrt( T );
+        double ln_step = 2.0 * hw / ( P - 1 );
         if ( ln_step <= 0 )
             ln_step = 1e-4;
-        lev.ln_step[k] = ln_step;
-        lev.offset[k] = (long)std::floor( ( ls0 - hw ) / ln_step );
+        lstep[k] = ln_step;
+        loff[k] = (long)std::floor( ( ls0 - hw ) / ln_step );
     }
-    ParallelFor( (int)( nd * kGridPoints ), [&]( int j ) {
-        const size_t k = j / kGridPoints;
-        const int i = j % kGridPoints;
-        const double x = ( lev.offset[k] + i ) * lev.ln_step[k];
-        sig[j] = mk.LocalVol( a, std::exp( x ), D[k] );
+    // the Dupire target on every layer is particle-free: evaluated first, in parallel
+    std::vector<double> sig( nt * P ), layers( nt * P, 0.0 );
+    ParallelFor( (int)( nt * P ), [&]( int j ) {
+        const size_t k = j / P;
+        const int i = j % P;
+        const double x = ( loff[k] + i ) * lstep[k];
+        sig[j] = mk.LocalVol( a, std::exp( x ), D[at[k]] );
     } );
     const std::vector<double>& Z = CalibNormals( nd > 1 ? nd - 1 : 1 );
     std::vector<double> spot( kParticles, s0 ), var( kParticles, h.v0 ), ln_spot( kParticles, ls0 );
     std::vector<double> bsum( kBins ), bmean( kBins );
     std::vector<int> bcnt( kBins );
+    size_t next = 0;  // next layer to calibrate
     for ( size_t k = 0; k < nd; k++ )
     {
-        // ConditionalVariance of the current ensemble (single.cpp:263-316)
-        double fallback = 0;
-        for ( double v : var )
-            fallback += v;
-        fallback /= (double)var.size();
-        const auto [mn, mx] = std::minmax_element( ln_spot.begin(), ln_spot.end() );
-        const bool flat = *mx - *mn < 1e-10;
-        double x0 = 0, dx = 1;
-        if ( !flat )
+        if ( next < nt && at[next] == k )
         {
-            const double width = ( *mx - *mn ) / kBins;
-            std::fill( bsum.begin(), bsum.end(), 0.0 );
-            std::fill( bcnt.begin(), bcnt.end(), 0 );
-            for ( int p = 0; p < kParticles; p++ )
+            // ConditionalVariance of the current ensemble (single.cpp:263-316)
+            double fallback = 0;
+            for ( double v : var )
+                fallback += v;
+            fallback /= (double)var.size();
+            const auto [mn, mx] = std::minmax_element( ln_spot.begin(), ln_spot.end() );
+            const bool flat = *mx - *mn < 1e-10;
+            double x0 = 0, dx = 1;
+            if ( !flat )
             {
-                int b = (int)( ( ln_spot[p] - *mn ) / width );
-                b = std::clamp( b, 0, kBins - 1 );
-                bsum[b] += var[p];
-                bcnt[b]++;
-            }
-            std::fill( bmean.begin(), bmean.end(), fallback );
-            for ( int b = 0; b < kBins; b++ )
-                if ( bcnt[b] >= kMinBinCount )
-                    bmean[b] = bsum[b] / bcnt[b];
-            for ( int b = 0; b < kBins; b++ )
-            {
-                if ( bcnt[b] >= kMinBinCount )
-                    continue;
-                for ( int d = 1; d < kBins; d++ )
+                const double width = ( *mx - *mn ) / kBins;
+                std::fill( bsum.begin(), bsum.end(), 0.0 );
+                std::fill( bcnt.begin(), bcnt.end(), 0 );
+                for ( int p = 0; p < kParticles; p++ )
                 {
-                    if ( b - d >= 0 && bcnt[b - d] >= kMinBinCount )
+                    int b = (int)( ( ln_spot[p] - *mn ) / width );
+                    b = std::clamp( b, 0, kBins - 1 );
+                    bsum[b] += var[p];
+                    bcnt[b]++;
+                }
+                std::fill( bmean.begin(), bmean.end(), fallback );
+                for ( int b = 0; b < kBins; b++ )
+                    if ( bcnt[b] >= kMinBinCount )
+                        bmean[b] = bsum[b] / bcnt[b];
+                for ( int b = 0; b < kBins; b++ )
+                {
+                    if ( bcnt[b] >= kMinBinCount )
+                        continue;
+                    for ( int d = 1; d < kBins; d++ )
                     {
-                        bmean[b] = bsum[b - d] / bcnt[b - d];
-                        break;
-                    }
-                    if ( b + d < kBins && bcnt[b + d] >= kMinBinCount )
-                    {
-                        bmean[b] = bsum[b + d] / bcnt[b + d];
-                        break;
+                        if ( b - d >= 0 && bcnt[b - d] >= kMinBinCount )
+                        {
+                            bmean[b] = bsum[b - d] / bcnt[b - d];
+                            break;
+                        }
+                        if ( b + d < kBins && bcnt[b + d] >= kMinBinCount )
+                        {
+                            bmean[b] = bsum[b + d] / bcnt[b + d];
+                            break;
+                        }
                     }
                 }
+                x0 = *mn + 0.5 * width;
+                dx = width;
             }
-            x0 = *mn + 0.5 * width;
-            dx = width;
-        }
-        auto cond = [&]( double X )  // single.cpp:318-337
-        {
-            if ( flat )
-                return fallback;
-            const double f = ( X - x0 ) / dx;
-            const double last = (double)( kBins - 1 );
-            if ( f <= 0 )
-                return bmean.front();
-            if ( f >= last )
-                return bmean.back();
-            const size_t i = (size_t)f;
-            const double w = f - (double)i;
-            return ( 1.0 - w ) * bmean[i] + w * bmean[i + 1];
-        };
-        double* layer = &lev.values[k * kGridPoints];
-        for ( int i = 0; i < kGridPoints; i++ )
-        {
-            const double x = ( lev.offset[k] + i ) * lev.ln_step[k];
-            const double ev = std::max( cond( x ), kMinCondVar );
-            layer[i] = std::clamp( sig[k * kGridPoints + i] / std::sqrt( ev ), kLevMin, kLevMax );
+            auto cond = [&]( double X )  // single.cpp:318-337
+            {
+                if ( flat )
+                    return fallback;
+                const double f = ( X - x0 ) / dx;
+                const double last = (double)( kBins - 1 );
+                if ( f <= 0 )
+                    return bmean.front();
+                if ( f >= last )
+                    return bmean.back();
+                const size_t i = (size_t)f;
+                const double w = f - (double)i;
+                return ( 1.0 - w ) * bmean[i] + w * bmean[i + 1];
+            };
+            double* layer = &layers[next * P];
+            for ( int i = 0; i < P; i++ )
+            {
+                const double x = ( loff[next] + i ) * lstep[next];
+                const double ev = std::max( cond( x ), kMinCondVar );
+                layer[i] = std::clamp( sig[next * P + i] / std::sqrt( ev ), kLevMin, kLevMax );
+            }
+            next++;
         }
         if ( k == 0 )
             continue;
+        // the particles step into date k on the latest calibrated layer
+        const size_t cur = next - 1;
+        const double* layer = &layers[cur * P];
         const double dt = YearFraction( D[k - 1], D[k] );
         const double carry = cum_carry( D[k] ) - cum_carry( D[k - 1] );
         const double* z = &Z[( k - 1 ) * 2 * kParticles];
-        const double lnstep = lev.ln_step[k], off = (double)lev.offset[k];
+        const double lnstep = lstep[cur], off = (double)loff[cur];
         ParallelFor( kParticles / 256, [&]( int blk ) {
             for ( int p = blk * 256; p < ( blk + 1 ) * 256; p++ )
             {
@@ -567,8 +596,8 @@ int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& lev
                 double lp;
                 if ( !( f > 0.0 ) )
                     lp = layer[0];
-                else if ( f >= (double)( kGridPoints - 1 ) )
-                    lp = layer[kGridPoints - 1];
+                else if ( f >= (double)( P - 1 ) )
+                    lp = layer[P - 1];
                 else
                 {
                     const size_t i = (size_t)f;
@@ -583,6 +612,45 @@ int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& lev
                 ln_spot[p] += step;
             }
         } );
+    }
+    lev.points = P;
+    if ( !surf )
+    {
+        lev.t = std::move( lt );
+        lev.ln_step = std::move( lstep );
+        lev.offset = std::move( loff );
+        lev.values = std::move( layers );
+        return THOTH_OK;
+    }
+    // per-date layers of the surface: LeverageSurface::GetLeverage at the date's
+    // time (leverage_surface.cpp:43-66), exact on the shared axis since linear
+    // interpolation in time and in log-spot commute there
+    lev.surface.t = lt;
+    lev.surface.ln_step = lstep[0];
+    lev.surface.offset = loff[0];
+    lev.surface.values = layers;
+    lev.t.assign( nd, 0 );
+    lev.ln_step.assign( nd, lstep[0] );
+    lev.offset.assign( nd, loff[0] );
+    lev.values.assign( nd * P, 0 );
+    for ( size_t i = 0; i < nd; i++ )
+    {
+        const double T = YearFraction( 0.0, D[i] );
+        lev.t[i] = T;
+        double* out = &lev.values[i * P];
+        if ( T <= lt.front() || T >= lt.back() )
+        {
+            const double* src = &layers[( T <= lt.front() ? 0 : nt - 1 ) * P];
+            std::copy( src, src + P, out );
+            continue;
+        }
+        size_t hi = 1;
+        while ( lt[hi] < T )
+            hi++;
+        const size_t lo = hi - 1;
+        const double w = ( T - lt[lo] ) / ( lt[hi] - lt[lo] );
+        for ( int j = 0; j < P; j++ )
+            out[j] = ( 1.0 - w ) * layers[lo * P + j] + w * layers[hi * P + j];
     }
     return THOTH_OK;
 }

@@ -48,7 +48,7 @@ def lib():
         L.thoth_mcl_release.restype = None
         L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
-                                                   P(C.c_double), P(C.c_int64), P(C.c_double)]
+                                                   P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_double)]
         L.thoth_mcl_price_pathwise.argtypes = [P(abi.Request), P(abi.PathwiseResult)]
         L.thoth_mcl_price_rqmc.argtypes = [P(abi.Request), C.c_int32, C.c_uint64, P(abi.RqmcResult)]
         L.thoth_grid_prepare.argtypes = [C.c_int32, P(abi.Request), P(abi.VanillaGrid), C.c_int32, P(C.c_void_p)]
@@ -155,18 +155,23 @@ def grid(built, cap=100000):
 
 
 def calibrate_leverage(built, asset, spot=None):
+    """The FP64 calibrated leverage layers of an LSV equity (host only, no device)."""
     days, _, _ = grid(built)
-    n = days.size
-    ln_step, offset, levels = np.zeros(n), np.zeros(n, np.int64), np.zeros((n, 201))
+    r = built.req
+    surface = r.leverage_time_layers > 0
+    n = min(days.size, r.leverage_time_layers) if surface else days.size
+    points = r.leverage_spot_points if surface else 201
+    t, ln_step, offset, levels = np.zeros(n), np.zeros(n), np.zeros(n, np.int64), np.zeros((n, points))
     if spot is None:
         spot = built.spec.equities[asset].spot
-    k = lib().thoth_mcl_calibrate_leverage(C.byref(built.req), asset, spot, n,
+    k = lib().thoth_mcl_calibrate_leverage(C.byref(r), asset, spot, n,
+                                           t.ctypes.data_as(C.POINTER(C.c_double)),
                                            ln_step.ctypes.data_as(C.POINTER(C.c_double)),
                                            offset.ctypes.data_as(C.POINTER(C.c_int64)),
                                            levels.ctypes.data_as(C.POINTER(C.c_double)))
     if k < 0:
         _check(-k)
-    return dict(ln_step=ln_step, offset=offset, levels=levels)
+    return dict(t=t[:k], ln_step=ln_step[:k], offset=offset[:k], levels=levels[:k])
 
 
 class BookArrays:
