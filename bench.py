@@ -64,6 +64,96 @@ def cpu_baseline(name, sample_paths):
                       "%.1f s on %s" % (sample_paths, sec, platform.processor() or platform.machine())}
 
 
+def _oracle_block(args):
+    """One CPU worker of the all-cores baseline: the oracle's sweep of a
+    disjoint Sobol block (run_cluster.cpp:308-339 splits paths the same way)."""
+    name, paths, skip = args
+    from oracle import pyoracle
+    spec = workload(name, paths)
+    spec.precision = abi.FP64
+    spec.mcl.sobol_skip = skip
+    sec, _ = pyoracle.sweep_seconds(BuiltRequest(spec))
+    return sec
+
+
+def cpu_baseline_all_cores(name, paths_per_core):
+    """The same oracle on every host core this process may use (at most 16, the
+    host share of one GPU), one process per core on disjoint Sobol blocks;
+    throughput = all paths / the slowest worker's sweep time."""
+    import multiprocessing as mp
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(16, cores))
+    try:
+        with mp.get_context("spawn").Pool(cores) as pool:
+            secs = pool.map(_oracle_block, [(name, paths_per_core, k * paths_per_core) for k in range(cores)])
+    except Exception as e:
+        return {"value": None, "unit": "paths/s", "cores": cores, "kind": "port", "sample": "failed: %s" % e}
+    return {"value": cores * paths_per_core / max(secs), "unit": "paths/s", "cores": cores, "kind": "port",
+            "sample": "%d processes x %d paths on disjoint Sobol blocks, path sweep only, slowest %.1f s"
+                      % (cores, paths_per_core, max(secs))}
+
+
+def _counter_kib(csv_path, kernel):
+    """Mean per-dispatch value of the one counter in a rocprofv3 csv for kernels whose name contains `kernel`."""
+    import csv
+    vals = []
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            if kernel in row["Kernel_Name"]:
+                vals.append(float(row["Counter_Value"]))
+    return sum(vals) / len(vals) if vals else None
+
+
+def _profile_counter(counter, cmd, kernel, workdir):
+    """Runs cmd under `rocprofv3 --pmc counter` (a child process, its own time
+    limit) and returns the counter's KiB per dispatch of `kernel`."""
+    import glob
+    import signal
+    import subprocess
+    out = os.path.join(workdir, counter + "_" + kernel.strip("_"))
+    full = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "-f", "csv", "--"] + cmd
+    p = subprocess.Popen(full, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True,
+                         cwd=workdir)
+    try:
+        p.wait(timeout=150)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        return None
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    return _counter_kib(files[0], kernel) if files else None
+
+
+def hbm_traffic(workload_name, paths, kernel="sweep_kernel"):
+    """HBM bytes per launch of the sweep kernel: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) of a one-step run of this workload, each
+    scaled by bytes / counter of a device-to-device copy of known size."""
+    import shutil
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None, "rocprofv3 not found"
+    probe_mib = 256
+    me = [sys.executable, str(Path(__file__).resolve())]
+    run = me + ["--workload", workload_name, "--paths", str(paths), "--steps", "1", "--warmup", "0",
+                "--no-cpu", "--no-t1bp", "--no-grid", "--no-traffic"]
+    probe = me + ["--copy-probe", str(probe_mib)]
+    detail = {}
+    total = 0.0
+    with tempfile.TemporaryDirectory() as wd:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            kib = _profile_counter(counter, run, kernel, wd)
+            ref = _profile_counter(counter, probe, "copyBuffer", wd)
+            if kib is None or not ref:
+                return None, "%s pass failed" % counter
+            factor = probe_mib * 1024 / ref
+            detail[counter] = {"kib_per_dispatch": kib, "copy_probe_factor": factor}
+            total += kib * 1024 * factor
+    return total, detail
+
+
 def time_to_1bp(replicates, paths, seed=101, target=0.01):
     """Wall time to a 95 % half-width of 1bp of notional (0.01 on the rebased
     100 notional) on the 5-asset LSV worst-of basket, plain Monte-Carlo in the
@@ -145,6 +235,9 @@ def main():
     ap.add_argument("--t1bp-paths", type=int, default=1 << 14)
     ap.add_argument("--no-grid", action="store_true", help="skip the vanilla-grid batch")
     ap.add_argument("--grid-paths", type=int, default=1 << 18)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    ap.add_argument("--cpu-all-cores-paths", type=int, default=1 << 14,
+                    help="paths per core of the all-cores CPU baseline (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -205,7 +298,9 @@ def main():
         "setup_s": setup_s,
         "premium": res.premium,
         "premium_trust": res.premium_trust,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "kernel": "sweep_kernel<%s, %d, ...>" % ("float" if fp32 else "double",
+                                                                            len(spec.equities)),
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None},
         "device": mcl.device_info(),
     }
@@ -213,9 +308,18 @@ def main():
         line["vanilla_grids"] = vanilla_grids(args.grid_paths)
     if not args.no_t1bp and args.workload == "lsv_worst_of_basket":
         line["time_to_1bp"] = time_to_1bp(args.t1bp_replicates, args.t1bp_paths)
+    if not args.no_traffic:
+        traffic, detail = hbm_traffic(args.workload, args.paths)
+        line["roofline"]["traffic"] = traffic
+        line["roofline"]["traffic_detail"] = detail
+        if traffic:
+            line["roofline"]["hbm_gbs"] = traffic / (kernel_ms * 1e-3) / 1e9
+            line["roofline"]["hbm_frac"] = line["roofline"]["hbm_gbs"] / HBM_PEAK_GBS
     if not args.no_cpu:
         cb = cpu_baseline(args.workload, args.cpu_sample)
         line["cpu_baseline"] = cb
+        if args.cpu_all_cores_paths:
+            line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(args.workload, args.cpu_all_cores_paths)
     print(json.dumps(line))
 
 
