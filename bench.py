@@ -184,6 +184,7 @@ def time_to_1bp(replicates, paths, seed=101, target=0.01):
                                    train_ms=train_ms,
                                    training={"paths": cfg.train_paths, "adam_steps": cfg.adam_steps,
                                              "batch_paths": cfg.batch_paths, "hedge_dates": cfg.hedge_dates,
+                                             "hedge_spacing": cfg.hedge_spacing,
                                              "learning_rate": cfg.learning_rate,
                                              "loss_first": rep.loss_first, "loss_last": rep.loss_last})
     t64 = out["plain_fp64"]["projected_ms"]
@@ -231,7 +232,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-t1bp", action="store_true", help="skip the time-to-1bp comparison")
-    ap.add_argument("--t1bp-replicates", type=int, default=32)
+    ap.add_argument("--t1bp-replicates", type=int, default=512)
     ap.add_argument("--t1bp-paths", type=int, default=1 << 14)
     ap.add_argument("--no-grid", action="store_true", help="skip the vanilla-grid batch")
     ap.add_argument("--grid-paths", type=int, default=1 << 18)

@@ -7,11 +7,13 @@
  * engine itself generates (FP32 sweep of the book) and then used as the
  * control variate
  *     CV = sum_j f(x_j) . sum_{i in interval j} df(t_i) (100/S0) M_i,
- *     M_i = S_i - E[S_i | path up to t_{i-1}]   (per equity),
- * the conditional mean taken under the simulated scheme itself (Black-Scholes
- * step: lognormal mean; Andersen QE step: the moment generating function of
- * the QE variance draw).  CV has zero mean for ANY weights, so P - CV is an
- * unbiased estimator of the premium P.  Networks: 11 -> 64 -> 64
+ *     M_i = the first-order Doob increment of S_i (per equity) under the
+ *           simulated scheme: S_{i-1} vol dW for a Black-Scholes step,
+ *           S_{i-1} (A (v_i - E[v_i | past]) + b z) for the Andersen QE step
+ *           (A its variance loading, b z its spot shock, z independent of v_i),
+ * whose conditional mean given the path up to t_{i-1} is exactly zero.  CV
+ * has zero mean for ANY weights, so P - CV is an unbiased estimator of the
+ * premium P.  Networks: 11 -> 64 -> 64
  * -> 64 -> 5, SiLU, BF16 operands with FP32 accumulation on the matrix cores,
  * FP32 master weights, Adam.  Plain Monte-Carlo is reported beside every
  * control-variate estimate as the yardstick of bias and variance reduction.
@@ -36,7 +38,7 @@ typedef struct thoth_cv_weights {
 } thoth_cv_weights;
 
 typedef struct thoth_cv_config {
-    int32_t hedge_dates;       /* J: hedge ratios are re-evaluated at dates round(j m / J) */
+    int32_t hedge_dates;       /* J: hedge ratios are re-evaluated at J dates (hedge_spacing) */
     int64_t train_paths;       /* training paths (Sobol points train_sobol_skip+1 ..) */
     int64_t train_sobol_skip;
     int32_t adam_steps;
@@ -44,6 +46,9 @@ typedef struct thoth_cv_config {
     float learning_rate;
     uint64_t init_seed;        /* weight initialisation */
     int32_t eval_mode;         /* THOTH_CV_IN_LOOP or THOTH_CV_TWO_PHASE (pricing only) */
+    float hedge_spacing;       /* p in [1, 8]: hedge date j at step m - round(m (1 - j/J)^p), at least one
+                                  step apart; 1 = uniform, 2 = uniform in sqrt(T - t) (denser toward
+                                  maturity, where the discrete-hedging error concentrates) */
 } thoth_cv_config;
 
 /* pricing with a trained network: the hedge network evaluated on the matrix
@@ -53,8 +58,8 @@ typedef struct thoth_cv_config {
 enum { THOTH_CV_IN_LOOP = 0, THOTH_CV_TWO_PHASE = 1 };
 
 typedef struct thoth_cv_train_report {
-    double loss_first;   /* mean Y^2 of the first minibatch (f = 0 => mean P^2) */
-    double loss_last;    /* mean Y^2 of the last minibatch */
+    double loss_first;   /* mean (Y - mean P)^2 of the first minibatch (f = 0 => variance of P) */
+    double loss_last;    /* the same for the last minibatch */
     double data_ms;      /* training-path sweep */
     double train_ms;     /* all Adam steps (one hipGraph replayed per step) */
     int32_t steps;
@@ -83,7 +88,8 @@ int thoth_cv_price(const thoth_mcl_request* req, const thoth_cv_config* cfg, con
  * replicates = 0, `paths` paths from Sobol points sobol_skip+1 ..; with
  * replicates R > 0, points 0 .. paths-1 under R random digital shifts.
  * feat [R'][J][paths][16] BF16 bits, dm [R'][J][paths][8], pay [R'][paths],
- * R' = max(1, R) (the layouts thoth_cv_loss_grad consumes for R' = 1). */
+ * R' = max(1, R) (the layouts thoth_cv_loss_grad consumes for R' = 1); hedge
+ * dates uniform, round(j m / J). */
 int thoth_cv_samples(const thoth_mcl_request* req, int32_t hedge_dates, int64_t paths, int64_t sobol_skip,
                      int32_t replicates, uint64_t rqmc_seed, uint16_t* feat, float* dm, float* pay);
 

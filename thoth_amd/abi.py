@@ -116,7 +116,7 @@ class CvWeights(C.Structure):
 class CvConfig(C.Structure):
     _fields_ = [("hedge_dates", _i32), ("train_paths", _i64), ("train_sobol_skip", _i64),
                 ("adam_steps", _i32), ("batch_paths", _i32), ("learning_rate", C.c_float),
-                ("init_seed", C.c_uint64), ("eval_mode", _i32)]
+                ("init_seed", C.c_uint64), ("eval_mode", _i32), ("hedge_spacing", C.c_float)]
 
 
 CV_IN_LOOP, CV_TWO_PHASE = 0, 1

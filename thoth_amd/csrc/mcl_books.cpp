@@ -175,6 +175,8 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
                     c[SC_A2] = (float)( 0.5 * dt * 0.5 );
                     c[SC_RX] = (float)( A.rho / A.xi );
                     c[SC_K3] = (float)( 0.5 * dt * ( 1.0 - A.rho * A.rho ) );
+                    c[SC_CVA] = (float)( 0.5 * dt * A.kappa * A.rho / A.xi + A.rho / A.xi );
+                    c[SC_CVB] = (float)( 0.5 * dt * 0.5 - 0.5 * ( 0.5 * dt * ( 1.0 - A.rho * A.rho ) ) );
                 }
             }
         }
@@ -260,7 +262,7 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
     return THOTH_OK;
 }
 
-// hedge dates d_j = round(j m / J) and the per-date increment scale
+// hedge dates d_j = m - round(m (1 - j/J)^p), at least one step apart, and the per-date increment scale
 // df(t_i) 100 / S0_a of the learned control variate
 int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Offsets& of, Arena& ar )
 {
@@ -274,11 +276,20 @@ int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Off
     if ( cv.mode == 2 && ( !cv.wimg || !cv.bias || in.n_contracts >= kMaxContracts ) )
         return SetError( THOTH_ERR_INVALID, Why( bi, "in-loop inference needs weights and a free result slot" ) );
     std::vector<int> start( nd, -1 ), end( nd, -1 );
+    if ( !( cv.spacing >= 1.0f && cv.spacing <= 8.0f ) )
+        return SetError( THOTH_ERR_INVALID, Why( bi, "hedge spacing must be in [1, 8]" ) );
+    const double pw = cv.spacing;
+    std::vector<int> dd( cv.J + 1, 0 );
+    for ( int j = 1; j <= cv.J; j++ )
+    {
+        const int g = m - (int)std::lround( m * std::pow( 1.0 - (double)j / cv.J, pw ) );
+        dd[j] = std::min( std::max( dd[j - 1] + 1, g ), m - ( cv.J - j ) );
+    }
     int prev = 0;
     for ( int j = 0; j < cv.J; j++ )
     {
-        const int d0 = (int)std::lround( (double)j * m / cv.J );
-        const int d1 = (int)std::lround( (double)( j + 1 ) * m / cv.J );
+        const int d0 = dd[j];
+        const int d1 = dd[j + 1];
         if ( d1 <= d0 || d0 != prev )
             return SetError( THOTH_ERR_INVALID, Why( bi, "hedge dates must be increasing" ) );
         start[d0] = j;
@@ -431,14 +442,13 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     }
     // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV) a
     // ring of 2G leverage layers per equity of the kernel instantiation (G = 4,
-    // or 2 when the hedge network also needs its LDS area: two blocks per CU)
+    // halved while two blocks per CU, the occupancy of the 5-asset kernels,
+    // would not fit the 160 KiB of LDS)
     const int na_inst = B.na_max <= 1 ? 1 : B.na_max <= 2 ? 2 : B.na_max <= 5 ? 5 : 8;
-    const int lev_group = B.cv_mode == 2 ? 2 : 4;
     size_t dims = 0, levw = 0;
     bool any_grid = false;
     for ( int b = 0; b < n_books; b++ )
     {
-        B.db[b].lev_group = lev_group;
         dims = std::max( dims, (size_t)B.db[b].m * B.db[b].factors );
         if ( cfg->precision == 1 && B.db[b].lev_points > 0 )
             levw = std::max( levw, (size_t)2 * na_inst * B.db[b].lev_points );
@@ -448,6 +458,11 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     B.lds_lev_words = levw;
     // tail areas, 16-byte aligned after the leverage ring (sweep_block's layout)
     B.lds_tail = B.cv_mode == 2 ? (size_t)kCvLdsBytes : any_grid ? (size_t)grid_lds_bytes( cfg->precision == 0 ? 8 : 4 ) : 0;
+    int lev_group = 4;
+    while ( lev_group > 1 && 2 * LdsBytes( B, lev_group ) > (size_t)160 * 1024 )
+        lev_group /= 2;
+    for ( int b = 0; b < n_books; b++ )
+        B.db[b].lev_group = lev_group;
     B.lds_bytes = LdsBytes( B, lev_group );
     // grid partials [R][blocks][nm][2 nk][2] and their sums [nm][2 nk][2], per book
     B.grid_off.assign( n_books, 0 );

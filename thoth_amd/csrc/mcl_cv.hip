@@ -4,7 +4,7 @@
 // Data (written by the FP32 path sweep, sample layout [replicate][j][path]):
 //   feat [R][J][N][16] BF16 features x_{p,j} at hedge date d_j,
 //   dm   [R][J][N][8]  sum over the steps of interval j of df(t_i) 100/S0_a M_{i,a},
-//        M_i = S_i - E[S_i | past] under the simulated scheme (zero mean),
+//        M_i the first-order Doob increment of S_i (zero conditional mean),
 //   pay  [R][N]        discounted book value P_p.
 // Control variate CV_p = sum_j f(x_{p,j}) . dm_{p,j} has mean zero for any f,
 // so Y_p = P_p - CV_p is an unbiased estimator of the premium; training

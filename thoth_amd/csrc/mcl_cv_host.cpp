@@ -67,7 +67,7 @@ void InitWeights( uint64_t seed, float* w )
     std::fill( w, w + kN, 0.0f );
     for ( int L = 0; L < 3; L++ )
     {
-        const float a = 1.0f / std::sqrt( (float)kIn[L] );
+        const float a = std::sqrt( 6.0f / (float)kIn[L] );  // variance 2 / fan-in (He)
         std::uniform_real_distribution<float> u( -a, a );
         for ( int o = 0; o < kOut[L]; o++ )
             for ( int i = 0; i < kIn[L]; i++ )
@@ -77,15 +77,16 @@ void InitWeights( uint64_t seed, float* w )
 
 int CheckConfig( const thoth_cv_config& c )
 {
-    if ( c.hedge_dates < 1 || c.batch_paths < 64 || c.batch_paths % 64 || c.adam_steps < 0 || c.train_paths < c.batch_paths ||
+    if ( c.hedge_dates < 1 || !( c.hedge_spacing >= 1.0f && c.hedge_spacing <= 8.0f ) || c.batch_paths < 64 ||
+         c.batch_paths % 64 || c.adam_steps < 0 || c.train_paths < c.batch_paths ||
          c.train_sobol_skip < 0 || !( c.learning_rate > 0 ) )
         return SetError( THOTH_ERR_INVALID, "control-variate config out of range" );
     return THOTH_OK;
 }
 
 // FP32 CV-data sweep of the base book: `paths` paths per replicate
-int CvSweep( const thoth_mcl_request& req, int J, int64_t paths, int64_t skip, int R, uint64_t rqmc_seed, Dev& dev,
-             CvSpec& cv, std::vector<thoth_mcl_book_out>& out, double& ms )
+int CvSweep( const thoth_mcl_request& req, int J, float spacing, int64_t paths, int64_t skip, int R, uint64_t rqmc_seed,
+             Dev& dev, CvSpec& cv, std::vector<thoth_mcl_book_out>& out, double& ms )
 {
     host::BookArrays A;
     host::Grid g;
@@ -94,6 +95,7 @@ int CvSweep( const thoth_mcl_request& req, int J, int64_t paths, int64_t skip, i
         return rc;
     const int Rr = std::max( 1, R );
     cv.J = J;
+    cv.spacing = spacing;
     cv.N = paths;
     cv.feat = dev.alloc<uint16_t>( (size_t)Rr * J * paths * 16 );
     cv.dm = dev.alloc<float>( (size_t)Rr * J * paths * 8 );
@@ -142,9 +144,26 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_train( cons
         double data_ms = 0;
         const int J = cfg->hedge_dates;
         const int64_t N = cfg->train_paths;
-        rc = CvSweep( *req, J, N, cfg->train_sobol_skip, 0, 0, dev, cv, out, data_ms );
+        rc = CvSweep( *req, J, cfg->hedge_spacing, N, cfg->train_sobol_skip, 0, 0, dev, cv, out, data_ms );
         if ( rc != THOTH_OK )
             return rc;
+        // centre the payoffs on their training mean: E[CV] = 0 for any weights,
+        // so the minimiser of mean (P - c - CV)^2 does not depend on c, while
+        // the gradient loses the noise of c dCV/dw
+        {
+            hipStream_t s = Stream();
+            std::vector<float> pay( N );
+            HIP_OK( hipMemcpyAsync( pay.data(), cv.pay, N * sizeof( float ), hipMemcpyDeviceToHost, s ) );
+            HIP_OK( hipStreamSynchronize( s ) );
+            double sum = 0;
+            for ( float x : pay )
+                sum += x;
+            const float mean = (float)( sum / (double)N );
+            for ( float& x : pay )
+                x -= mean;
+            HIP_OK( hipMemcpyAsync( cv.pay, pay.data(), N * sizeof( float ), hipMemcpyHostToDevice, s ) );
+            HIP_OK( hipStreamSynchronize( s ) );
+        }
         const int B = cfg->batch_paths;
         const CvTrainLayout lay = cv_train_layout( B, J );
         float* dw = dev.alloc<float>( kN );
@@ -290,6 +309,7 @@ int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const
     CvSpec cv;
     cv.mode = 2;
     cv.J = cfg.hedge_dates;
+    cv.spacing = cfg.hedge_spacing;
     cv.N = req.mcl.paths;
     cv.wimg = fwd;
     cv.bias = dw + kWSize;
@@ -329,7 +349,7 @@ int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, con
     double sweep_ms = 0;
     const int J = cfg.hedge_dates;
     const int64_t N = req.mcl.paths;
-    int rc = CvSweep( req, J, N, 0, R, rqmc_seed, dev, cv, out, sweep_ms );
+    int rc = CvSweep( req, J, cfg.hedge_spacing, N, 0, R, rqmc_seed, dev, cv, out, sweep_ms );
     if ( rc != THOTH_OK )
         return rc;
     float* dw = dev.alloc<float>( kN );
@@ -493,7 +513,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_samples( co
         CvSpec cv;
         std::vector<thoth_mcl_book_out> out;
         double ms = 0;
-        rc = CvSweep( *req, hedge_dates, paths, sobol_skip, replicates, rqmc_seed, dev, cv, out, ms );
+        rc = CvSweep( *req, hedge_dates, 1.0f, paths, sobol_skip, replicates, rqmc_seed, dev, cv, out, ms );
         if ( rc != THOTH_OK )
             return rc;
         hipStream_t s = Stream();

@@ -52,6 +52,8 @@ enum StepConst
     SC_SQDT,       // sqrt(dt)
     SC_HALF_V2DT,  // BS: v^2 / 2 dt
     SC_VOL,        // BS: v
+    SC_CVA,        // SC_A1 + SC_RX                : variance loading of the hedge increment
+    SC_CVB,        // SC_A2 - SC_K3 / 2            :   A = L SC_CVA - L^2 SC_CVB
     SC_COUNT = 16
 };
 

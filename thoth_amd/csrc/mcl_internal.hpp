@@ -76,6 +76,7 @@ struct CvSpec
 {
     int mode = 1;
     int J = 0;
+    float spacing = 1.0f;  // hedge date j at step m - round(m (1 - j/J)^spacing)
     int64_t N = 0;
     uint16_t* feat = nullptr;
     float* dm = nullptr;

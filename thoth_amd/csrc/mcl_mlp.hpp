@@ -62,14 +62,20 @@ __device__ __forceinline__ bf16x4 bits_bf16x4( uint2 w )
     return r;
 }
 
+typedef float f32x2_t __attribute__( ( ext_vector_type( 2 ) ) );
+typedef __bf16 bf16x2_t __attribute__( ( ext_vector_type( 2 ) ) );
+typedef uint32_t u32x2_t __attribute__( ( ext_vector_type( 2 ) ) );
+
+// two FP32 -> packed BF16 (a in the low half), round to nearest even: one
+// v_cvt_pk_bf16_f32 on gfx950, the same result as to_bf16 for finite values
+__device__ __forceinline__ uint32_t pack2_bf16( float a, float b )
+{
+    return __builtin_bit_cast( uint32_t, __builtin_convertvector( f32x2_t{ a, b }, bf16x2_t ) );
+}
+
 __device__ __forceinline__ bf16x4 pack_bf16( f32x4 v )
 {
-    bf16x4 r;
-    r[0] = to_bf16( v[0] );
-    r[1] = to_bf16( v[1] );
-    r[2] = to_bf16( v[2] );
-    r[3] = to_bf16( v[3] );
-    return r;
+    return __builtin_bit_cast( bf16x4, u32x2_t{ pack2_bf16( v[0], v[1] ), pack2_bf16( v[2], v[3] ) } );
 }
 
 __device__ __forceinline__ float silu( float z ) { return z * __builtin_amdgcn_rcpf( 1.0f + __expf( -z ) ); }

@@ -233,13 +233,11 @@ __device__ __forceinline__ double qe_f64( double v, double dt, double z, double 
     return ( u <= p ) ? 0.0 : log( ( 1.0 - p ) / ( 1.0 - u ) ) / beta;
 }
 
-// Parameters of the QE draw actually taken: quadratic branch v' = a (sqrt(b2)
-// + Z)^2, or exponential branch (mass p = 1 - omp at 0, else Exp(omp / m)).
+// The QE draw actually taken: quadratic branch v' = a (sqrt(b2) + Z)^2, or
+// exponential branch (mass p at 0, else exponential with rate beta).
 struct QeDraw
 {
-    bool quad;
-    float a, b2;   // quadratic
-    float omp, m;  // exponential
+    float mean;  // E[v' | v] of the FP32 draw as computed below
 };
 
 // Same scheme in FP32 from host-precomputed step constants.  1-u is formed
@@ -254,24 +252,17 @@ __device__ __forceinline__ float qe_f32( float v, float z, const float* __restri
         const float c = fast_div( 2.0f, psi );
         const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
         const float a = fast_div( m, 1.0f + b2 );
-        const float bz = __builtin_amdgcn_sqrtf( b2 ) + z;
+        const float sb = __builtin_amdgcn_sqrtf( b2 );
+        const float bz = sb + z;
         if ( draw )
-        {
-            draw->quad = true;
-            draw->a = a;
-            draw->b2 = b2;
-        }
+            draw->mean = a * fmaf( sb, sb, 1.0f );  // E[a (sb + Z)^2]
         return a * bz * bz;
     }
     // p = (psi-1)/(psi+1), beta = (1-p)/m; v = log((1-p)/(1-u)) / beta
     const float inv = fast_rcp( psi + 1.0f );
     const float one_minus_p = 2.0f * inv;
     if ( draw )
-    {
-        draw->quad = false;
-        draw->omp = one_minus_p;
-        draw->m = m;
-    }
+        draw->mean = one_minus_p * m * 0.5f * ( psi + 1.0f );  // (1 - p) / beta
     const float one_minus_u = 0.5f * erfc_fast( z * 0.70710678118654752f );
     return ( one_minus_u >= one_minus_p ) ? 0.0f
                                           : fast_log( fast_div( one_minus_p, one_minus_u ) ) * m * 0.5f * ( psi + 1.0f );
@@ -284,64 +275,6 @@ __device__ __forceinline__ R wave_sum( R x )
     for ( int off = 32; off >= 1; off >>= 1 )
         x += __shfl_xor( x, off, 64 );
     return x;
-}
-
-// expm1 and log1p for the small arguments of the hedge increments: Taylor
-// polynomials where they are exact to FP32 rounding (truncation below 3e-11
-// relative), the library functions elsewhere.
-__device__ __forceinline__ float expm1_small( float x )
-{
-    if ( fabsf( x ) < 0.125f )
-    {
-        float p = 1.0f / 5040.0f;
-        p = fmaf( p, x, 1.0f / 720.0f );
-        p = fmaf( p, x, 1.0f / 120.0f );
-        p = fmaf( p, x, 1.0f / 24.0f );
-        p = fmaf( p, x, 1.0f / 6.0f );
-        p = fmaf( p, x, 0.5f );
-        p = fmaf( p, x, 1.0f );
-        return p * x;
-    }
-    return expm1f( x );
-}
-
-__device__ __forceinline__ float log1p_small( float x )
-{
-    if ( fabsf( x ) < 0.0625f )
-    {
-        float p = -1.0f / 8.0f;
-        p = fmaf( p, x, 1.0f / 7.0f );
-        p = fmaf( p, x, -1.0f / 6.0f );
-        p = fmaf( p, x, 1.0f / 5.0f );
-        p = fmaf( p, x, -1.0f / 4.0f );
-        p = fmaf( p, x, 1.0f / 3.0f );
-        p = fmaf( p, x, -0.5f );
-        p = fmaf( p, x, 1.0f );
-        return p * x;
-    }
-    return log1pf( x );
-}
-
-// E[exp(A v') | v] - 1 for the QE draw v' (Andersen 2008, the moment
-// generating functions behind his martingale correction), from the draw's own
-// FP32 parameters: quadratic branch exp(A b2 a / (1 - 2 A a)) / sqrt(1 - 2 A a);
-// exponential branch p + (1 - p) beta / (beta - A), beta = (1 - p) / m.
-// Returns false where the moment does not exist.
-__device__ __forceinline__ bool qe_mgf_m1( const QeDraw& d, float A, float& out )
-{
-    if ( d.quad )
-    {
-        const float q = 1.0f - 2.0f * A * d.a;
-        if ( !( q > 0.0f ) )
-            return false;
-        out = expm1_small( fast_div( A * d.b2 * d.a, q ) - 0.5f * log1p_small( -2.0f * A * d.a ) );
-        return true;
-    }
-    const float Am = A * d.m;
-    if ( !( Am < d.omp ) )
-        return false;
-    out = fast_div( d.omp * Am, d.omp - Am );
-    return true;
 }
 
 // Brownian bridge of the reference schedule (path_generator.cpp:49-198)
@@ -610,12 +543,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         f[10] = fmaf( 2.0f * (float)B.t[i], B.cv_inv_T, -1.0f );
 #pragma unroll
         for ( int k = 0; k < 8; k++ )
-        {
-            uint32_t lo = __float_as_uint( f[2 * k] ), hi = __float_as_uint( f[2 * k + 1] );
-            lo = ( lo + 0x7fffu + ( ( lo >> 16 ) & 1u ) ) >> 16;
-            hi = ( hi + 0x7fffu + ( ( hi >> 16 ) & 1u ) ) >> 16;
-            w[k] = lo | ( hi << 16 );
-        }
+            w[k] = pack2_bf16( f[2 * k], f[2 * k + 1] );
     };
     auto cv_features = [&]( int i, int j ) __attribute__( ( always_inline ) )
     {
@@ -985,9 +913,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     S[a] = sold * __expf( expo );
                     if constexpr ( cv_on )
                     {
-                        // M = S e^{det} (e^{b z} - e^{b^2/2}): zero conditional mean
-                        const float M = sold * __expf( cd - sc[SC_HALF_V2DT] ) *
-                                        ( expm1_small( sc[SC_VOL] * dw ) - expm1_small( sc[SC_HALF_V2DT] ) );
+                        // first-order Doob increment S_{i-1} vol dW: zero conditional mean
+                        const float M = sold * sc[SC_VOL] * dw;
                         if constexpr ( cv_infer )
                             cvacc = fmaf( hr[a] * B.cv_scale[(size_t)a * nd + i], M, cvacc );
                         else
@@ -1091,30 +1018,19 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 S[a] = sold * __expf( step );
                 if constexpr ( cv_on )
                 {
-                    // spot increment minus its conditional mean given the past:
-                    // log S_i - log S_{i-1} = c0 + X with c0 known at t_{i-1} and
-                    // X = A v_i + b z - b^2/2, E[e^X | past] = E[e^{A v_i} | v_{i-1}],
-                    // so M = S_{i-1} e^{c0} (expm1(X) - (E[e^{A v_i}] - 1)).
-                    // Where that moment does not exist (or xi = 0): the increment
-                    // given v_i as well, S_{i-1} e^{det} (e^{b z} - e^{b^2/2}).
-                    float M;
-                    float mgf = 0.0f;
+                    // first-order Doob increment of the spot.  Given v_i, the step
+                    // is lognormal with E[S_i | v_i, past] = S_{i-1} e^{c + A v_i},
+                    // A = K2 + L^2 K3 / 2 = L SC_CVA - L^2 SC_CVB; its linear part
+                    //   M = S_{i-1} (A (v_i - E[v_i | past]) + b z)
+                    // has zero conditional mean exactly: the draw's own mean is
+                    // subtracted and z is independent of v_i and of the past.
+                    float M = bvol * z;
                     if ( B.asset[a].xi > 1e-12 )
                     {
-                        const float half_l2k3 = 0.5f * L * L * sc[SC_K3];
-                        const float Acoef = L * sc[SC_A1] - L * L * sc[SC_A2] + L * sc[SC_RX] + half_l2k3;
-                        if ( qe_mgf_m1( draw, Acoef, mgf ) )
-                        {
-                            const float c0 = cd + L * sc[SC_K0] + ( L * sc[SC_A1] - L * L * sc[SC_A2] - L * sc[SC_RX] +
-                                                                    half_l2k3 ) * vp;
-                            const float X = Acoef * vc + bvol * z - 0.5f * bvol * bvol;
-                            M = sold * __expf( c0 ) * ( expm1_small( X ) - mgf );
-                        }
-                        else
-                            M = sold * __expf( det ) * ( expm1_small( bvol * z ) - expm1_small( 0.5f * bvol * bvol ) );
+                        const float A = L * fmaf( -L, sc[SC_CVB], sc[SC_CVA] );
+                        M = fmaf( A, vc - draw.mean, M );
                     }
-                    else
-                        M = sold * __expf( det ) * ( expm1_small( bvol * z ) - expm1_small( 0.5f * bvol * bvol ) );
+                    M *= sold;
                     if constexpr ( cv_infer )
                         cvacc = fmaf( hr[a] * B.cv_scale[(size_t)a * nd + i], M, cvacc );
                     else
