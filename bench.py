@@ -154,13 +154,20 @@ def hbm_traffic(workload_name, paths, kernel="sweep_kernel"):
     return total, detail
 
 
-def time_to_1bp(replicates, paths, seed=101, target=0.01):
+def time_to_1bp(replicates, paths, seed=101, target=0.01, other_regimes=((512, 1 << 14),)):
     """Wall time to a 95 % half-width of 1bp of notional (0.01 on the rebased
     100 notional) on the 5-asset LSV worst-of basket, plain Monte-Carlo in the
     FP64 oracle-matched mode against the FP32 learned control variate.  Each
-    estimator is run once at R randomised-QMC replicates of `paths` Sobol
-    points (the same digital shifts for all three) and its device time is
-    projected to the target by the 1/sqrt(N) law of the half-width."""
+    estimator is run at R randomised-QMC replicates of `paths` Sobol points
+    (the same digital shifts for all) and its device time is projected to the
+    target by the 1/sqrt(N) law of the half-width.
+
+    The regime matters for plain MC, whose RQMC gain grows with the points per
+    replicate: at the target it needs about 26 replicates of 4096 points (a CI
+    needs at least 16 replicates, so larger replicates overshoot the target),
+    and 4096 points per replicate is the default.  The learned control variate
+    has no such gain (its residual is the high-frequency hedging error).  The
+    same comparison at other (replicates, points) is reported beside it."""
     out = {"target_halfwidth": target, "replicates": replicates, "paths_per_replicate": paths,
            "projection": "device ms x (halfwidth / target)^2"}
 
@@ -169,28 +176,45 @@ def time_to_1bp(replicates, paths, seed=101, target=0.01):
         e.update(kw)
         return e
 
-    b64 = BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=abi.FP64))
-    r64 = mcl.price_rqmc(b64, replicates, seed)
-    out["plain_fp64"] = entry(r64.mean, r64.halfwidth, r64.sweep_ms, per_path_var=r64.var)
-    b32 = BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=abi.FP32))
-    r32 = mcl.price_rqmc(b32, replicates, seed)
-    out["plain_fp32"] = entry(r32.mean, r32.halfwidth, r32.sweep_ms, per_path_var=r32.var)
     cfg = mcl.cv_config()
-    w, rep = mcl.cv_train(b32, cfg)
-    rcv = mcl.cv_price(b32, cfg, w, replicates, seed)
+    w, rep = mcl.cv_train(BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=abi.FP32)), cfg)
     train_ms = rep.data_ms + rep.train_ms
-    out["learned_cv_fp32"] = entry(rcv.cv_mean, rcv.cv_halfwidth, rcv.sweep_ms + rcv.eval_ms,
-                                   per_path_var=rcv.cv_var, variance_ratio=rcv.plain_var / rcv.cv_var,
-                                   train_ms=train_ms,
-                                   training={"paths": cfg.train_paths, "adam_steps": cfg.adam_steps,
-                                             "batch_paths": cfg.batch_paths, "hedge_dates": cfg.hedge_dates,
-                                             "hedge_spacing": cfg.hedge_spacing,
-                                             "learning_rate": cfg.learning_rate,
-                                             "loss_first": rep.loss_first, "loss_last": rep.loss_last})
-    t64 = out["plain_fp64"]["projected_ms"]
-    tcv = out["learned_cv_fp32"]["projected_ms"]
-    out["speedup_vs_plain_fp64"] = t64 / tcv
-    out["speedup_vs_plain_fp64_incl_training"] = t64 / (tcv + train_ms)
+
+    def regime(R, N):
+        res = {}
+        b64 = BuiltRequest(configs.lsv_worst_of_basket(paths=N, precision=abi.FP64))
+        r64 = mcl.price_rqmc(b64, R, seed)
+        res["plain_fp64"] = entry(r64.mean, r64.halfwidth, r64.sweep_ms, per_path_var=r64.var)
+        b32 = BuiltRequest(configs.lsv_worst_of_basket(paths=N, precision=abi.FP32))
+        r32 = mcl.price_rqmc(b32, R, seed)
+        res["plain_fp32"] = entry(r32.mean, r32.halfwidth, r32.sweep_ms, per_path_var=r32.var)
+        rcv = mcl.cv_price(b32, cfg, w, R, seed)
+        res["learned_cv_fp32"] = entry(rcv.cv_mean, rcv.cv_halfwidth, rcv.sweep_ms + rcv.eval_ms,
+                                       per_path_var=rcv.cv_var, variance_ratio=rcv.plain_var / rcv.cv_var)
+        res["speedup_vs_plain_fp64"] = res["plain_fp64"]["projected_ms"] / res["learned_cv_fp32"]["projected_ms"]
+        return res
+
+    out.update(regime(replicates, paths))
+    out["learned_cv_fp32"].update(
+        train_ms=train_ms,
+        training={"paths": cfg.train_paths, "adam_steps": cfg.adam_steps, "batch_paths": cfg.batch_paths,
+                  "hedge_dates": cfg.hedge_dates, "hedge_spacing": cfg.hedge_spacing,
+                  "learning_rate": cfg.learning_rate, "loss_first": rep.loss_first, "loss_last": rep.loss_last})
+    out["speedup_vs_plain_fp64_incl_training"] = out["plain_fp64"]["projected_ms"] / (
+        out["learned_cv_fp32"]["projected_ms"] + train_ms)
+    # one trained network serves every re-pricing of the book: pricings at the
+    # target after which the training has paid for itself
+    out["training_break_even_pricings"] = train_ms / (
+        out["plain_fp64"]["projected_ms"] - out["learned_cv_fp32"]["projected_ms"])
+    out["other_regimes"] = []
+    for R, N in other_regimes:
+        r = regime(R, N)
+        out["other_regimes"].append({"replicates": R, "paths_per_replicate": N,
+                                     "speedup_vs_plain_fp64": r["speedup_vs_plain_fp64"],
+                                     "plain_fp64_halfwidth": r["plain_fp64"]["halfwidth"],
+                                     "learned_cv_halfwidth": r["learned_cv_fp32"]["halfwidth"],
+                                     "plain_fp64_ms": r["plain_fp64"]["ms"],
+                                     "learned_cv_ms": r["learned_cv_fp32"]["ms"]})
     return out
 
 
@@ -232,8 +256,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-t1bp", action="store_true", help="skip the time-to-1bp comparison")
-    ap.add_argument("--t1bp-replicates", type=int, default=512)
-    ap.add_argument("--t1bp-paths", type=int, default=1 << 14)
+    ap.add_argument("--t1bp-replicates", type=int, default=2048)
+    ap.add_argument("--t1bp-paths", type=int, default=1 << 12)
     ap.add_argument("--no-grid", action="store_true", help="skip the vanilla-grid batch")
     ap.add_argument("--grid-paths", type=int, default=1 << 18)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
