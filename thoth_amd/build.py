@@ -34,6 +34,7 @@ HOST_FLAGS = ["-O2", "-std=c++2b", "-fPIC", "-Wall", "-Wextra", "-fvisibility=hi
 UNITS = [
     ("mcl_sweep_f64.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_sweep_f32.hip", "hipcc", []),
+    ("mcl_sweep_f32_uniform.hip", "hipcc", []),
     ("mcl_util.hip", "hipcc", []),
     ("mcl_cv.hip", "hipcc", []),
     ("mcl_pathwise.hip", "hipcc", ["-ffp-contract=off"]),

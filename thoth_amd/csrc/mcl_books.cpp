@@ -440,6 +440,27 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
         B.na_max = std::max( B.na_max, B.db[b].n_assets );
     }
+    // FP32 books whose equities are all LSV (or all Heston) with vol-of-vol
+    // > 1e-12, uncorrelated stochastic spots and the same equity count run
+    // the uniform-book kernel (no per-equity model branches)
+    B.kind = 0;
+    if ( cfg->precision == 1 )
+    {
+        int kind = -1;
+        for ( int b = 0; b < n_books && kind != 0; b++ )
+        {
+            const DevBook& d = B.db[b];
+            for ( int a = 0; a < d.n_assets; a++ )
+            {
+                const DevAsset& A = d.asset[a];
+                int k = A.model == THOTH_MODEL_LSV ? 1 : A.model == THOTH_MODEL_HESTON ? 2 : 0;
+                if ( !( A.xi > 1e-12 ) || d.corr_stoch || d.n_assets != B.na_max )
+                    k = 0;
+                kind = kind < 0 || kind == k ? k : 0;
+            }
+        }
+        B.kind = std::max( kind, 0 );
+    }
     // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV) a
     // ring of 2G leverage layers per equity of the kernel instantiation (G = 4,
     // halved while two blocks per CU, the occupancy of the 5-asset kernels,
@@ -604,8 +625,11 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
     bool ok = B.precision == 0
                   ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
                                       B.first, B.last, B.dshifts )
-                  : launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                      B.first, B.last, B.dshifts, B.cv_mode );
+                  : ( B.kind && launch_sweep_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                          DeviceDirections(), B.q0, B.first, B.last, B.dshifts,
+                                                          B.cv_mode ) ) ||
+                        launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
+                                          B.first, B.last, B.dshifts, B.cv_mode );
     if ( !ok )
     {
         (void)hipEventDestroy( e0 );

@@ -71,8 +71,13 @@ int Enqueue( const GridPlan& P, hipStream_t s )
     const bool ok = B.precision == 0
                         ? launch_persistent_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(),
                                                  B.q0, B.first, B.last, P.ditems, P.n_items, B.nblocks, P.dcounter )
-                        : launch_persistent_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(),
-                                                 B.q0, B.first, B.last, P.ditems, P.n_items, B.nblocks, P.dcounter );
+                        : ( B.kind && launch_persistent_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes,
+                                                                     B.dbooks, DeviceDirections(), B.q0, B.first,
+                                                                     B.last, P.ditems, P.n_items, B.nblocks,
+                                                                     P.dcounter ) ) ||
+                              launch_persistent_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                     DeviceDirections(), B.q0, B.first, B.last, P.ditems, P.n_items,
+                                                     B.nblocks, P.dcounter );
     if ( !ok )
         return SetError( THOTH_ERR_UNSUPPORTED, "no kernel instantiation for this book shape" );
     HIP_OK( hipGetLastError() );

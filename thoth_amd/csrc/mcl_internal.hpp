@@ -48,6 +48,7 @@ struct DeviceBatch
     uint32_t q0 = 0, nblocks = 0, first = 0, last = 0;  // q0: first 256-index block
     size_t lds_bytes = 0;
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
+    int kind = 0;  // uniform-book kernel: 1 all LSV, 2 all Heston (see sweep_block), 0 generic
     int cv_mode = 0;  // 0 none, 1 control-variate samples, 2 in-loop hedge inference
     bool pathwise = false;  // pathwise-Greek sweep: three estimators per block
     // vanilla grids: per-(replicate, block) partials and their per-book sums

@@ -44,6 +44,15 @@ bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, 
 bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts,
                        int cv_mode = 0 );
+// uniform books (every equity LSV: kind 1, or Heston: kind 2, vol-of-vol > 0,
+// uncorrelated stochastic spots, exactly na equities): false where no
+// specialised instantiation exists (the caller then runs the generic kernel)
+bool launch_sweep_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
+                               const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
+                               const uint32_t* shifts, int cv_mode );
+bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
+                                    const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first,
+                                    uint32_t last, const int2* items, int n_items, uint32_t ntiles, int* counter );
 void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bwd );
 void launch_cv_eval( hipStream_t s, const uint16_t* fwd, const float* bias, const uint16_t* feat, const float* dm,
                      const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks );

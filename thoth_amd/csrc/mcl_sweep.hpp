@@ -385,13 +385,19 @@ static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 
 // and accumulates P - CV in an extra result slot after the contracts.
 // One block of 256 consecutive Sobol indices (Sobol block bq) of book B,
 // replicate z (RQMC), partial-sum slot (z, tile) of ntiles per replicate.
-template <class R, int NA, int DEPTH, int CV>
+// KIND: 0 any book; 1 (2) every equity of every book is LSV (Heston) with
+// vol-of-vol > 1e-12 and uncorrelated stochastic spots, and the book has
+// exactly NA equities -- the host checks this -- so the per-equity model
+// branches fold away and the equities' steps interleave in one basic block.
+template <class R, int NA, int DEPTH, int CV, int KIND = 0>
 __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* __restrict__ V, uint32_t bq,
                                              uint32_t first_idx, uint32_t last_idx,
                                              const uint32_t* __restrict__ shifts, int z, uint32_t tile,
                                              uint32_t ntiles )
 {
     constexpr bool F64 = std::is_same_v<R, double>;
+    constexpr bool uniform = KIND != 0;
+    static_assert( KIND == 0 || !F64, "uniform-book kernels are FP32" );
     constexpr int NF = 2 * NA;
     constexpr int NFV = NF <= 2 ? 2 : NF <= 4 ? 4 : NF <= 8 ? 8 : 16;
     constexpr int NAV = NA <= 1 ? 1 : NA <= 2 ? 2 : NA <= 4 ? 4 : 8;
@@ -536,7 +542,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             if ( a < na && a < 5 )
             {
                 f[a] = ( fast_log( (float)S[a] ) - B.cv_lnS0[a] ) * B.cv_ln_scale[a];
-                if ( B.asset[a].model != 0 )
+                if ( uniform || B.asset[a].model != 0 )
                     f[5 + a] = 2.0f * fmaf( __builtin_amdgcn_sqrtf( fmaxf( (float)Var[a], 0.0f ) ),
                                             B.cv_vol_scale[a], -1.0f );
             }
@@ -865,12 +871,13 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 #pragma unroll
         for ( int a = 0; a < NA; a++ )
         {
-            if ( a >= na )
-                continue;
-            const int model = B.asset[a].model;
+            if constexpr ( !uniform )
+                if ( a >= na )
+                    continue;
+            const int model = KIND == 1 ? 2 : KIND == 2 ? 1 : B.asset[a].model;
             // correlated noise (correlated_noise_node.cpp:15-27, lower triangle, name order)
             R eps = nz[a];
-            if ( na > 1 && ( model == 0 || B.corr_stoch ) )
+            if ( !uniform && na > 1 && ( model == 0 || B.corr_stoch ) )
             {
                 eps = R( 0 );
 #pragma unroll
@@ -999,7 +1006,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             else
             {
                 float det, bvol;
-                if ( B.asset[a].xi > 1e-12 )
+                if ( uniform || B.asset[a].xi > 1e-12 )
                 {
                     const float aa = L * sc[SC_A1] - L * L * sc[SC_A2];
                     const float K1 = aa - L * sc[SC_RX];
@@ -1025,7 +1032,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     // has zero conditional mean exactly: the draw's own mean is
                     // subtracted and z is independent of v_i and of the past.
                     float M = bvol * z;
-                    if ( B.asset[a].xi > 1e-12 )
+                    if ( uniform || B.asset[a].xi > 1e-12 )
                     {
                         const float A = L * fmaf( -L, sc[SC_CVB], sc[SC_CVA] );
                         M = fmaf( A, vc - draw.mean, M );
@@ -1087,12 +1094,12 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 }
 
 // one Sobol block per workgroup: grid (blocks, books, replicates)
-template <class R, int NA, int DEPTH, int CV = 0>
+template <class R, int NA, int DEPTH, int CV = 0, int KIND = 0>
 __global__ void __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( CV == 2 ? 2 : 1 ) ) )
 sweep_kernel( const DevBook* __restrict__ books, const uint32_t* __restrict__ V, uint32_t bq0, uint32_t first_idx,
               uint32_t last_idx, const uint32_t* __restrict__ shifts )
 {
-    sweep_block<R, NA, DEPTH, CV>( books[blockIdx.y], V, bq0 + blockIdx.x, first_idx, last_idx, shifts, blockIdx.z,
+    sweep_block<R, NA, DEPTH, CV, KIND>( books[blockIdx.y], V, bq0 + blockIdx.x, first_idx, last_idx, shifts, blockIdx.z,
                                    blockIdx.x, gridDim.x );
 }
 
@@ -1101,7 +1108,7 @@ sweep_kernel( const DevBook* __restrict__ books, const uint32_t* __restrict__ V,
 // counter is zeroed by a kernel of the same graph).  Every item writes its
 // own partial-sum slot, so the result does not depend on which workgroup
 // took which item.
-template <class R, int NA, int DEPTH>
+template <class R, int NA, int DEPTH, int KIND = 0>
 __global__ void __launch_bounds__( kBlock )
 sweep_persistent( const DevBook* __restrict__ books, const uint32_t* __restrict__ V, uint32_t bq0, uint32_t first_idx,
                   uint32_t last_idx, const int2* __restrict__ items, int n_items, uint32_t ntiles,
@@ -1118,7 +1125,7 @@ sweep_persistent( const DevBook* __restrict__ books, const uint32_t* __restrict_
         if ( it >= n_items )
             break;
         const int2 w = items[it];
-        sweep_block<R, NA, DEPTH, 0>( books[w.x], V, bq0 + (uint32_t)w.y, first_idx, last_idx, nullptr, 0,
+        sweep_block<R, NA, DEPTH, 0, KIND>( books[w.x], V, bq0 + (uint32_t)w.y, first_idx, last_idx, nullptr, 0,
                                       (uint32_t)w.y, ntiles );
     }
 }
