@@ -3,14 +3,19 @@
 Form (the erfinv parameterisation of M. Giles, "Approximating the erfinv
 function", GPU Computing Gems, 2011; coefficients fitted here):
     x = 2u - 1,  w = -log(1 - x^2) = -log(4 u (1 - u))
-    Phi^-1(u) = x * P_c(w - 2.5)            for w < 5
+    Phi^-1(u) = x * P_c(w - 4)              for w < 8
               = x * P_t(sqrt(w) - 3)        otherwise
-P_c, P_t: degree-8 polynomials fitted by weighted least squares on Chebyshev
-nodes to the relative error against scipy's ndtri, sqrt(2) folded in.
+P_c (degree 10) and P_t (degree 8) fitted by weighted least squares on
+Chebyshev nodes to the relative error against scipy's ndtri, sqrt(2) folded
+in.  The central piece reaches w = 8, so the tail polynomial is needed for
+1.7e-4 of the draws: in about 10 % of the steps of a 64-lane wave drawing
+10 normals, against 89 % with a cutoff at w = 5.
 Run to regenerate the constants printed for mcl_sweep.hpp.
 """
 import numpy as np
 from scipy.special import ndtri
+
+W_SPLIT, W_CENTER = 8.0, 4.0
 
 
 def target(w):
@@ -35,9 +40,9 @@ def fit(lo, hi, center, transform, deg=8, n=4000):
 
 
 def main():
-    cc = fit(0.0, 5.0, 2.5, lambda s: s)
-    ct = fit(np.sqrt(5.0) - 3.0, np.sqrt(23.0) - 3.0, 0.0, lambda s: (s + 3.0) ** 2)
-    print("central (w - 2.5), degree 0..8:", ", ".join("%.9e" % c for c in cc))
+    cc = fit(0.0, W_SPLIT, W_CENTER, lambda s: s, deg=10)
+    ct = fit(np.sqrt(W_SPLIT) - 3.0, np.sqrt(23.0) - 3.0, 0.0, lambda s: (s + 3.0) ** 2)
+    print("central (w - %g), degree 0..10:" % W_CENTER, ", ".join("%.9e" % c for c in cc))
     print("tail (sqrt(w) - 3), degree 0..8:", ", ".join("%.9e" % c for c in ct))
     # FP32 evaluation from 32-bit integer coordinates vs FP64 truth
     rng = np.random.default_rng(0)
@@ -62,9 +67,9 @@ def evaluate_f32(x, cc, ct):
         for k in range(len(c) - 2, -1, -1):
             r = (r * v + c[k]).astype(f32)
         return r
-    pc = horner(cc, (w - f32(2.5)).astype(f32))
+    pc = horner(cc, (w - f32(W_CENTER)).astype(f32))
     pt = horner(ct, (np.sqrt(w) - f32(3.0)).astype(f32))
-    return xp * np.where(w < f32(5.0), pc, pt)
+    return xp * np.where(w < f32(W_SPLIT), pc, pt)
 
 
 if __name__ == "__main__":

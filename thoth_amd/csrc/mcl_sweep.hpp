@@ -158,11 +158,12 @@ __device__ __forceinline__ double normal_f64( uint32_t x )
 // FP32 inverse normal from the 32-bit integer coordinate, in the erfinv form
 // of M. Giles ("Approximating the erfinv function", GPU Computing Gems 2011)
 // with coefficients fitted by thoth_amd/csrc/fit_inverse_normal.py to
-// Phi^-1 directly (max relative error 6.2e-7 over every 32-bit input):
+// Phi^-1 directly (max relative error 3.0e-7 over every 32-bit input):
 //   x = 2u - 1, w = -log(4u(1-u)), Phi^-1(u) = x P(w).
 // 4u(1-u) is formed from the integers u*2^32 and (1-u)*2^32, so it keeps its
-// precision in both tails; one log per normal, the tail polynomial only runs
-// in the ~0.3 % of draws with w >= 5.  The FP64 oracle mode keeps Acklam.
+// precision in both tails; one log per normal.  The central polynomial
+// (degree 10) reaches w = 8, so the tail polynomial runs for 1.7e-4 of the
+// draws.  The FP64 oracle mode keeps Acklam.
 __device__ __forceinline__ float normal_f32( uint32_t x )
 {
     // u = 0 (x = 0, reachable under a random digital shift) is read as the
@@ -173,32 +174,60 @@ __device__ __forceinline__ float normal_f32( uint32_t x )
     const float w = -fast_log( a * b * 2.168404344971009e-19f );  // 2^-62
     const float xs = (float)(int32_t)( x - 0x80000000u ) * 4.656612873077393e-10f;  // 2^-31
     float p;
-    if ( w < 5.0f )
+    if ( w < 8.0f )
     {
-        const float t = w - 2.5f;
-        p = 3.958320987e-08f;
-        p = fmaf( p, t, 4.851791016e-07f );
-        p = fmaf( p, t, -4.981194608e-06f );
-        p = fmaf( p, t, -6.207880344e-06f );
-        p = fmaf( p, t, 3.091150937e-04f );
-        p = fmaf( p, t, -1.773041744e-03f );
-        p = fmaf( p, t, -5.908129905e-03f );
-        p = fmaf( p, t, 3.488026846e-01f );
-        p = fmaf( p, t, 2.123313560e+00f );
+        const float t = w - 4.0f;
+        p = 5.622668877e-10f;
+        p = fmaf( p, t, -7.025212874e-10f );
+        p = fmaf( p, t, -4.983229672e-08f );
+        p = fmaf( p, t, 2.970335407e-07f );
+        p = fmaf( p, t, 7.164527608e-07f );
+        p = fmaf( p, t, -2.224123390e-05f );
+        p = fmaf( p, t, 1.679629502e-04f );
+        p = fmaf( p, t, -2.876509369e-04f );
+        p = fmaf( p, t, -1.021180627e-02f );
+        p = fmaf( p, t, 3.229416742e-01f );
+        p = fmaf( p, t, 2.628710527e+00f );
     }
     else
     {
         const float t = __builtin_amdgcn_sqrtf( w ) - 3.0f;
-        p = 5.523418413e-06f;
-        p = fmaf( p, t, -2.638466750e-04f );
-        p = fmaf( p, t, 1.661917106e-03f );
-        p = fmaf( p, t, -4.731942225e-03f );
-        p = fmaf( p, t, 8.164933407e-03f );
-        p = fmaf( p, t, -1.093819435e-02f );
-        p = fmaf( p, t, 1.334743934e-02f );
-        p = fmaf( p, t, 1.416595396e+00f );
-        p = fmaf( p, t, 4.006434468e+00f );
+        p = 5.879603428e-05f;
+        p = fmaf( p, t, -5.247431012e-04f );
+        p = fmaf( p, t, 2.067292677e-03f );
+        p = fmaf( p, t, -4.845630704e-03f );
+        p = fmaf( p, t, 7.938630347e-03f );
+        p = fmaf( p, t, -1.079775749e-02f );
+        p = fmaf( p, t, 1.336126330e-02f );
+        p = fmaf( p, t, 1.416582332e+00f );
+        p = fmaf( p, t, 4.006434188e+00f );
     }
+    return xs * p;
+}
+
+// The central piece of normal_f32 alone (no branch); `tail` is set where
+// normal_f32 takes the tail polynomial instead, and for those lanes the
+// caller replaces the value by normal_f32's.
+__device__ __forceinline__ float normal_f32_central( uint32_t x, bool& tail )
+{
+    x = x > 1u ? x : 1u;
+    const float a = (float)x;
+    const float b = (float)( 0u - x );
+    const float w = -fast_log( a * b * 2.168404344971009e-19f );  // 2^-62
+    const float xs = (float)(int32_t)( x - 0x80000000u ) * 4.656612873077393e-10f;  // 2^-31
+    const float t = w - 4.0f;
+    float p = 5.622668877e-10f;
+    p = fmaf( p, t, -7.025212874e-10f );
+    p = fmaf( p, t, -4.983229672e-08f );
+    p = fmaf( p, t, 2.970335407e-07f );
+    p = fmaf( p, t, 7.164527608e-07f );
+    p = fmaf( p, t, -2.224123390e-05f );
+    p = fmaf( p, t, 1.679629502e-04f );
+    p = fmaf( p, t, -2.876509369e-04f );
+    p = fmaf( p, t, -1.021180627e-02f );
+    p = fmaf( p, t, 3.229416742e-01f );
+    p = fmaf( p, t, 2.628710527e+00f );
+    tail = !( w < 8.0f );
     return xs * p;
 }
 
@@ -240,32 +269,55 @@ struct QeDraw
     float mean;  // E[v' | v] of the FP32 draw as computed below
 };
 
-// Same scheme in FP32 from host-precomputed step constants.  1-u is formed
-// directly as Phi(-z) so the exponential tail keeps its precision.
-__device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc, QeDraw* draw = nullptr )
+// Same scheme in FP32 from host-precomputed step constants, in three parts
+// so that the uniform-book kernels can run the quadratic branch for every
+// lane and the exponential one for the lanes that need it (same arithmetic
+// as taking the branch).  1-u is formed directly as Phi(-z) so the
+// exponential tail keeps its precision.
+struct QeMoments
+{
+    float m, psi;
+};
+
+__device__ __forceinline__ QeMoments qe_moments( float v, const float* __restrict__ sc )
 {
     const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
     const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
-    const float psi = ( m > 0.0f ) ? fast_div( s2, m * m ) : 0.0f;
-    if ( psi <= 1.5f )
-    {
-        const float c = fast_div( 2.0f, psi );
-        const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
-        const float a = fast_div( m, 1.0f + b2 );
-        const float sb = __builtin_amdgcn_sqrtf( b2 );
-        const float bz = sb + z;
-        if ( draw )
-            draw->mean = a * fmaf( sb, sb, 1.0f );  // E[a (sb + Z)^2]
-        return a * bz * bz;
-    }
-    // p = (psi-1)/(psi+1), beta = (1-p)/m; v = log((1-p)/(1-u)) / beta
-    const float inv = fast_rcp( psi + 1.0f );
+    return { m, ( m > 0.0f ) ? fast_div( s2, m * m ) : 0.0f };
+}
+
+// psi <= 1.5: v' = a (sqrt(b2) + z)^2
+__device__ __forceinline__ float qe_quadratic( const QeMoments& q, float z, QeDraw* draw )
+{
+    const float c = fast_div( 2.0f, q.psi );
+    const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
+    const float a = fast_div( q.m, 1.0f + b2 );
+    const float sb = __builtin_amdgcn_sqrtf( b2 );
+    const float bz = sb + z;
+    if ( draw )
+        draw->mean = a * fmaf( sb, sb, 1.0f );  // E[a (sb + Z)^2]
+    return a * bz * bz;
+}
+
+// psi > 1.5: p = (psi-1)/(psi+1), beta = (1-p)/m; v' = log((1-p)/(1-u)) / beta
+__device__ __forceinline__ float qe_exponential( const QeMoments& q, float z, QeDraw* draw )
+{
+    const float inv = fast_rcp( q.psi + 1.0f );
     const float one_minus_p = 2.0f * inv;
     if ( draw )
-        draw->mean = one_minus_p * m * 0.5f * ( psi + 1.0f );  // (1 - p) / beta
+        draw->mean = one_minus_p * q.m * 0.5f * ( q.psi + 1.0f );  // (1 - p) / beta
     const float one_minus_u = 0.5f * erfc_fast( z * 0.70710678118654752f );
-    return ( one_minus_u >= one_minus_p ) ? 0.0f
-                                          : fast_log( fast_div( one_minus_p, one_minus_u ) ) * m * 0.5f * ( psi + 1.0f );
+    return ( one_minus_u >= one_minus_p )
+               ? 0.0f
+               : fast_log( fast_div( one_minus_p, one_minus_u ) ) * q.m * 0.5f * ( q.psi + 1.0f );
+}
+
+__device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc, QeDraw* draw = nullptr )
+{
+    const QeMoments q = qe_moments( v, sc );
+    if ( q.psi <= 1.5f )
+        return qe_quadratic( q, z, draw );
+    return qe_exponential( q, z, draw );
 }
 
 template <class R>
@@ -397,6 +449,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 {
     constexpr bool F64 = std::is_same_v<R, double>;
     constexpr bool uniform = KIND != 0;
+    constexpr bool qe_phase = uniform;
     static_assert( KIND == 0 || !F64, "uniform-book kernels are FP32" );
     constexpr int NF = 2 * NA;
     constexpr int NFV = NF <= 2 ? 2 : NF <= 4 ? 4 : NF <= 8 ? 8 : 16;
@@ -754,6 +807,37 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     // walk), then pop W(t_{p+1}) as the normalised increment.  It depends on
     // the Sobol points only, so it runs one step ahead of the equities and the
     // scheduler overlaps it with their (path-dependent) arithmetic.
+    // the normals of build index bidx for every factor.  Uniform FP32
+    // kernels evaluate the central polynomial for all factors straight-line
+    // and redo the lanes in the tail (w >= 8) in one block afterwards.
+    auto normals = [&]( int bidx, R* zf ) __attribute__( ( always_inline ) )
+    {
+        if constexpr ( uniform )
+        {
+            uint32_t tails = 0;
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+            {
+                bool t;
+                zf[f] = normal_f32_central( sob( bidx * NF + f ), t );
+                tails |= t ? 1u << f : 0u;
+            }
+            if ( __builtin_amdgcn_ballot_w64( tails != 0 ) )
+            {
+#pragma unroll
+                for ( int f = 0; f < NF; f++ )
+                    if ( ( tails >> f ) & 1u )
+                        zf[f] = normal_f32( sob( bidx * NF + f ) );
+            }
+        }
+        else
+        {
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+                if ( f < F )
+                    zf[f] = normal_of<R>( sob( bidx * F + f ) );
+        }
+    };
     auto bridge_step = [&]( int p, facv& nz ) __attribute__( ( always_inline ) )
     {
         const int i = p + 1;
@@ -782,12 +866,14 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 #pragma unroll
             for ( int f = 0; f < NFV; f++ )
                 nz[f] = R( 0 );
+            R zf[NF];
+            normals( bidx, zf );
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
             {
                 if ( f < F )
                 {
-                    const R z = normal_of<R>( sob( bidx * F + f ) );
+                    const R z = zf[f];
                     const R w = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
                     if constexpr ( F64 )
                         nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
@@ -816,12 +902,14 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 rw = B.op_wf[3 * o + 1];
                 sd = B.op_wf[3 * o + 2];
             }
+            R zf[NF];
+            normals( bidx, zf );
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
             {
                 if ( f < F )
                 {
-                    const R z = normal_of<R>( sob( bidx * F + f ) );
+                    const R z = zf[f];
                     // path_generator.cpp:175-182
                     const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
                     stk[f][pos] = top[f];
@@ -866,6 +954,35 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             __syncthreads();  // the group of layer i is visible, the previous group is free
             if ( i + G <= m )
                 stage_group( i + G );
+        }
+        // ---- uniform FP32 kernels: the QE variance draws of every equity,
+        // quadratic branch straight-line, the exponential-branch lanes redone
+        // in one block (qe_f32's arithmetic either way)
+        [[maybe_unused]] float vq[NA];
+        [[maybe_unused]] QeDraw dq[NA];
+        if constexpr ( qe_phase )
+        {
+            uint32_t expo = 0;
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+            {
+                const float* sc = B.sc + ( (size_t)a * nd + i ) * SC_COUNT;
+                const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+                const QeMoments q = qe_moments( Var[a], sc );
+                vq[a] = qe_quadratic( q, nz[vf], cv_on ? &dq[a] : nullptr );
+                expo |= q.psi <= 1.5f ? 0u : 1u << a;
+            }
+            if ( __builtin_amdgcn_ballot_w64( expo != 0 ) )
+            {
+#pragma unroll
+                for ( int a = 0; a < NA; a++ )
+                    if ( ( expo >> a ) & 1u )
+                    {
+                        const float* sc = B.sc + ( (size_t)a * nd + i ) * SC_COUNT;
+                        const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+                        vq[a] = qe_exponential( qe_moments( Var[a], sc ), nz[vf], cv_on ? &dq[a] : nullptr );
+                    }
+            }
         }
         // ---- equities
 #pragma unroll
@@ -938,6 +1055,12 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             [[maybe_unused]] QeDraw draw;
             if constexpr ( F64 )
                 vc = qe_f64( vp, B.dt[i], zv, B.asset[a].kappa, B.asset[a].theta, B.asset[a].xi );
+            else if constexpr ( qe_phase )
+            {
+                vc = vq[a];
+                if constexpr ( cv_on )
+                    draw = dq[a];
+            }
             else if constexpr ( cv_on )
                 vc = qe_f32( vp, zv, sc, &draw );
             else
