@@ -154,7 +154,7 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
         const DevAsset& A = db.asset[a];
         for ( int i = 1; i < nd; i++ )
         {
-            float* c = &sc[( (size_t)a * nd + i ) * SC_COUNT];
+            float* c = &sc[( (size_t)i * na + a ) * SC_COUNT];  // date-major: one base per step
             const double dt = in.dt[i];
             c[SC_CARRY] = (float)( in.drift[(size_t)a * nd + i] * in.t[i] - in.drift[(size_t)a * nd + i - 1] * in.t[i - 1] );
             c[SC_DT] = (float)dt;
@@ -194,12 +194,13 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
         const size_t nl = (size_t)na * nd;
         std::vector<float> levf( in.lev_values, in.lev_values + nl * P );
         std::vector<double> off( nl );
-        std::vector<float> offf( nl ), inv( nl );
+        std::vector<float> offf( nl ), inv( nl );  // FP32 copies date-major [i][a]
         for ( size_t k = 0; k < nl; k++ )
         {
             off[k] = (double)in.lev_offset[k];
-            offf[k] = (float)in.lev_offset[k];
-            inv[k] = in.lev_ln_step[k] > 0 ? (float)( 1.0 / in.lev_ln_step[k] ) : 0.0f;
+            const size_t a = k / nd, i = k % nd, kf = i * na + a;
+            offf[kf] = (float)in.lev_offset[k];
+            inv[kf] = in.lev_ln_step[k] > 0 ? (float)( 1.0 / in.lev_ln_step[k] ) : 0.0f;
         }
         of.lev = ar.put( in.lev_values, nl * P );
         of.levf = ar.put( levf );
@@ -299,7 +300,7 @@ int PackCv( int bi, const thoth_mcl_book& in, const CvSpec& cv, DevBook& db, Off
     std::vector<float> scale( (size_t)na * nd, 0.0f );
     for ( int a = 0; a < na; a++ )
         for ( int i = 1; i < nd; i++ )
-            scale[(size_t)a * nd + i] =
+            scale[(size_t)i * na + a] =  // date-major
                 (float)( std::exp( in.t[0] * in.disc_rate[0] - in.t[i] * in.disc_rate[i] ) * 100.0 / in.spot[a] );
     db.cv_J = cv.J;
     db.cv_N = cv.N;

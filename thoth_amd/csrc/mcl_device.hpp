@@ -88,13 +88,13 @@ struct DevBook
     double chol[kMaxAssets * kMaxAssets];
     float cholf[kMaxAssets * kMaxAssets];
     const double* drift;  // [n_assets * n_dates] drift-node value
-    const float* sc;      // [n_assets * n_dates * SC_COUNT]
+    const float* sc;      // date-major [n_dates][n_assets][SC_COUNT]
     const double* lev;    // [n_assets * n_dates * lev_points]
     const float* levf;
     const double* lev_ln_step;  // [n_assets * n_dates]
     const double* lev_offset;   // [n_assets * n_dates] (integral values)
-    const float* lev_offset_f;  // FP32 copy (exact: |offset| < 2^24)
-    const float* lev_inv_step;  // [n_assets * n_dates] 1 / ln_step
+    const float* lev_offset_f;  // FP32 copy (exact: |offset| < 2^24), date-major [n_dates][n_assets]
+    const float* lev_inv_step;  // [n_dates][n_assets] 1 / ln_step
     // payoffs
     int c_kind[kMaxContracts], c_call[kMaxContracts], c_udl[kMaxContracts];
     int c_up[kMaxContracts], c_in[kMaxContracts];
@@ -109,7 +109,7 @@ struct DevBook
     int cv_J;
     const int* cv_start;        // [n_dates] j when date i = d_j (j < J), else -1
     const int* cv_end;          // [n_dates] j when date i = d_{j+1}, else -1
-    const float* cv_scale;      // [n_assets * n_dates] df(t_i) * 100 / S0_a
+    const float* cv_scale;      // [n_dates][n_assets] df(t_i) * 100 / S0_a
     float cv_inv_T;             // 1 / t_m
     float cv_lnS0[kMaxAssets];
     float cv_ln_scale[kMaxAssets];   // 1 / (reference vol * sqrt(t_m)): standardised log-moneyness

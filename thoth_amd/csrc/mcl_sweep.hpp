@@ -450,6 +450,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     constexpr bool F64 = std::is_same_v<R, double>;
     constexpr bool uniform = KIND != 0;
     constexpr bool qe_phase = uniform;
+    const int nas = uniform ? NA : B.n_assets;  // row length of the date-major per-step arrays
     static_assert( KIND == 0 || !F64, "uniform-book kernels are FP32" );
     constexpr int NF = 2 * NA;
     constexpr int NFV = NF <= 2 ? 2 : NF <= 4 ? 4 : NF <= 8 ? 8 : 16;
@@ -966,7 +967,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 #pragma unroll
             for ( int a = 0; a < NA; a++ )
             {
-                const float* sc = B.sc + ( (size_t)a * nd + i ) * SC_COUNT;
+                const float* sc = B.sc + ( (size_t)i * nas + a ) * SC_COUNT;
                 const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                 const QeMoments q = qe_moments( Var[a], sc );
                 vq[a] = qe_quadratic( q, nz[vf], cv_on ? &dq[a] : nullptr );
@@ -978,7 +979,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 for ( int a = 0; a < NA; a++ )
                     if ( ( expo >> a ) & 1u )
                     {
-                        const float* sc = B.sc + ( (size_t)a * nd + i ) * SC_COUNT;
+                        const float* sc = B.sc + ( (size_t)i * nas + a ) * SC_COUNT;
                         const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                         vq[a] = qe_exponential( qe_moments( Var[a], sc ), nz[vf], cv_on ? &dq[a] : nullptr );
                     }
@@ -1009,7 +1010,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                         eps += nz[v] * B.cholf[a * kMaxAssets + v];
                 }
             }
-            const float* sc = B.sc + ( (size_t)a * nd + i ) * SC_COUNT;
+            const float* sc = B.sc + ( (size_t)i * nas + a ) * SC_COUNT;
             R cd;
             if constexpr ( F64 )
                 cd = B.drift[(size_t)a * nd + i] * B.t[i] - B.drift[(size_t)a * nd + i - 1] * B.t[i - 1];
@@ -1040,9 +1041,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                         // first-order Doob increment S_{i-1} vol dW: zero conditional mean
                         const float M = sold * sc[SC_VOL] * dw;
                         if constexpr ( cv_infer )
-                            cvacc = fmaf( hr[a] * B.cv_scale[(size_t)a * nd + i], M, cvacc );
+                            cvacc = fmaf( hr[a] * B.cv_scale[(size_t)i * nas + a], M, cvacc );
                         else
-                            dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
+                            dmacc[a] = fmaf( B.cv_scale[(size_t)i * nas + a], M, dmacc[a] );
                     }
                 }
                 continue;
@@ -1076,7 +1077,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 if constexpr ( F64 )
                     f = log( S[a] ) / B.lev_ln_step[li] - B.lev_offset[li];
                 else
-                    f = fast_log( S[a] ) * B.lev_inv_step[li] - B.lev_offset_f[li];
+                    f = fast_log( S[a] ) * B.lev_inv_step[(size_t)i * nas + a] - B.lev_offset_f[(size_t)i * nas + a];
                 if constexpr ( F64 )
                 {
                     const double* lv = B.lev + li * B.lev_points;
@@ -1162,9 +1163,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     }
                     M *= sold;
                     if constexpr ( cv_infer )
-                        cvacc = fmaf( hr[a] * B.cv_scale[(size_t)a * nd + i], M, cvacc );
+                        cvacc = fmaf( hr[a] * B.cv_scale[(size_t)i * nas + a], M, cvacc );
                     else
-                        dmacc[a] = fmaf( B.cv_scale[(size_t)a * nd + i], M, dmacc[a] );
+                        dmacc[a] = fmaf( B.cv_scale[(size_t)i * nas + a], M, dmacc[a] );
                 }
             }
         }
