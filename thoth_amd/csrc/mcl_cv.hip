@@ -149,6 +149,7 @@ __global__ void __launch_bounds__( 256 ) cv_part_kernel( const uint2* __restrict
 // (deterministic).  loss_part[block] = sum of Y^2 over the block's paths when
 // the block holds chunk 0, else 0.
 constexpr int kGradN = kMlpWSize + kMlpBSize;
+constexpr int kGradGroups = kCvGradGroups;  // first-level groups of the gradient-slice sum
 constexpr size_t kCvGradLds = 4 * sizeof( MlpActLds );
 static_assert( kCvGradLds >= kGradN * sizeof( float ), "gradient reduction fits in the activation area" );
 
@@ -258,16 +259,31 @@ __global__ void __launch_bounds__( 256 ) cv_grad_kernel( const uint2* __restrict
 
 // Adam (Kingma & Ba): the gradient of element e is the ordered sum of the
 // per-block slices; FP32 master weights and moments.
+// first level of the gradient sum: group y of kGradGroups adds its slices in
+// order (a fixed grouping, so the sum is the same on every run)
+__global__ void cv_gsum_kernel( const float* __restrict__ gpart, int nslices, float* __restrict__ gsum )
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( e >= kGradN )
+        return;
+    const int per = ( nslices + kGradGroups - 1 ) / kGradGroups;
+    const int k0 = blockIdx.y * per, k1 = min( nslices, k0 + per );
+    float g = 0.0f;
+    for ( int k = k0; k < k1; k++ )
+        g += gpart[(size_t)k * kGradN + e];
+    gsum[(size_t)blockIdx.y * kGradN + e] = g;
+}
+
 __global__ void cv_adam_kernel( float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
-                                const float* __restrict__ gpart, int nslices, const int* __restrict__ step, float lr,
-                                float b1, float b2, float eps )
+                                const float* __restrict__ gsum, const int* __restrict__ step, float lr, float b1,
+                                float b2, float eps )
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if ( e >= kGradN )
         return;
     float g = 0.0f;
-    for ( int k = 0; k < nslices; k++ )
-        g += gpart[(size_t)k * kGradN + e];
+    for ( int k = 0; k < kGradGroups; k++ )
+        g += gsum[(size_t)k * kGradN + e];
     const int t = *step + 1;
     const float mm = b1 * m[e] + ( 1.0f - b1 ) * g;
     const float vv = b2 * v[e] + ( 1.0f - b2 ) * g * g;
@@ -318,7 +334,7 @@ CvTrainLayout cv_train_layout( int B, int J )
 
 bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd, float* w,
                            float* m, float* v, const uint16_t* feat, const float* dm, const float* pay, int64_t N,
-                           int* step, float* cvpart, float* gpart, float* loss_part, float lr )
+                           int* step, float* cvpart, float* gpart, float* gsum, float* loss_part, float lr )
 {
     if ( l.blocks < 1 || l.blocks * 4 != l.waves )
         return false;
@@ -327,8 +343,10 @@ bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t
     hipLaunchKernelGGL( cv_grad_kernel, dim3( l.blocks ), dim3( 256 ), kCvGradLds, s,
                         reinterpret_cast<const uint2*>( fwd ), reinterpret_cast<const uint2*>( bwd ), w + kMlpWSize,
                         feat, dm, pay, N, l.J, l.B, l.dpc, l.chunks, step, cvpart, gpart, loss_part );
-    hipLaunchKernelGGL( cv_adam_kernel, dim3( ( kGradN + 255 ) / 256 ), dim3( 256 ), 0, s, w, m, v, gpart, l.blocks,
-                        step, lr, 0.9f, 0.999f, 1e-8f );
+    hipLaunchKernelGGL( cv_gsum_kernel, dim3( ( kGradN + 255 ) / 256, kGradGroups ), dim3( 256 ), 0, s, gpart, l.blocks,
+                        gsum );
+    hipLaunchKernelGGL( cv_adam_kernel, dim3( ( kGradN + 255 ) / 256 ), dim3( 256 ), 0, s, w, m, v, gsum, step, lr,
+                        0.9f, 0.999f, 1e-8f );
     hipLaunchKernelGGL( mlp_pack_kernel, dim3( ( kMlpWSize + 255 ) / 256 ), dim3( 256 ), 0, s, w,
                         const_cast<uint16_t*>( fwd ), const_cast<uint16_t*>( bwd ) );
     hipLaunchKernelGGL( cv_step_kernel, dim3( 1 ), dim3( 1 ), 0, s, step );

@@ -173,9 +173,10 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_train( cons
         uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
         float* cvpart = dev.alloc<float>( (size_t)lay.chunks * B );
         float* gpart = dev.alloc<float>( (size_t)lay.blocks * kN );
+        float* gsum = dev.alloc<float>( (size_t)kCvGradGroups * kN );
         float* loss_part = dev.alloc<float>( lay.blocks );
         int* step = dev.alloc<int>( 1 );
-        if ( !dw || !dm_ || !dv || !fwd || !bwd || !cvpart || !gpart || !loss_part || !step )
+        if ( !dw || !dm_ || !dv || !fwd || !bwd || !cvpart || !gpart || !gsum || !loss_part || !step )
             return SetError( THOTH_ERR_DEVICE, "cannot allocate the training buffers" );
         HIP_OK( cv_train_init() );
         std::vector<float> w0( kN );
@@ -191,7 +192,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_train( cons
         hipGraphExec_t exec = nullptr;
         HIP_OK( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
         const bool launched = launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, cv.feat, cv.dm, cv.pay, N, step,
-                                                    cvpart, gpart, loss_part, cfg->learning_rate );
+                                                    cvpart, gpart, gsum, loss_part, cfg->learning_rate );
         HIP_OK( hipStreamEndCapture( s, &graph ) );
         if ( !launched )
         {
@@ -456,9 +457,11 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_loss_grad( 
     float* dpay = dev.alloc<float>( B );
     float* cvpart = dev.alloc<float>( (size_t)lay.chunks * B );
     float* gpart = dev.alloc<float>( (size_t)lay.blocks * kN );
+    float* gsum = dev.alloc<float>( (size_t)kCvGradGroups * kN );
     float* loss_part = dev.alloc<float>( lay.blocks );
     int* step = dev.alloc<int>( 1 );
-    if ( !dw || !dm_ || !dv || !fwd || !bwd || !dfeat || !ddm || !dpay || !cvpart || !gpart || !loss_part || !step )
+    if ( !dw || !dm_ || !dv || !fwd || !bwd || !dfeat || !ddm || !dpay || !cvpart || !gpart || !gsum || !loss_part ||
+         !step )
         return SetError( THOTH_ERR_DEVICE, "allocation failed" );
     HIP_OK( cv_train_init() );
     hipStream_t s = Stream();
@@ -471,8 +474,8 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_loss_grad( 
     HIP_OK( hipMemsetAsync( step, 0, sizeof( int ), s ) );
     launch_mlp_pack( s, dw, fwd, bwd );
     // gradient only: the training step with a zero learning rate leaves w unchanged
-    if ( !launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, dfeat, ddm, dpay, B, step, cvpart, gpart, loss_part,
-                                0.0f ) )
+    if ( !launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, dfeat, ddm, dpay, B, step, cvpart, gpart, gsum,
+                                loss_part, 0.0f ) )
         return SetError( THOTH_ERR_INVALID, "batch layout not supported" );
     HIP_OK( hipGetLastError() );
     std::vector<float> gp( (size_t)lay.blocks * kN ), lp( lay.blocks );

@@ -64,10 +64,12 @@ struct CvTrainLayout
 };
 CvTrainLayout cv_train_layout( int B, int J );
 hipError_t cv_train_init();  // once per process before the first step (LDS limit of the gradient kernel)
-// buffers: cvpart [chunks * B], gpart [blocks * THOTH_CV_WEIGHTS], loss_part [blocks]
+// buffers: cvpart [chunks * B], gpart [blocks * THOTH_CV_WEIGHTS], gsum [kCvGradGroups * THOTH_CV_WEIGHTS],
+// loss_part [blocks]
+constexpr int kCvGradGroups = 32;
 bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd, float* w,
                            float* m, float* v, const uint16_t* feat, const float* dm, const float* pay, int64_t N,
-                           int* step, float* cvpart, float* gpart, float* loss_part, float lr );
+                           int* step, float* cvpart, float* gpart, float* gsum, float* loss_part, float lr );
 bool launch_persistent_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                            const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
                            int n_items, uint32_t ntiles, int* counter );
