@@ -338,19 +338,50 @@ __device__ __forceinline__ R wave_sum( R x )
 template <class R, int NF, int DEPTH>
 struct BridgeWalk
 {
+    // depth-8 stacks two factors per 16-element vector (indexed by one
+    // s_set_gpr_idx move instead of eight selects), as in sweep_block
+    static constexpr bool paired = DEPTH == 8 && NF % 2 == 0;
     typedef R stackv __attribute__( ( ext_vector_type( DEPTH ) ) );
-    stackv stk[NF];
+    typedef R stack2v __attribute__( ( ext_vector_type( 16 ) ) );
+    stackv stk[paired ? 1 : NF];
+    stack2v stk2[paired ? NF / 2 : 1];
     R top[NF], prev[NF];
     int sp;
+
+    __device__ __forceinline__ R stk_get( int f, int pos ) const
+    {
+        if constexpr ( paired )
+            return stk2[f >> 1][pos + ( ( f & 1 ) << 3 )];
+        else
+            return stk[f][pos];
+    }
+    __device__ __forceinline__ void stk_set( int f, int pos, R x )
+    {
+        if constexpr ( paired )
+            stk2[f >> 1][pos + ( ( f & 1 ) << 3 )] = x;
+        else
+            stk[f][pos] = x;
+    }
 
     // endpoint first (build index 0, path_generator.cpp:166-167)
     __device__ __forceinline__ void start( const DevBook& B, const SobolLane& sob )
     {
         const int F = B.factors;
+        if constexpr ( paired )
+        {
+#pragma unroll
+            for ( int k = 0; k < NF / 2; k++ )
+                stk2[k] = (stack2v)( R( 0 ) );
+        }
+        else
+        {
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+                stk[f] = (stackv)( R( 0 ) );
+        }
 #pragma unroll
         for ( int f = 0; f < NF; f++ )
         {
-            stk[f] = (stackv)( R( 0 ) );
             prev[f] = R( 0 );
             top[f] = R( 0 );
             if ( f < F )
@@ -399,7 +430,7 @@ struct BridgeWalk
                 {
                     const R z = normal_of<R>( sob( bidx * F + f ) );
                     const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
-                    stk[f][pos] = top[f];
+                    stk_set( f, pos, top[f] );
                     top[f] = nv;
                 }
             }
@@ -419,7 +450,7 @@ struct BridgeWalk
                     nz[f] = ( w - prev[f] ) * B.binv[i];
                 prev[f] = w;
                 if ( sp >= 2 )
-                    top[f] = stk[f][pos];
+                    top[f] = stk_get( f, pos );
             }
         }
         sp--;
@@ -533,12 +564,44 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     __syncthreads();
 
     // ---- bridge state: endpoint first (build index 0, path_generator.cpp:166-167)
-    stackv stk[NF];
+    // Stacks of depth 8 are held two factors per 16-element register
+    // vector (factor 2k in elements 0..7, 2k+1 in 8..15): the backend indexes
+    // a 16-element vector with s_set_gpr_idx and one move, while it lowers a
+    // variable index into an 8-element vector as eight selects.
+    constexpr bool paired = DEPTH == 8;
+    typedef R stack2v __attribute__( ( ext_vector_type( 16 ) ) );
+    [[maybe_unused]] stackv stk[paired ? 1 : NF];
+    [[maybe_unused]] stack2v stk2[paired ? NF / 2 : 1];
+    auto stk_get = [&]( int f, int pos ) __attribute__( ( always_inline ) ) -> R
+    {
+        if constexpr ( paired )
+            return stk2[f >> 1][pos + ( ( f & 1 ) << 3 )];
+        else
+            return stk[f][pos];
+    };
+    auto stk_set = [&]( int f, int pos, R x ) __attribute__( ( always_inline ) )
+    {
+        if constexpr ( paired )
+            stk2[f >> 1][pos + ( ( f & 1 ) << 3 )] = x;
+        else
+            stk[f][pos] = x;
+    };
+    if constexpr ( paired )
+    {
+#pragma unroll
+        for ( int k = 0; k < NF / 2; k++ )
+            stk2[k] = (stack2v)( R( 0 ) );
+    }
+    else
+    {
+#pragma unroll
+        for ( int f = 0; f < NF; f++ )
+            stk[f] = (stackv)( R( 0 ) );
+    }
     R top[NF], prev[NF];
 #pragma unroll
     for ( int f = 0; f < NF; f++ )
     {
-        stk[f] = (stackv)( R( 0 ) );
         prev[f] = R( 0 );
         top[f] = R( 0 );
         if ( f < F )
@@ -913,7 +976,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     const R z = zf[f];
                     // path_generator.cpp:175-182
                     const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
-                    stk[f][pos] = top[f];
+                    stk_set( f, pos, top[f] );
                     top[f] = nv;
                 }
             }
@@ -937,7 +1000,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                         nz[f] = ( w - prev[f] ) * B.binv[i];
                     prev[f] = w;
                     if ( sp >= 2 )
-                        top[f] = stk[f][pos];
+                        top[f] = stk_get( f, pos );
                 }
             }
             sp--;
