@@ -138,7 +138,7 @@ def hbm_traffic(workload_name, paths, kernel="sweep_kernel"):
     probe_mib = 256
     me = [sys.executable, str(Path(__file__).resolve())]
     run = me + ["--workload", workload_name, "--paths", str(paths), "--steps", "1", "--warmup", "0",
-                "--no-cpu", "--no-t1bp", "--no-grid", "--no-traffic"]
+                "--no-cpu", "--no-t1bp", "--no-grid", "--no-traffic", "--no-heston"]
     probe = me + ["--copy-probe", str(probe_mib)]
     detail = {}
     total = 0.0
@@ -218,6 +218,39 @@ def time_to_1bp(replicates, paths, seed=101, target=0.01, other_regimes=((512, 1
     return out
 
 
+def heston_barrier(paths=1 << 20, steps=252, repeats=3):
+    """The single-asset Heston down-and-out call (v0 = theta = 0.04, kappa
+    1.5, xi 0.5, rho -0.7, K 100, H 80 monitored daily) on 252 uniform steps:
+    FP32 device time of the plain sweep with the bump Greeks (base + delta,
+    gamma and vega scenario books in one launch) and of the pathwise kernel
+    (delta, vega, gamma from pathwise deltas), and the FP32 - FP64 difference
+    of every figure on the same Sobol points."""
+    greeks = abi.GREEK_DELTA | abi.GREEK_GAMMA | abi.GREEK_VEGA
+    out = {"workload": "heston_down_out_call", "paths": paths, "steps": steps}
+    res = {}
+    for prec, name in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
+        spec = configs.heston_down_out_call(paths=paths, precision=prec, uniform_steps=steps, greeks=greeks)
+        b = BuiltRequest(spec)
+        plan = mcl.Plan(b)
+        plan.run()
+        ms = [plan.run().sweep_ms for _ in range(repeats)]
+        r = plan.run()
+        pw = [mcl.price_pathwise(b) for _ in range(repeats)]
+        res[name] = {"bump": {"premium": r.premium, "premium_trust": r.premium_trust, "delta": r.delta,
+                              "gamma": r.gamma, "vega": r.vega, "ms": min(ms), "books": r.n_scenarios},
+                     "pathwise": {"premium": pw[-1].premium, "delta": pw[-1].delta, "delta_trust": pw[-1].delta_trust,
+                                  "gamma": pw[-1].gamma, "vega": pw[-1].vega, "vega_trust": pw[-1].vega_trust,
+                                  "ms": min(p.sweep_ms for p in pw)}}
+    out.update(res["fp32"])
+    out["paths_per_s_bump"] = paths * res["fp32"]["bump"]["books"] / (res["fp32"]["bump"]["ms"] * 1e-3)
+    out["paths_per_s_pathwise"] = paths / (res["fp32"]["pathwise"]["ms"] * 1e-3)
+    out["fp32_minus_fp64"] = {
+        kind: {k: res["fp32"][kind][k] - res["fp64"][kind][k] for k in ("premium", "delta", "gamma", "vega")}
+        for kind in ("bump", "pathwise")}
+    out["fp64_ms"] = {kind: res["fp64"][kind]["ms"] for kind in ("bump", "pathwise")}
+    return out
+
+
 def vanilla_grids(paths, replays=5):
     """The 5 LSV underlyings' vanilla grids (41 strikes x 12 monthly maturities,
     calls and puts, base + spot-bump + vol-bump books = 15 books of `paths`
@@ -261,6 +294,7 @@ def main():
     ap.add_argument("--no-grid", action="store_true", help="skip the vanilla-grid batch")
     ap.add_argument("--grid-paths", type=int, default=1 << 18)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    ap.add_argument("--no-heston", action="store_true", help="skip the Heston barrier (bump and pathwise Greeks)")
     ap.add_argument("--cpu-all-cores-paths", type=int, default=1 << 14,
                     help="paths per core of the all-cores CPU baseline (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -331,6 +365,8 @@ def main():
     }
     if not args.no_grid:
         line["vanilla_grids"] = vanilla_grids(args.grid_paths)
+    if not args.no_heston:
+        line["heston_barrier"] = heston_barrier()
     if not args.no_t1bp and args.workload == "lsv_worst_of_basket":
         line["time_to_1bp"] = time_to_1bp(args.t1bp_replicates, args.t1bp_paths)
     if not args.no_traffic:
