@@ -1,0 +1,153 @@
+"""Edge cases of the device path: ragged and tiny path counts, Sobol skips
+that start inside a 256-point block, books of 7 and 8 equities (the widest
+instantiation, partly filled), the path-count limits, and requests that must
+come back as error codes before any launch.
+
+The reference's engine takes points skip+1 .. skip+paths of its Sobol
+sequence (sobol.cpp Skip, pricer_mcl.cpp:204-209) for any path count in
+(1, 2e8] (mcl_configuration.cpp:30-46); the device assigns each Sobol index to
+one lane of the 256-point block that owns it, so a ragged count only masks
+lanes off.  FP64 is held to the oracle at 1e-10; FP32 has no oracle of its own
+precision, so its ragged runs are held to exact additivity: the per-path
+values do not depend on which launch computes them, and the premium of
+[a, c) is the path-weighted mean of those of [a, b) and [b, c) up to the
+FP64 summation order (1e-12 relative)."""
+import pytest
+
+from thoth_amd import abi, configs
+from thoth_amd.book import Barrier, BookSpec, BuiltRequest, Equity, McConfig, RainbowUdl, Sabr, Vanilla
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def _with(spec, paths, skip=0, precision=None):
+    spec.mcl.paths = paths
+    spec.mcl.sobol_skip = skip
+    if precision is not None:
+        spec.precision = precision
+    return spec
+
+
+@pytest.mark.parametrize("paths,skip", [(2, 0), (3, 0), (63, 0), (65, 0), (255, 0), (257, 0), (1000, 0),
+                                        (4097, 0), (3, 253), (1000, 17), (1000, 255), (700, 256), (5000, 1000)])
+def test_ragged_paths_black_scholes_fp64(gpu, oracle, paths, skip):
+    print("case: Black-Scholes FP64 paths=%d skip=%d" % (paths, skip), flush=True)
+    b = BuiltRequest(_with(configs.black_scholes_call(greeks=abi.GREEK_DELTA), paths, skip))
+    g, o = gpu.price(b), oracle.price(b)
+    assert rel(g.premium, o.premium) < REL, (g.premium, o.premium)
+    assert abs(g.delta - o.delta) < 1e-8
+
+
+@pytest.mark.parametrize("paths,skip", [(1000, 17), (333, 4000)])
+def test_ragged_paths_heston_barrier_fp64(gpu, oracle, paths, skip):
+    print("case: Heston barrier FP64 paths=%d skip=%d" % (paths, skip), flush=True)
+    b = BuiltRequest(_with(configs.heston_down_out_call(precision=abi.FP64, maturity_days=60), paths, skip))
+    g, o = gpu.price(b), oracle.price(b)
+    assert rel(g.premium, o.premium) < REL, (g.premium, o.premium)
+
+
+def _fp32_additivity(gpu, make, a, b, c):
+    """premium over [a, c) against those over [a, b) and [b, c), FP32 sweeps."""
+    p = {}
+    for lo, hi in ((a, b), (b, c), (a, c)):
+        print("case: FP32 paths=%d skip=%d" % (hi - lo, lo), flush=True)
+        p[lo, hi] = gpu.price(BuiltRequest(_with(make(), hi - lo, lo, abi.FP32))).premium
+    whole = p[a, c] * (c - a)
+    parts = p[a, b] * (b - a) + p[b, c] * (c - b)
+    assert abs(whole - parts) <= 1e-12 * abs(whole), (whole, parts)
+
+
+def test_fp32_ragged_additivity_lsv_basket(gpu):
+    # uniform-book LSV kernel (5 equities, depth-8 bridge stacks)
+    _fp32_additivity(gpu, lambda: configs.lsv_worst_of_basket(maturity_days=60), 7, 1030, 2500)
+
+
+def test_fp32_ragged_additivity_heston_barrier(gpu):
+    # uniform-book Heston kernel (1 equity, daily monitoring)
+    _fp32_additivity(gpu, lambda: configs.heston_down_out_call(maturity_days=90), 100, 357, 4097)
+
+
+def test_fp32_ragged_additivity_generic_book(gpu):
+    # generic kernel: mixed models, rainbow and continuous barrier
+    _fp32_additivity(gpu, lambda: _wide_book(3), 1, 513, 1300)
+
+
+def _wide_book(n, paths=1 << 11):
+    kinds = [abi.VOL_BS, abi.VOL_HESTON, abi.VOL_LSV]
+    eqs = []
+    for a in range(n):
+        k = kinds[a % 3]
+        if k == abi.VOL_BS:
+            eqs.append(Equity(spot=80 + 5 * a, volatility=18.0 + a))
+        elif k == abi.VOL_HESTON:
+            eqs.append(Equity(spot=60 + 7 * a, vol_kind=abi.VOL_HESTON, init_vol=22, long_vol=25, kappa=1.5 + 0.1 * a,
+                              vol_of_vol=0.5, rho_spot_var=-0.6))
+        else:
+            eqs.append(Equity(spot=90 + 3 * a, vol_kind=abi.VOL_LSV, init_vol=20, long_vol=24, kappa=2.0,
+                              vol_of_vol=0.4, rho_spot_var=-0.5,
+                              surface=Sabr([0.25, 1.0], [0.21, 0.23], [1, 1], [-0.4, -0.4], [0.5, 0.4])))
+    corr = [[1.0 if i == j else 0.25 for j in range(n)] for i in range(n)]
+    contracts = [Vanilla(a, 20, 100.0, is_call=a % 2 == 0) for a in range(n)]
+    contracts += [Vanilla(-1, 20, 100.0), Barrier(0, 20, 100.0, abi.UP_OUT, 120.0, discrete=False)]
+    return BookSpec(mcl=McConfig(paths=paths, max_day_step=1), equities=eqs, correlation=corr, contracts=contracts,
+                    rainbows=[RainbowUdl(list(range(n)), best=False)], rate_values=[2.0])
+
+
+@pytest.mark.parametrize("n", [7, 8])
+def test_widest_books_fp64(gpu, oracle, n):
+    # 7 equities run the 8-wide instantiation with one slot unused
+    print("case: %d-equity book FP64" % n, flush=True)
+    b = BuiltRequest(_wide_book(n))
+    g, o = gpu.price(b), oracle.price(b)
+    assert rel(g.premium, o.premium) < REL, (g.premium, o.premium)
+    for c in range(len(b.spec.contracts)):
+        assert rel(g.contract_premium[c], o.contract_premium[c]) < REL, c
+
+
+def test_maximum_path_count(gpu):
+    """2e8 paths, the largest count the engine accepts, in FP32 and FP64 on the
+    same Sobol points: FP32 within its per-premium bound of FP64."""
+    print("case: Black-Scholes 2e8 paths", flush=True)
+    r = {p: gpu.price(BuiltRequest(configs.black_scholes_call(paths=200_000_000, precision=p)))
+         for p in (abi.FP64, abi.FP32)}
+    assert abs(r[abi.FP32].premium - r[abi.FP64].premium) < 1e-5 * r[abi.FP64].premium
+    assert 9.0 < r[abi.FP64].premium < 10.0
+
+
+def _error(gpu, spec):
+    with pytest.raises(gpu.McError) as e:
+        gpu.price(BuiltRequest(spec))
+    return e.value.code
+
+
+def test_limits_and_invalid_requests_are_errors(gpu):
+    bs = configs.black_scholes_call
+    assert _error(gpu, bs(paths=1)) == abi.ERR_INVALID
+    assert _error(gpu, bs(paths=200_000_001)) == abi.ERR_INVALID
+    assert _error(gpu, _with(bs(), 1000, (1 << 32) - 100)) == abi.ERR_UNSUPPORTED  # Sobol index past 2^32 - 2
+    s = bs()
+    s.mcl.use_sobol = False
+    assert _error(gpu, s) == abi.ERR_UNSUPPORTED
+    s = bs()
+    s.contracts = [Vanilla(0, 365, 100.0 + k) for k in range(33)]
+    assert _error(gpu, s) == abi.ERR_UNSUPPORTED
+    s = bs()
+    s.contracts = []
+    assert _error(gpu, s) == abi.ERR_UNSUPPORTED
+    s = bs()
+    s.contracts = [Vanilla(0, 0, 100.0)]
+    assert _error(gpu, s) == abi.ERR_INVALID
+    s = _wide_book(3)
+    s.correlation = [[1, 0.9, -0.9], [0.9, 1, 0.9], [-0.9, 0.9, 1]]  # not positive definite
+    assert _error(gpu, s) == abi.ERR_INVALID
+    s = bs()
+    s.contracts = [Vanilla(1, 365, 100.0)]  # no equity 1
+    assert _error(gpu, s) == abi.ERR_INVALID
+    # the device is still usable after every refusal
+    assert gpu.price(BuiltRequest(bs(paths=1 << 10))).premium > 0

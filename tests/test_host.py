@@ -29,3 +29,34 @@ def test_surface_layout_is_checked():
     for bad in ((1, 50), (513, 50), (100, 0), (0, 50)):
         with pytest.raises(RuntimeError):
             mcl.calibrate_leverage(BuiltRequest(_lsv(surface=bad)), 0)
+
+
+def _mutated(change):
+    b = BuiltRequest(_lsv())
+    change(b.req)
+    return b
+
+
+@pytest.mark.parametrize("name,change", [
+    ("no equities", lambda r: setattr(r, "n_assets", 0)),
+    ("nine equities", lambda r: setattr(r, "n_assets", 9)),
+    ("rate pillars", lambda r: setattr(r.rate, "n", abi.MAX_PILLARS + 1)),
+    ("rate days", lambda r: (setattr(r.rate, "n", 2), r.rate.days.__setitem__(1, r.rate.days[0]))),
+    ("spot", lambda r: setattr(r.assets[0], "spot", 0.0)),
+    ("nan spot", lambda r: setattr(r.assets[0], "spot", float("nan"))),
+    ("vol model", lambda r: setattr(r.assets[0], "vol_kind", 7)),
+    ("surface pillars", lambda r: setattr(r.assets[0].surface, "n", abi.MAX_PILLARS + 1)),
+    ("lsv without surface", lambda r: setattr(r.assets[0].surface, "n", 0)),
+    ("surface maturities", lambda r: r.assets[0].surface.maturities.__setitem__(1, 0.5)),
+    ("rainbows", lambda r: setattr(r, "n_rainbows", -1)),
+])
+def test_request_shape_is_checked_before_use(name, change):
+    """Counts beyond the fixed arrays of thoth_mcl.h and pillars the
+    interpolations would divide by zero on are refused with THOTH_ERR_INVALID
+    by every entry point (here the host-only date grid and calibration)."""
+    b = _mutated(change)
+    with pytest.raises(mcl.McError) as e:
+        mcl.grid(b)
+    assert e.value.code == abi.ERR_INVALID, name
+    with pytest.raises(mcl.McError):
+        mcl.calibrate_leverage(b, 0)

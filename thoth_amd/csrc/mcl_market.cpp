@@ -229,8 +229,39 @@ double Market::LocalVol( int a, double K, double day ) const
     return std::sqrt( lv > 1e-8 ? lv : 1e-8 );
 }
 
+// Counts are checked against the fixed arrays of thoth_mcl.h before anything is
+// read, and the pillars against what the interpolations divide by.
+static void CheckRequestShape( const thoth_mcl_request& req )
+{
+    if ( !req.assets || req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS )
+        throw Invalid( THOTH_ERR_INVALID, "1..8 equities" );
+    if ( req.rate.n < 0 || req.rate.n > THOTH_MAX_PILLARS )
+        throw Invalid( THOTH_ERR_INVALID, "rate curve: 0..16 pillars" );
+    for ( int i = 1; i < req.rate.n; i++ )
+        if ( req.rate.days[i] <= req.rate.days[i - 1] )
+            throw Invalid( THOTH_ERR_INVALID, "rate curve days must increase" );
+    if ( req.n_rainbows < 0 || ( req.n_rainbows > 0 && !req.rainbows ) )
+        throw Invalid( THOTH_ERR_INVALID, "rainbow count" );
+    for ( int a = 0; a < req.n_assets; a++ )
+    {
+        const thoth_asset& in = req.assets[a];
+        if ( !( in.spot > 0 ) || !std::isfinite( in.spot ) )
+            throw Invalid( THOTH_ERR_INVALID, "spot must be positive" );
+        if ( in.vol_kind != THOTH_VOL_BS && in.vol_kind != THOTH_VOL_HESTON && in.vol_kind != THOTH_VOL_LSV )
+            throw Invalid( THOTH_ERR_INVALID, "unknown volatility model" );
+        if ( in.surface.n < 0 || in.surface.n > THOTH_MAX_PILLARS )
+            throw Invalid( THOTH_ERR_INVALID, "implied-vol surface: 0..16 pillars" );
+        if ( in.vol_kind == THOTH_VOL_LSV && in.surface.n < 1 )
+            throw Invalid( THOTH_ERR_INVALID, "an LSV equity needs its implied-vol surface" );
+        for ( int i = 1; i < in.surface.n; i++ )
+            if ( !( in.surface.maturities[i] > in.surface.maturities[i - 1] ) )
+                throw Invalid( THOTH_ERR_INVALID, "surface maturities must increase" );
+    }
+}
+
 Market BuildMarket( const thoth_mcl_request& req )
 {
+    CheckRequestShape( req );
     Market mk;
     for ( int a = 0; a < req.n_assets; a++ )
     {
