@@ -49,7 +49,8 @@ struct Offsets
 
 std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
 
-int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena& ar, int& need_depth )
+int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena& ar, int& need_depth,
+              std::vector<int>& step_ops )
 {
     const int nd = in.n_dates;
     const int na = in.n_assets;
@@ -140,6 +141,7 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
     of.bsq = ar.put( bsq );
     of.binv = ar.put( binv );
     of.step_ops = ar.put( bp.step_ops );
+    step_ops = bp.step_ops;
     of.op_b = ar.put( bp.op_b );
     of.op_left = ar.put( bp.op_left );
     of.op_w = ar.put( bp.op_w );
@@ -358,9 +360,33 @@ T* At( char* base, size_t off )
 
 }  // namespace
 
+// Slots of the sweep's Sobol ring for steps per barrier G: the ops of
+// bridge_step(0) and of the first group are staged together, then at the
+// start of each group those of the next one while the current group's are
+// read, so the ring holds any two consecutive groups.  A power of two.
+static int RingSlots( const std::vector<int>& so, int G )
+{
+    const int m = (int)so.size() - 1;
+    auto at = [&]( int i ) { return so[std::min( i, m )]; };
+    int need = at( G + 1 );
+    for ( int i = 1; i <= m; i += G )
+        need = std::max( need, at( i + 2 * G ) - at( i ) );
+    int S = 1;
+    while ( S < need )
+        S *= 2;
+    return S;
+}
+
 static size_t LdsBytes( const DeviceBatch& B, int G )
 {
-    const size_t words = B.lds_dims + (size_t)G * B.lds_lev_words;
+    // the pathwise kernel reads a Sobol block table; the sweep a block table
+    // (not in the in-loop inference kernels) and a ring
+    size_t ring = 0;
+    if ( !B.pathwise )
+        for ( const std::vector<int>& so : B.step_ops )
+            ring = std::max( ring, (size_t)RingSlots( so, G ) * B.nf_inst * 32 );
+    const size_t table = B.pathwise ? B.lds_dims : B.cv_mode == 2 ? 0 : ( B.lds_dims + 3 ) & ~size_t( 3 );
+    const size_t words = table + ring + (size_t)G * B.lds_lev_words;
     if ( !B.lds_tail )
         return words * sizeof( uint32_t );
     return ( ( words + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + B.lds_tail;
@@ -414,12 +440,13 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
 
     Arena ar;
     B.db.assign( n_books, DevBook{} );
+    B.step_ops.assign( n_books, {} );
     std::vector<Offsets> of( n_books );
     B.depth = 1;
     B.na_max = 1;
     for ( int b = 0; b < n_books; b++ )
     {
-        rc = PackBook( b, books[b], B.db[b], of[b], ar, B.depth );
+        rc = PackBook( b, books[b], B.db[b], of[b], ar, B.depth, B.step_ops[b] );
         if ( rc != THOTH_OK )
             return rc;
         if ( cv && cv->J > 0 )
@@ -462,10 +489,11 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
         B.kind = std::max( kind, 0 );
     }
-    // dynamic LDS: the Sobol block table of the widest book, plus (FP32 LSV) a
-    // ring of 2G leverage layers per equity of the kernel instantiation (G = 4,
-    // halved while two blocks per CU, the occupancy of the 5-asset kernels,
-    // would not fit the 160 KiB of LDS)
+    // dynamic LDS: the sweep's Sobol ring (the pathwise kernel: the Sobol block
+    // table of the widest book), plus (FP32 LSV) a ring of 2G leverage layers
+    // per equity of the kernel instantiation (G = 4, halved while two blocks
+    // per CU, the occupancy of the 5-asset kernels, would not fit the 160 KiB
+    // of LDS with the kernel's static LDS)
     const int na_inst = B.na_max <= 1 ? 1 : B.na_max <= 2 ? 2 : B.na_max <= 5 ? 5 : 8;
     size_t dims = 0, levw = 0;
     bool any_grid = false;
@@ -480,11 +508,15 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     B.lds_lev_words = levw;
     // tail areas, 16-byte aligned after the leverage ring (sweep_block's layout)
     B.lds_tail = B.cv_mode == 2 ? (size_t)kCvLdsBytes : any_grid ? (size_t)grid_lds_bytes( cfg->precision == 0 ? 8 : 4 ) : 0;
+    B.nf_inst = 2 * na_inst;
     int lev_group = 4;
-    while ( lev_group > 1 && 2 * LdsBytes( B, lev_group ) > (size_t)160 * 1024 )
+    while ( lev_group > 1 && 2 * ( LdsBytes( B, lev_group ) + kSweepStaticLds ) > (size_t)160 * 1024 )
         lev_group /= 2;
     for ( int b = 0; b < n_books; b++ )
+    {
         B.db[b].lev_group = lev_group;
+        B.db[b].sob_slots = RingSlots( B.step_ops[b], lev_group );
+    }
     B.lds_bytes = LdsBytes( B, lev_group );
     // grid partials [R][blocks][nm][2 nk][2] and their sums [nm][2 nk][2], per book
     B.grid_off.assign( n_books, 0 );
@@ -686,8 +718,11 @@ int SetLevGroup( DeviceBatch& B, int G )
 {
     if ( G < 1 || G > 16 )
         return SetError( THOTH_ERR_INVALID, "leverage group out of range" );
-    for ( DevBook& d : B.db )
-        d.lev_group = G;
+    for ( size_t b = 0; b < B.db.size(); b++ )
+    {
+        B.db[b].lev_group = G;
+        B.db[b].sob_slots = RingSlots( B.step_ops[b], G );
+    }
     B.lds_bytes = LdsBytes( B, G );
     hipStream_t s = Stream();
     HIP_OK( hipMemcpyAsync( const_cast<DevBook*>( B.dbooks ), B.db.data(), B.db.size() * sizeof( DevBook ),

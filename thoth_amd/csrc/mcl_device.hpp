@@ -1,6 +1,7 @@
 // mcl_device.hpp -- device-side book layout shared by the host launcher
 // (mcl_gpu.cpp) and the path-sweep kernels (mcl_sweep.hpp).
 #pragma once
+#include <cstddef>
 #include <cstdint>
 
 namespace thoth
@@ -11,7 +12,12 @@ constexpr int kMaxContracts = 32;
 constexpr int kMaxRainbows = 4;
 constexpr int kWavesPerBlock = 4;          // 256-thread blocks
 constexpr int kBlock = 64 * kWavesPerBlock;
+// static LDS of the sweep kernels (per-wave sums, persistent work item) with slack
+constexpr int kSweepStaticLds = kWavesPerBlock * ( kMaxContracts + 1 ) * 2 * 8 + 64;
 constexpr int kSobolDimsDev = 4097;       // implicit first dimension + 4096 Joe-Kuo rows
+// the device direction table [kSobolDimsDev][32] is followed by per-dimension
+// lane nibble tables [kSobolDimsDev][32] (see the Sobol ring of the sweep)
+constexpr size_t kSobolNibbles = (size_t)kSobolDimsDev * 32;
 
 // LDS of the in-loop hedge inference: BF16 A-operand image of the network
 // (40 tiles x 64 lanes x 8 bytes), FP32 biases (208), and per wave a
@@ -69,7 +75,8 @@ struct DevBook
     // grid
     int n_dates, m, n_assets, factors;
     int n_contracts, n_rainbows, corr_stoch, lev_points;
-    int lev_group;        // FP32 LSV: leverage layers staged in LDS per barrier (ring of 2 groups)
+    int lev_group;        // G: steps per LDS barrier (FP32 LSV leverage ring of 2G layers, Sobol ring)
+    int sob_slots;        // S: bridge ops held in the Sobol ring of the sweep (a power of two)
     int n_ops;
     double sd0;           // bridge endpoint std dev sqrt(t_m)
     float sd0f;

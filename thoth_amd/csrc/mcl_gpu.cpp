@@ -43,7 +43,7 @@ struct DeviceState
     bool probed = false;
     bool ok = false;
     std::string info;
-    uint32_t* d_dirs = nullptr;  // [kSobolDims][32]
+    uint32_t* d_dirs = nullptr;  // [kSobolDims][32] directions, then the lane nibble tables
     hipStream_t stream = nullptr;
 };
 DeviceState g_dev;
@@ -138,9 +138,29 @@ int EnsureDevice()
     }
     HIP_OK( hipSetDevice( 0 ) );
     HIP_OK( hipStreamCreate( &g_dev.stream ) );
+    // [dims][32] direction numbers, then [dims][32] lane nibble tables: words
+    // c < 16 XOR V[d][k] over the bits k < 4 of c, words 16 + c XOR V[d][4 + k]
     const std::vector<uint32_t>& V = SobolDirections();
-    HIP_OK( hipMalloc( &g_dev.d_dirs, V.size() * sizeof( uint32_t ) ) );
-    HIP_OK( hipMemcpy( g_dev.d_dirs, V.data(), V.size() * sizeof( uint32_t ), hipMemcpyHostToDevice ) );
+    const size_t dims = V.size() / 32;
+    if ( dims != (size_t)kSobolDimsDev )
+        return Fail( THOTH_ERR_DEVICE, "Sobol direction table size" );
+    std::vector<uint32_t> tables( V );
+    tables.resize( 2 * V.size() );
+    for ( size_t d = 0; d < dims; d++ )
+        for ( int c = 0; c < 16; c++ )
+        {
+            uint32_t lo = 0, hi = 0;
+            for ( int k = 0; k < 4; k++ )
+                if ( ( c >> k ) & 1 )
+                {
+                    lo ^= V[d * 32 + k];
+                    hi ^= V[d * 32 + 4 + k];
+                }
+            tables[kSobolNibbles + d * 32 + c] = lo;
+            tables[kSobolNibbles + d * 32 + 16 + c] = hi;
+        }
+    HIP_OK( hipMalloc( &g_dev.d_dirs, tables.size() * sizeof( uint32_t ) ) );
+    HIP_OK( hipMemcpy( g_dev.d_dirs, tables.data(), tables.size() * sizeof( uint32_t ), hipMemcpyHostToDevice ) );
     g_dev.ok = true;
     return THOTH_OK;
 }

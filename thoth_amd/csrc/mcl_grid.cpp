@@ -103,21 +103,18 @@ int TimeOnce( const GridPlan& P, double& ms )
     return rc;
 }
 
-// Autotuner: every (workgroups per CU, leverage layers per barrier) candidate
+// Autotuner: every (workgroups per CU, steps per LDS barrier) candidate
 // is launched twice on the plan's own books and the second time counts; the
 // fastest is kept.
 int Autotune( GridPlan& P )
 {
     const int bpc[] = { 1, 2, 4, 6, 8 };
     const int lg[] = { 1, 2, 4, 8 };
-    const bool has_lev = P.batch.lds_lev_words > 0;
     double best = 1e300;
     int best_b = P.blocks_per_cu, best_g = P.lev_group;
     P.candidates = 0;
     for ( int g : lg )
     {
-        if ( !has_lev && g != lg[0] )
-            continue;
         int rc = SetLevGroup( P.batch, g );
         if ( rc != THOTH_OK )
             return rc;

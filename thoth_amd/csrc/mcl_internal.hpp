@@ -62,6 +62,10 @@ struct DeviceBatch
     // dynamic-LDS recipe: Sobol table words, leverage words per staged layer,
     // fixed tail bytes (hedge-network or grid area)
     size_t lds_dims = 0, lds_lev_words = 0, lds_tail = 0;
+    // host copies of the books' bridge step offsets (Sobol ring sizing) and
+    // the factor stride NF = 2 NA of the kernel instantiation
+    std::vector<std::vector<int>> step_ops;
+    int nf_inst = 2;
     int64_t paths = 0;
     DeviceBatch() = default;
     DeviceBatch( const DeviceBatch& ) = delete;

@@ -211,11 +211,11 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
     {
         const int i = p + 1;
         bw.step( B, sob, p, nz );
-        const float* sc = B.sc + (size_t)i * SC_COUNT;
+        cptr<float> sc = as_const( B.sc ) + (size_t)i * SC_COUNT;
         R eps = nz[0];
         R cd;
         if constexpr ( F64 )
-            cd = B.drift[i] * B.t[i] - B.drift[i - 1] * B.t[i - 1];
+            cd = as_const( B.drift )[i] * as_const( B.t )[i] - as_const( B.drift )[i - 1] * as_const( B.t )[i - 1];
         else
             cd = sc[SC_CARRY];
         // ---- the step's mean and scale of log S, and their vega tangents
@@ -225,10 +225,10 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
         {
             if constexpr ( F64 )
             {
-                mu = cd - A.vol * A.vol / 2 * B.dt[i];
-                sig = A.vol * B.sqrt_dt[i];
-                dmu = -A.vol * B.dt[i];
-                dsig = B.sqrt_dt[i];
+                mu = cd - A.vol * A.vol / 2 * as_const( B.dt )[i];
+                sig = A.vol * as_const( B.sqrt_dt )[i];
+                dmu = -A.vol * as_const( B.dt )[i];
+                dsig = as_const( B.sqrt_dt )[i];
             }
             else
             {
@@ -244,8 +244,8 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
             R e, M0, S1, S0;
             if constexpr ( F64 )
             {
-                vc = qe_f64( v, B.dt[i], zv, A.kappa, A.theta, A.xi );
-                e = exp( -A.kappa * B.dt[i] );
+                vc = qe_f64( v, as_const( B.dt )[i], zv, A.kappa, A.theta, A.xi );
+                e = exp( -A.kappa * as_const( B.dt )[i] );
                 M0 = A.theta * ( 1 - e );
                 S1 = A.xi * A.xi * e / A.kappa * ( 1 - e );
                 S0 = A.theta * A.xi * A.xi / ( 2 * A.kappa ) * ( 1 - e ) * ( 1 - e );
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
             R K0, dK0;
             if constexpr ( F64 )
             {
-                const double dt = B.dt[i];
+                const double dt = as_const( B.dt )[i];
                 if ( A.xi > 1e-12 )
                 {
                     const double aa = 0.5 * dt * ( A.kappa * A.rho / A.xi - 0.5 );
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
             dsig = sig > R( 0 ) ? ( degenerate ? K3 * dv : K3 * ( dv + dvc ) ) / ( R( 2 ) * sig ) : R( 0 );
         }
         // ---- one-step survival at a monitored date of the knock-out barrier
-        const bool monitored = barrier && ( B.mon_mask[i] & 1u ) && sig > R( 0 );
+        const bool monitored = barrier && ( as_const( B.mon_mask )[i] & 1u ) && sig > R( 0 );
         if ( monitored )
         {
             const R x = log( S );
@@ -342,10 +342,10 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
         {
             if constexpr ( F64 )
             {
-                const double w1 = Wb + B.sqrt_dt[i] * eps;
+                const double w1 = Wb + as_const( B.sqrt_dt )[i] * eps;
                 const double dw = w1 - Wb;
                 Wb = w1;
-                const double expo = cd - A.vol * A.vol / 2 * B.dt[i] + A.vol * dw;
+                const double expo = cd - A.vol * A.vol / 2 * as_const( B.dt )[i] + A.vol * dw;
                 S = ( S - 0 ) * exp( expo ) + 0;
             }
             else
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
                     S = S * exp( mu + sqrt( var ) * eps );
                 }
                 else
-                    S = S * exp( cd - 0.5 * v * B.dt[i] + sqrt( v * B.dt[i] ) * eps );
+                    S = S * exp( cd - 0.5 * v * as_const( B.dt )[i] + sqrt( v * as_const( B.dt )[i] ) * eps );
             }
             else
                 S = S * __expf( mu + sig * eps );
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
             dv = dvc;
         }
         // ---- the contract's flow
-        if ( B.flow_mask[i] & 1u )
+        if ( as_const( B.flow_mask )[i] & 1u )
         {
             const R f0 = call ? S - K : K - S;
             const R f = f0 > R( 0 ) ? f0 : R( 0 );

@@ -22,6 +22,19 @@ namespace thoth
 
 __device__ __forceinline__ uint32_t gray32( uint32_t n ) { return n ^ ( n >> 1 ); }
 
+// Tables of a book that no kernel writes while the sweep runs (bridge program,
+// step constants, leverage scales, payoff masks) are read through the
+// constant address space: at wave-uniform indices those loads become scalar
+// loads into SGPRs (scalar cache) instead of flat vector loads into VGPRs
+// that wait on vector memory.
+template <class T>
+using cptr = const __attribute__( ( address_space( 4 ) ) ) T*;
+template <class T>
+__device__ __forceinline__ cptr<T> as_const( const T* p )
+{
+    return (cptr<T>)p;
+}
+
 // FP32 hardware transcendentals of the production path (v_rcp_f32, v_log_f32:
 // about 1 ulp on the normal range, which is all these arguments take)
 __device__ __forceinline__ float fast_rcp( float x ) { return __builtin_amdgcn_rcpf( x ); }
@@ -279,7 +292,8 @@ struct QeMoments
     float m, psi;
 };
 
-__device__ __forceinline__ QeMoments qe_moments( float v, const float* __restrict__ sc )
+template <class P>
+__device__ __forceinline__ QeMoments qe_moments( float v, P sc )
 {
     const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
     const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
@@ -312,7 +326,8 @@ __device__ __forceinline__ float qe_exponential( const QeMoments& q, float z, Qe
                : fast_log( fast_div( one_minus_p, one_minus_u ) ) * q.m * 0.5f * ( q.psi + 1.0f );
 }
 
-__device__ __forceinline__ float qe_f32( float v, float z, const float* __restrict__ sc, QeDraw* draw = nullptr )
+template <class P>
+__device__ __forceinline__ float qe_f32( float v, float z, P sc, QeDraw* draw = nullptr )
 {
     const QeMoments q = qe_moments( v, sc );
     if ( q.psi <= 1.5f )
@@ -502,18 +517,25 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     const int nd = B.n_dates;
     const int nc = B.n_contracts;
 
-    // dynamic LDS: [m*F Sobol block table][FP32 LSV: 2 x NA x lev_points leverage
-    // layers][in-loop hedge inference area, 16-byte aligned]
+    // dynamic LDS: [Sobol block table: m F words padded to 4, except in the
+    // in-loop inference kernels][Sobol ring: nslots x NF factors x 32 words]
+    // [FP32 LSV: 2G x NA x lev_points leverage layers][in-loop hedge inference
+    // or grid area, 16-byte aligned]
+    constexpr bool block_table = CV != 2;
     extern __shared__ uint32_t lds[];
+    const int nslots = B.sob_slots;
+    const int tab_words = block_table ? ( m * F + 3 ) & ~3 : 0;
+    const int ring_words = tab_words + nslots * NF * 32;  // through the end of the ring
     uint32_t* tab = lds;
-    float* levs = reinterpret_cast<float*>( lds + m * F );
+    uint32_t* ring = lds + tab_words;
+    float* levs = reinterpret_cast<float*>( lds + ring_words );
     // vanilla-grid area (never together with the control variate)
     [[maybe_unused]] char* gl = nullptr;
     if constexpr ( CV == 0 )
         if ( B.grid_nm > 0 )
         {
             const int lev_words = ( !F64 && B.lev_points > 0 ) ? 2 * B.lev_group * NA * B.lev_points : 0;
-            gl = reinterpret_cast<char*>( lds + ( ( m * F + lev_words + 3 ) & ~3 ) );
+            gl = reinterpret_cast<char*>( lds + ( ( ring_words + lev_words + 3 ) & ~3 ) );
         }
     constexpr bool cv_on = CV != 0 && !F64;
     constexpr bool cv_samples = CV == 1 && !F64;
@@ -522,7 +544,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     if constexpr ( cv_infer )
     {
         const int lev_words = B.lev_points > 0 ? 2 * B.lev_group * NA * B.lev_points : 0;
-        cvl = reinterpret_cast<char*>( lds + ( ( m * F + lev_words + 3 ) & ~3 ) );
+        cvl = reinterpret_cast<char*>( lds + ( ( ring_words + lev_words + 3 ) & ~3 ) );
         const uint32_t* src = B.cv_wimg;
         uint32_t* dst = reinterpret_cast<uint32_t*>( cvl );
         for ( int k = threadIdx.x; k < kCvImgBytes / 4; k += kBlock )
@@ -534,11 +556,97 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     __shared__ double acc[kWavesPerBlock][kMaxContracts + 1][2];
     for ( int k = threadIdx.x; k < kWavesPerBlock * ( kMaxContracts + 1 ) * 2; k += kBlock )
         ( &acc[0][0][0] )[k] = 0.0;
-    sobol_block_table( tab, V, m * F, bq, sh );
-    SobolLane sob;
-    sob.V = V;
-    sob.tab = tab;
-    sob.set_lane( wib, lane );
+    // ---- Sobol points.  Point n = 256 bq + l (l = threadIdx.x) of dimension d
+    // is shift[d] ^ XOR of V[d][k] over the set bits k of gray(n), and
+    // gray(n) = gray(l) ^ gray(256 bq) since gray code is linear in XOR.  The
+    // block part (shift and gray(256 bq)) is tabulated per dimension at the
+    // start, or, in the in-loop inference kernels whose LDS holds the network,
+    // computed where it is needed.  The lane part splits into the nibbles of
+    // the 8-bit gray(l), whose 16-entry tables per dimension
+    //   lo[c] = XOR_{k < 4, bit k of c} V[d][k],  hi[c] = XOR_{k < 4, bit k of c} V[d][4 + k]
+    // follow the direction numbers in device memory.  For every bridge op o
+    // (build index b: dimensions b F + f) the block copies, per factor, lo and
+    // hi ^ (block part) into an LDS ring (op o in slot o mod nslots), and lane
+    // l reads lo[gray(l) & 15] ^ hi[gray(l) >> 4]: two LDS reads and one XOR
+    // per normal.  The copies of a group of G steps are made one group ahead
+    // and published by the barrier that starts the group (the leverage
+    // barrier).
+    typedef uint32_t u32x4 __attribute__( ( ext_vector_type( 4 ) ) );
+    const uint32_t* nib = V + kSobolNibbles;
+    const uint32_t gbq = gray32( bq * 256u );
+    const uint32_t gln = gray32( (uint32_t)threadIdx.x );
+    const uint32_t* rlo = ring + ( gln & 15u );
+    const uint32_t* rhi = ring + 16 + ( gln >> 4 );
+    auto stage_ops = [&]( int o0, int o1 ) __attribute__( ( always_inline ) )
+    {
+        // four adjacent lanes per (op, factor), lane q of the four owning
+        // words 8q .. 8q+7 of the direction numbers and of the nibble tables
+        // (n and kBlock are multiples of four: the four lanes run together)
+        const int n = ( o1 - o0 ) * NF * 4;
+        for ( int k = threadIdx.x; k < n; k += kBlock )
+        {
+            const int q = k & 3;
+            const int pair = k >> 2;  // (o - o0) NF + f
+            const int o = o0 + pair / NF;
+            const int f = pair - ( pair / NF ) * NF;
+            if ( f >= F )
+                continue;
+            const int d = B.op_b[o] * F + f;
+            uint32_t blk;
+            if constexpr ( block_table )
+                blk = tab[d];
+            else
+            {
+                // XOR of V[d][k] over the bits k of gray(256 bq), eight words
+                // per lane, then across the quad (DPP quad permutations
+                // [1 0 3 2] and [2 3 0 1]), and the shift
+                const u32x4* vr = reinterpret_cast<const u32x4*>( V + (size_t)d * 32 + 8 * q );
+                const u32x4 v0 = vr[0], v1 = vr[1];
+                const uint32_t bits = gbq >> ( 8 * q );
+                blk = 0;
+#pragma unroll
+                for ( int j = 0; j < 4; j++ )
+                {
+                    blk ^= v0[j] & ( 0u - ( ( bits >> j ) & 1u ) );
+                    blk ^= v1[j] & ( 0u - ( ( bits >> ( 4 + j ) ) & 1u ) );
+                }
+                blk ^= (uint32_t)__builtin_amdgcn_mov_dpp( (int)blk, 0xB1, 0xF, 0xF, true );
+                blk ^= (uint32_t)__builtin_amdgcn_mov_dpp( (int)blk, 0x4E, 0xF, 0xF, true );
+                blk ^= sh ? sh[d] : 0u;
+            }
+            const u32x4* src = reinterpret_cast<const u32x4*>( nib + (size_t)d * 32 + 8 * q );
+            u32x4 x0 = src[0], x1 = src[1];
+            if ( q >= 2 )
+            {
+                x0 ^= blk;
+                x1 ^= blk;
+            }
+            u32x4* dst = reinterpret_cast<u32x4*>( ring + ( ( o & ( nslots - 1 ) ) * NF + f ) * 32 + 8 * q );
+            dst[0] = x0;
+            dst[1] = x1;
+        }
+    };
+    // the point of op o, factor f (o wave-uniform)
+    auto sob_op = [&]( int o, int f ) __attribute__( ( always_inline ) ) -> uint32_t
+    {
+        const int base = ( ( o & ( nslots - 1 ) ) * NF + f ) * 32;
+        return rlo[base] ^ rhi[base];
+    };
+    // a whole point, for the bridge endpoint (build index 0, dimensions 0 .. F-1)
+    auto sob_point = [&]( int d ) __attribute__( ( always_inline ) ) -> uint32_t
+    {
+        if constexpr ( block_table )
+        {
+            const uint32_t* t = nib + (size_t)d * 32;
+            return tab[d] ^ t[gln & 15u] ^ t[16 + ( gln >> 4 )];
+        }
+        const uint32_t* v = V + (size_t)d * 32;
+        uint32_t x = sh ? sh[d] : 0u;
+        for ( uint32_t g = gbq ^ gln; g; g &= g - 1 )
+            x ^= v[__builtin_ctz( g )];
+        return x;
+    };
+    auto so = [&]( int i ) __attribute__( ( always_inline ) ) { return B.step_ops[min( i, m )]; };
     // FP32 LSV books stage the leverage layers in LDS, G layers at a time into
     // a ring of 2G slots (layer i in slot (i - 1) mod 2G): one barrier per G
     // steps publishes the current group and frees the previous group's slots
@@ -559,6 +667,13 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             }
         }
     };
+    if constexpr ( block_table )
+    {
+        sobol_block_table( tab, V, m * F, bq, sh );
+        __syncthreads();
+    }
+    // the ops of bridge_step(0) and of the first group's steps
+    stage_ops( 0, so( G + 1 ) );
     if ( lev_lds && m >= 1 )
         stage_group( 1 );
     __syncthreads();
@@ -606,7 +721,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         top[f] = R( 0 );
         if ( f < F )
         {
-            const R z = normal_of<R>( sob( f ) );
+            const R z = normal_of<R>( sob_point( f ) );
             if constexpr ( F64 )
                 top[f] = B.sd0 * z;
             else
@@ -713,11 +828,11 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     {
         if constexpr ( cv_infer )
         {
-            if ( B.cv_start[i] >= 0 )
+            if ( as_const( B.cv_start )[i] >= 0 )
                 hedge_eval( i );
             return;
         }
-        const int je = B.cv_end[i];
+        const int je = as_const( B.cv_end )[i];
         if ( je >= 0 )
         {
             if ( cv_store )
@@ -735,7 +850,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             for ( int a = 0; a < NA; a++ )
                 dmacc[a] = 0.0f;
         }
-        const int js = B.cv_start[i];
+        const int js = as_const( B.cv_start )[i];
         if ( js >= 0 && cv_store )
             cv_features( i, js );
     };
@@ -758,7 +873,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 
     auto observe = [&]( int i ) __attribute__( ( always_inline ) )
     {
-        uint32_t mon = B.mon_mask[i];
+        uint32_t mon = as_const( B.mon_mask )[i];
         while ( mon )
         {
             const int c = __builtin_ctz( mon );
@@ -769,7 +884,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             if ( h )
                 hit |= 1u << c;
         }
-        uint32_t fl = B.flow_mask[i];
+        uint32_t fl = as_const( B.flow_mask )[i];
         while ( fl )
         {
             const int c = __builtin_ctz( fl );
@@ -874,7 +989,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     // the normals of build index bidx for every factor.  Uniform FP32
     // kernels evaluate the central polynomial for all factors straight-line
     // and redo the lanes in the tail (w >= 8) in one block afterwards.
-    auto normals = [&]( int bidx, R* zf ) __attribute__( ( always_inline ) )
+    auto normals = [&]( int o, R* zf ) __attribute__( ( always_inline ) )
     {
         if constexpr ( uniform )
         {
@@ -883,7 +998,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             for ( int f = 0; f < NF; f++ )
             {
                 bool t;
-                zf[f] = normal_f32_central( sob( bidx * NF + f ), t );
+                zf[f] = normal_f32_central( sob_op( o, f ), t );
                 tails |= t ? 1u << f : 0u;
             }
             if ( __builtin_amdgcn_ballot_w64( tails != 0 ) )
@@ -891,7 +1006,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 #pragma unroll
                 for ( int f = 0; f < NF; f++ )
                     if ( ( tails >> f ) & 1u )
-                        zf[f] = normal_f32( sob( bidx * NF + f ) );
+                        zf[f] = normal_f32( sob_op( o, f ) );
             }
         }
         else
@@ -899,7 +1014,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
                 if ( f < F )
-                    zf[f] = normal_of<R>( sob( bidx * F + f ) );
+                    zf[f] = normal_of<R>( sob_op( o, f ) );
         }
     };
     auto bridge_step = [&]( int p, facv& nz ) __attribute__( ( always_inline ) )
@@ -912,7 +1027,6 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             // straight back, so the new point is consumed without touching the
             // stack (same arithmetic as the general walk below)
             const int o = o0;
-            const int bidx = B.op_b[o];
             const bool left = B.op_left[o] != 0;
             R lw, rw, sd;
             if constexpr ( F64 )
@@ -931,7 +1045,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             for ( int f = 0; f < NFV; f++ )
                 nz[f] = R( 0 );
             R zf[NF];
-            normals( bidx, zf );
+            normals( o, zf );
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
             {
@@ -950,7 +1064,6 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
         for ( int o = o0; o < o1; o++ )
         {
-            const int bidx = B.op_b[o];
             const bool left = B.op_left[o] != 0;
             const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
             R lw, rw, sd;
@@ -967,7 +1080,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 sd = B.op_wf[3 * o + 2];
             }
             R zf[NF];
-            normals( bidx, zf );
+            normals( o, zf );
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
             {
@@ -1013,11 +1126,14 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     for ( int p = 0; p < m; p++ )
     {
         const int i = p + 1;
-        if ( lev_lds && ( i - 1 ) % G == 0 )
+        if ( ( i - 1 ) % G == 0 )
         {
-            __syncthreads();  // the group of layer i is visible, the previous group is free
-            if ( i + G <= m )
+            // the group's leverage layers and Sobol tables are visible, the
+            // previous group's slots are free: stage the next group
+            __syncthreads();
+            if ( lev_lds && i + G <= m )
                 stage_group( i + G );
+            stage_ops( so( i + G ), so( i + 2 * G ) );
         }
         // ---- uniform FP32 kernels: the QE variance draws of every equity,
         // quadratic branch straight-line, the exponential-branch lanes redone
@@ -1234,8 +1350,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
         observe( i );
         if constexpr ( CV == 0 )
-            if ( B.grid_nm > 0 && B.grid_mat_at[i] >= 0 )
-                grid_observe( B.grid_mat_at[i] );
+            if ( B.grid_nm > 0 && as_const( B.grid_mat_at )[i] >= 0 )
+                grid_observe( as_const( B.grid_mat_at )[i] );
         if constexpr ( cv_on )
             cv_date( i );
         if ( i < m )
