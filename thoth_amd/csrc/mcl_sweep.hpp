@@ -287,9 +287,13 @@ struct QeDraw
 // lane and the exponential one for the lanes that need it (same arithmetic
 // as taking the branch).  1-u is formed directly as Phi(-z) so the
 // exponential tail keeps its precision.
+// psi = s2 / m^2 is formed only where the exponential branch needs it: the
+// branch test is s2 <= 1.5 m^2 and the quadratic branch reads 2 / psi as
+// 2 m^2 / s2 (one reciprocal instead of two).
 struct QeMoments
 {
-    float m, psi;
+    float m, m2, s2;
+    bool quad;  // psi <= 1.5 (psi = 0 when m <= 0)
 };
 
 template <class P>
@@ -297,13 +301,14 @@ __device__ __forceinline__ QeMoments qe_moments( float v, P sc )
 {
     const float m = fmaf( v, sc[SC_E], sc[SC_M0] );
     const float s2 = fmaf( v, sc[SC_S1], sc[SC_S0] );
-    return { m, ( m > 0.0f ) ? fast_div( s2, m * m ) : 0.0f };
+    const float m2 = m * m;
+    return { m, m2, s2, !( m > 0.0f ) || s2 <= 1.5f * m2 };
 }
 
 // psi <= 1.5: v' = a (sqrt(b2) + z)^2
 __device__ __forceinline__ float qe_quadratic( const QeMoments& q, float z, QeDraw* draw )
 {
-    const float c = fast_div( 2.0f, q.psi );
+    const float c = 2.0f * q.m2 * fast_rcp( q.s2 );
     const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
     const float a = fast_div( q.m, 1.0f + b2 );
     const float sb = __builtin_amdgcn_sqrtf( b2 );
@@ -316,21 +321,22 @@ __device__ __forceinline__ float qe_quadratic( const QeMoments& q, float z, QeDr
 // psi > 1.5: p = (psi-1)/(psi+1), beta = (1-p)/m; v' = log((1-p)/(1-u)) / beta
 __device__ __forceinline__ float qe_exponential( const QeMoments& q, float z, QeDraw* draw )
 {
-    const float inv = fast_rcp( q.psi + 1.0f );
+    const float psi = q.s2 * fast_rcp( q.m2 );
+    const float inv = fast_rcp( psi + 1.0f );
     const float one_minus_p = 2.0f * inv;
     if ( draw )
-        draw->mean = one_minus_p * q.m * 0.5f * ( q.psi + 1.0f );  // (1 - p) / beta
+        draw->mean = one_minus_p * q.m * 0.5f * ( psi + 1.0f );  // (1 - p) / beta
     const float one_minus_u = 0.5f * erfc_fast( z * 0.70710678118654752f );
     return ( one_minus_u >= one_minus_p )
                ? 0.0f
-               : fast_log( fast_div( one_minus_p, one_minus_u ) ) * q.m * 0.5f * ( q.psi + 1.0f );
+               : fast_log( fast_div( one_minus_p, one_minus_u ) ) * q.m * 0.5f * ( psi + 1.0f );
 }
 
 template <class P>
 __device__ __forceinline__ float qe_f32( float v, float z, P sc, QeDraw* draw = nullptr )
 {
     const QeMoments q = qe_moments( v, sc );
-    if ( q.psi <= 1.5f )
+    if ( q.quad )
         return qe_quadratic( q, z, draw );
     return qe_exponential( q, z, draw );
 }
@@ -730,7 +736,10 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     }
     int sp = 1;
 
-    // ---- equity state
+    // ---- equity state.  The uniform LSV kernel keeps log S in S (the leverage
+    // is read at log S; the spot is exponentiated only where a payoff or the
+    // control variate reads it); every other kernel keeps S.
+    constexpr bool log_spot = KIND == 1;
     assetv S, Var, W;
 #pragma unroll
     for ( int a = 0; a < NAV; a++ )
@@ -743,9 +752,16 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     for ( int a = 0; a < NA; a++ )
         if ( a < na )
         {
-            S[a] = R( B.asset[a].spot );
+            S[a] = log_spot ? R( log( B.asset[a].spot ) ) : R( B.asset[a].spot );
             Var[a] = R( B.asset[a].v0 );
         }
+    auto spot_of = [&]( int a ) __attribute__( ( always_inline ) ) -> R
+    {
+        if constexpr ( log_spot )
+            return __expf( S[a] );
+        else
+            return S[a];
+    };
     uint32_t hit = 0;
     double book = 0.0;
 
@@ -773,7 +789,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         for ( int a = 0; a < NA; a++ )
             if ( a < na && a < 5 )
             {
-                f[a] = ( fast_log( (float)S[a] ) - B.cv_lnS0[a] ) * B.cv_ln_scale[a];
+                f[a] = ( ( log_spot ? (float)S[a] : fast_log( (float)S[a] ) ) - B.cv_lnS0[a] ) * B.cv_ln_scale[a];
                 if ( uniform || B.asset[a].model != 0 )
                     f[5 + a] = 2.0f * fmaf( __builtin_amdgcn_sqrtf( fmaxf( (float)Var[a], 0.0f ) ),
                                             B.cv_vol_scale[a], -1.0f );
@@ -858,14 +874,14 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     auto udl_value = [&]( int u ) __attribute__( ( always_inline ) ) -> R
     {
         if ( u >= 0 )
-            return S[__builtin_amdgcn_readfirstlane( u )];
+            return spot_of( __builtin_amdgcn_readfirstlane( u ) );
         const int r = -1 - u;  // rainbow_node.cpp:14-27
         const bool best = B.rb_best[r] != 0;
         R x = best ? R( -INFINITY ) : R( INFINITY );
         const int n = B.rb_n[r];
         for ( int j = 0; j < n; j++ )
         {
-            const R perf = S[__builtin_amdgcn_readfirstlane( B.rb_member[r][j] )] * R( B.rb_weight[r][j] );
+            const R perf = spot_of( __builtin_amdgcn_readfirstlane( B.rb_member[r][j] ) ) * R( B.rb_weight[r][j] );
             x = best ? ( x < perf ? perf : x ) : ( perf < x ? perf : x );
         }
         return x;
@@ -1150,7 +1166,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                 const QeMoments q = qe_moments( Var[a], sc );
                 vq[a] = qe_quadratic( q, nz[vf], cv_on ? &dq[a] : nullptr );
-                expo |= q.psi <= 1.5f ? 0u : 1u << a;
+                expo |= q.quad ? 0u : 1u << a;
             }
             if ( __builtin_amdgcn_ballot_w64( expo != 0 ) )
             {
@@ -1256,7 +1272,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 if constexpr ( F64 )
                     f = log( S[a] ) / B.lev_ln_step[li] - B.lev_offset[li];
                 else
-                    f = fast_log( S[a] ) * B.lev_inv_step[(size_t)i * nas + a] - B.lev_offset_f[(size_t)i * nas + a];
+                    f = ( log_spot ? S[a] : fast_log( S[a] ) ) * B.lev_inv_step[(size_t)i * nas + a] -
+                        B.lev_offset_f[(size_t)i * nas + a];
                 if constexpr ( F64 )
                 {
                     const double* lv = B.lev + li * B.lev_points;
@@ -1324,8 +1341,17 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     bvol = L * __builtin_amdgcn_sqrtf( vp * sc[SC_DT] );
                 }
                 step = det + bvol * z;
-                const float sold = S[a];
-                S[a] = sold * __expf( step );
+                float sold;
+                if constexpr ( log_spot )
+                {
+                    sold = cv_on ? __expf( S[a] ) : 0.0f;
+                    S[a] += step;
+                }
+                else
+                {
+                    sold = S[a];
+                    S[a] = sold * __expf( step );
+                }
                 if constexpr ( cv_on )
                 {
                     // first-order Doob increment of the spot.  Given v_i, the step
