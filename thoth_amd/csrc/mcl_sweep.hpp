@@ -429,7 +429,7 @@ struct BridgeWalk
         for ( int o = o0; o < o1; o++ )
         {
             const int bidx = B.op_b[o];
-            const bool left = B.op_left[o] != 0;
+            const bool left = as_const( B.op_left )[o] != 0;
             const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
             R lw, rw, sd;
             if constexpr ( F64 )
@@ -440,9 +440,9 @@ struct BridgeWalk
             }
             else
             {
-                lw = B.op_wf[3 * o];
-                rw = B.op_wf[3 * o + 1];
-                sd = B.op_wf[3 * o + 2];
+                lw = as_const( B.op_wf )[3 * o];
+                rw = as_const( B.op_wf )[3 * o + 1];
+                sd = as_const( B.op_wf )[3 * o + 2];
             }
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
@@ -468,7 +468,7 @@ struct BridgeWalk
                 if constexpr ( F64 )
                     nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
                 else
-                    nz[f] = ( w - prev[f] ) * B.binv[i];
+                    nz[f] = ( w - prev[f] ) * as_const( B.binv )[i];
                 prev[f] = w;
                 if ( sp >= 2 )
                     top[f] = stk_get( f, pos );
@@ -505,8 +505,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     const int nas = uniform ? NA : B.n_assets;  // row length of the date-major per-step arrays
     static_assert( KIND == 0 || !F64, "uniform-book kernels are FP32" );
     constexpr int NF = 2 * NA;
-    constexpr int NFV = NF <= 2 ? 2 : NF <= 4 ? 4 : NF <= 8 ? 8 : 16;
-    constexpr int NAV = NA <= 1 ? 1 : NA <= 2 ? 2 : NA <= 4 ? 4 : 8;
+    constexpr int NFV = NF;
+    constexpr int NAV = NA;
     typedef R stackv __attribute__( ( ext_vector_type( DEPTH ) ) );
     typedef R facv __attribute__( ( ext_vector_type( NFV ) ) );
     typedef R assetv __attribute__( ( ext_vector_type( NAV ) ) );
@@ -1043,7 +1043,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             // straight back, so the new point is consumed without touching the
             // stack (same arithmetic as the general walk below)
             const int o = o0;
-            const bool left = B.op_left[o] != 0;
+            const bool left = as_const( B.op_left )[o] != 0;
             R lw, rw, sd;
             if constexpr ( F64 )
             {
@@ -1053,9 +1053,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             }
             else
             {
-                lw = B.op_wf[3 * o];
-                rw = B.op_wf[3 * o + 1];
-                sd = B.op_wf[3 * o + 2];
+                lw = as_const( B.op_wf )[3 * o];
+                rw = as_const( B.op_wf )[3 * o + 1];
+                sd = as_const( B.op_wf )[3 * o + 2];
             }
 #pragma unroll
             for ( int f = 0; f < NFV; f++ )
@@ -1072,7 +1072,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     if constexpr ( F64 )
                         nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
                     else
-                        nz[f] = ( w - prev[f] ) * B.binv[i];
+                        nz[f] = ( w - prev[f] ) * as_const( B.binv )[i];
                     prev[f] = w;
                 }
             }
@@ -1080,7 +1080,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
         for ( int o = o0; o < o1; o++ )
         {
-            const bool left = B.op_left[o] != 0;
+            const bool left = as_const( B.op_left )[o] != 0;
             const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
             R lw, rw, sd;
             if constexpr ( F64 )
@@ -1091,9 +1091,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             }
             else
             {
-                lw = B.op_wf[3 * o];
-                rw = B.op_wf[3 * o + 1];
-                sd = B.op_wf[3 * o + 2];
+                lw = as_const( B.op_wf )[3 * o];
+                rw = as_const( B.op_wf )[3 * o + 1];
+                sd = as_const( B.op_wf )[3 * o + 2];
             }
             R zf[NF];
             normals( o, zf );
@@ -1126,7 +1126,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     if constexpr ( F64 )
                         nz[f] = ( B.bsq[i] > 0 ) ? ( w - prev[f] ) / B.bsq[i] : 0.0;
                     else
-                        nz[f] = ( w - prev[f] ) * B.binv[i];
+                        nz[f] = ( w - prev[f] ) * as_const( B.binv )[i];
                     prev[f] = w;
                     if ( sp >= 2 )
                         top[f] = stk_get( f, pos );
@@ -1162,7 +1162,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 #pragma unroll
             for ( int a = 0; a < NA; a++ )
             {
-                const float* sc = B.sc + ( (size_t)i * nas + a ) * SC_COUNT;
+                cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
                 const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                 const QeMoments q = qe_moments( Var[a], sc );
                 vq[a] = qe_quadratic( q, nz[vf], cv_on ? &dq[a] : nullptr );
@@ -1174,7 +1174,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 for ( int a = 0; a < NA; a++ )
                     if ( ( expo >> a ) & 1u )
                     {
-                        const float* sc = B.sc + ( (size_t)i * nas + a ) * SC_COUNT;
+                        cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
                         const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                         vq[a] = qe_exponential( qe_moments( Var[a], sc ), nz[vf], cv_on ? &dq[a] : nullptr );
                     }
@@ -1205,7 +1205,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                         eps += nz[v] * B.cholf[a * kMaxAssets + v];
                 }
             }
-            const float* sc = B.sc + ( (size_t)i * nas + a ) * SC_COUNT;
+            cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
             R cd;
             if constexpr ( F64 )
                 cd = B.drift[(size_t)a * nd + i] * B.t[i] - B.drift[(size_t)a * nd + i - 1] * B.t[i - 1];
@@ -1236,9 +1236,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                         // first-order Doob increment S_{i-1} vol dW: zero conditional mean
                         const float M = sold * sc[SC_VOL] * dw;
                         if constexpr ( cv_infer )
-                            cvacc = fmaf( hr[a] * B.cv_scale[(size_t)i * nas + a], M, cvacc );
+                            cvacc = fmaf( hr[a] * as_const( B.cv_scale )[(size_t)i * nas + a], M, cvacc );
                         else
-                            dmacc[a] = fmaf( B.cv_scale[(size_t)i * nas + a], M, dmacc[a] );
+                            dmacc[a] = fmaf( as_const( B.cv_scale )[(size_t)i * nas + a], M, dmacc[a] );
                     }
                 }
                 continue;
@@ -1272,8 +1272,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 if constexpr ( F64 )
                     f = log( S[a] ) / B.lev_ln_step[li] - B.lev_offset[li];
                 else
-                    f = ( log_spot ? S[a] : fast_log( S[a] ) ) * B.lev_inv_step[(size_t)i * nas + a] -
-                        B.lev_offset_f[(size_t)i * nas + a];
+                    f = ( log_spot ? S[a] : fast_log( S[a] ) ) * as_const( B.lev_inv_step )[(size_t)i * nas + a] -
+                        as_const( B.lev_offset_f )[(size_t)i * nas + a];
                 if constexpr ( F64 )
                 {
                     const double* lv = B.lev + li * B.lev_points;
@@ -1368,9 +1368,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     }
                     M *= sold;
                     if constexpr ( cv_infer )
-                        cvacc = fmaf( hr[a] * B.cv_scale[(size_t)i * nas + a], M, cvacc );
+                        cvacc = fmaf( hr[a] * as_const( B.cv_scale )[(size_t)i * nas + a], M, cvacc );
                     else
-                        dmacc[a] = fmaf( B.cv_scale[(size_t)i * nas + a], M, dmacc[a] );
+                        dmacc[a] = fmaf( as_const( B.cv_scale )[(size_t)i * nas + a], M, dmacc[a] );
                 }
             }
         }
