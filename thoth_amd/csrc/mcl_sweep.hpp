@@ -177,6 +177,31 @@ __device__ __forceinline__ double normal_f64( uint32_t x )
 // precision in both tails; one log per normal.  The central polynomial
 // (degree 10) reaches w = 8, so the tail polynomial runs for 1.7e-4 of the
 // draws.  The FP64 oracle mode keeps Acklam.
+//
+// Instruction economy: x = 2u - 1 is the exact integer x - 2^31 times 2^-31,
+// and the 2^-31 is carried by the polynomial coefficients (scaling by a power
+// of two commutes with rounding, so the product is the same to the bit); the
+// branch test w < 8 is made on L = log2(4u(1-u)) as L > -8 / ln 2.
+constexpr float kInvNormScale = 0x1p-31f;
+constexpr float kInvNormLogSplit = -11.5415603271117f;  // -8 / ln 2
+
+__device__ __forceinline__ float inv_normal_central( float t )
+{
+    constexpr float s = kInvNormScale;
+    float p = 5.622668877e-10f * s;
+    p = fmaf( p, t, -7.025212874e-10f * s );
+    p = fmaf( p, t, -4.983229672e-08f * s );
+    p = fmaf( p, t, 2.970335407e-07f * s );
+    p = fmaf( p, t, 7.164527608e-07f * s );
+    p = fmaf( p, t, -2.224123390e-05f * s );
+    p = fmaf( p, t, 1.679629502e-04f * s );
+    p = fmaf( p, t, -2.876509369e-04f * s );
+    p = fmaf( p, t, -1.021180627e-02f * s );
+    p = fmaf( p, t, 3.229416742e-01f * s );
+    p = fmaf( p, t, 2.628710527e+00f * s );
+    return p;
+}
+
 __device__ __forceinline__ float normal_f32( uint32_t x )
 {
     // u = 0 (x = 0, reachable under a random digital shift) is read as the
@@ -184,64 +209,40 @@ __device__ __forceinline__ float normal_f32( uint32_t x )
     x = x > 1u ? x : 1u;
     const float a = (float)x;
     const float b = (float)( 0u - x );
-    const float w = -fast_log( a * b * 2.168404344971009e-19f );  // 2^-62
-    const float xs = (float)(int32_t)( x - 0x80000000u ) * 4.656612873077393e-10f;  // 2^-31
+    const float L = __builtin_amdgcn_logf( a * b * 0x1p-62f );
+    const float xi = (float)(int32_t)( x - 0x80000000u );
     float p;
-    if ( w < 8.0f )
-    {
-        const float t = w - 4.0f;
-        p = 5.622668877e-10f;
-        p = fmaf( p, t, -7.025212874e-10f );
-        p = fmaf( p, t, -4.983229672e-08f );
-        p = fmaf( p, t, 2.970335407e-07f );
-        p = fmaf( p, t, 7.164527608e-07f );
-        p = fmaf( p, t, -2.224123390e-05f );
-        p = fmaf( p, t, 1.679629502e-04f );
-        p = fmaf( p, t, -2.876509369e-04f );
-        p = fmaf( p, t, -1.021180627e-02f );
-        p = fmaf( p, t, 3.229416742e-01f );
-        p = fmaf( p, t, 2.628710527e+00f );
-    }
+    if ( L > kInvNormLogSplit )
+        p = inv_normal_central( fmaf( L, -0.693147180559945309f, -4.0f ) );
     else
     {
-        const float t = __builtin_amdgcn_sqrtf( w ) - 3.0f;
-        p = 5.879603428e-05f;
-        p = fmaf( p, t, -5.247431012e-04f );
-        p = fmaf( p, t, 2.067292677e-03f );
-        p = fmaf( p, t, -4.845630704e-03f );
-        p = fmaf( p, t, 7.938630347e-03f );
-        p = fmaf( p, t, -1.079775749e-02f );
-        p = fmaf( p, t, 1.336126330e-02f );
-        p = fmaf( p, t, 1.416582332e+00f );
-        p = fmaf( p, t, 4.006434188e+00f );
+        constexpr float s = kInvNormScale;
+        const float t = __builtin_amdgcn_sqrtf( L * -0.693147180559945309f ) - 3.0f;
+        p = 5.879603428e-05f * s;
+        p = fmaf( p, t, -5.247431012e-04f * s );
+        p = fmaf( p, t, 2.067292677e-03f * s );
+        p = fmaf( p, t, -4.845630704e-03f * s );
+        p = fmaf( p, t, 7.938630347e-03f * s );
+        p = fmaf( p, t, -1.079775749e-02f * s );
+        p = fmaf( p, t, 1.336126330e-02f * s );
+        p = fmaf( p, t, 1.416582332e+00f * s );
+        p = fmaf( p, t, 4.006434188e+00f * s );
     }
-    return xs * p;
+    return xi * p;
 }
 
 // The central piece of normal_f32 alone (no branch); `tail` is set where
 // normal_f32 takes the tail polynomial instead, and for those lanes the
-// caller replaces the value by normal_f32's.
+// caller replaces the value by normal_f32's.  x = 0 gives L = -inf: a tail
+// lane, so the clamp of normal_f32 is not needed here.
 __device__ __forceinline__ float normal_f32_central( uint32_t x, bool& tail )
 {
-    x = x > 1u ? x : 1u;
     const float a = (float)x;
     const float b = (float)( 0u - x );
-    const float w = -fast_log( a * b * 2.168404344971009e-19f );  // 2^-62
-    const float xs = (float)(int32_t)( x - 0x80000000u ) * 4.656612873077393e-10f;  // 2^-31
-    const float t = w - 4.0f;
-    float p = 5.622668877e-10f;
-    p = fmaf( p, t, -7.025212874e-10f );
-    p = fmaf( p, t, -4.983229672e-08f );
-    p = fmaf( p, t, 2.970335407e-07f );
-    p = fmaf( p, t, 7.164527608e-07f );
-    p = fmaf( p, t, -2.224123390e-05f );
-    p = fmaf( p, t, 1.679629502e-04f );
-    p = fmaf( p, t, -2.876509369e-04f );
-    p = fmaf( p, t, -1.021180627e-02f );
-    p = fmaf( p, t, 3.229416742e-01f );
-    p = fmaf( p, t, 2.628710527e+00f );
-    tail = !( w < 8.0f );
-    return xs * p;
+    const float L = __builtin_amdgcn_logf( a * b * 0x1p-62f );
+    const float xi = (float)(int32_t)( x - 0x80000000u );
+    tail = !( L > kInvNormLogSplit );
+    return xi * inv_normal_central( fmaf( L, -0.693147180559945309f, -4.0f ) );
 }
 
 template <class R>
@@ -1328,12 +1329,13 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 float det, bvol;
                 if ( uniform || B.asset[a].xi > 1e-12 )
                 {
-                    const float aa = L * sc[SC_A1] - L * L * sc[SC_A2];
-                    const float K1 = aa - L * sc[SC_RX];
-                    const float K2 = aa + L * sc[SC_RX];
-                    const float var = fmaxf( L * L * sc[SC_K3] * ( vp + vc ), 0.0f );
-                    det = cd + L * sc[SC_K0] + K1 * vp + K2 * vc;
-                    bvol = __builtin_amdgcn_sqrtf( var );
+                    // K0 + K1 vp + K2 vc with K1,2 = L A1 - L^2 A2 -+ L RX, grouped as
+                    // L (K0 + A1 (vp + vc) + RX (vc - vp)) - L^2 A2 (vp + vc)
+                    const float sv = vp + vc;
+                    const float u = fmaf( sc[SC_RX], vc - vp, fmaf( sc[SC_A1], sv, sc[SC_K0] ) );
+                    const float l2s = L * L * sv;
+                    det = fmaf( -sc[SC_A2], l2s, fmaf( L, u, cd ) );
+                    bvol = __builtin_amdgcn_sqrtf( fmaxf( sc[SC_K3] * l2s, 0.0f ) );
                 }
                 else
                 {
