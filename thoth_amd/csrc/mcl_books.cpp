@@ -134,7 +134,13 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
     db.sd0 = bp.sd0;
     db.sd0f = (float)bp.sd0;
     db.n_ops = (int)bp.op_b.size();
+    // FP32 weights: the left weight of an op without a left anchor is 0, so the
+    // FP32 kernels form lw W_left + rw W_right + sd z without a select
+    // (0 W_left adds an exact zero)
     std::vector<float> op_wf( bp.op_w.begin(), bp.op_w.end() );
+    for ( size_t o = 0; o < bp.op_left.size(); o++ )
+        if ( !bp.op_left[o] )
+            op_wf[3 * o] = 0.0f;
     of.t = ar.put( in.t, nd );
     of.dt = ar.put( in.dt, nd );
     of.sqrt_dt = ar.put( sqrt_dt );

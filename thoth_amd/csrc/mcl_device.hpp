@@ -89,7 +89,7 @@ struct DevBook
     const int* op_b;      // [n_ops] bridge build index of the op (Sobol dim = b * factors + f)
     const int* op_left;   // [n_ops] has a left anchor
     const double* op_w;   // [n_ops * 3] left weight, right weight, std dev
-    const float* op_wf;   // FP32 copy
+    const float* op_wf;   // FP32 copy, left weight 0 where the op has no left anchor
     // equities
     DevAsset asset[kMaxAssets];
     double chol[kMaxAssets * kMaxAssets];

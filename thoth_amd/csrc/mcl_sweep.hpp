@@ -1044,7 +1044,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             // straight back, so the new point is consumed without touching the
             // stack (same arithmetic as the general walk below)
             const int o = o0;
-            const bool left = as_const( B.op_left )[o] != 0;
+            // FP32 left weights are 0 where the op has no left anchor
+            const bool left = !F64 || as_const( B.op_left )[o] != 0;
             R lw, rw, sd;
             if constexpr ( F64 )
             {
@@ -1081,7 +1082,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
         for ( int o = o0; o < o1; o++ )
         {
-            const bool left = as_const( B.op_left )[o] != 0;
+            // FP32 left weights are 0 where the op has no left anchor
+            const bool left = !F64 || as_const( B.op_left )[o] != 0;
             const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
             R lw, rw, sd;
             if constexpr ( F64 )
