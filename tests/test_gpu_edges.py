@@ -151,3 +151,31 @@ def test_limits_and_invalid_requests_are_errors(gpu):
     assert _error(gpu, s) == abi.ERR_INVALID
     # the device is still usable after every refusal
     assert gpu.price(BuiltRequest(bs(paths=1 << 10))).premium > 0
+
+
+@pytest.mark.parametrize("kind,days", [("bs", 3000), ("heston", 2000), ("lsv2", 700)])
+def test_long_bridges_fp64_and_fp32(gpu, oracle, kind, days):
+    """Long daily grids: deeper bridge stacks (depth 10 to 16 instantiations)
+    and larger Sobol rings; FP64 against the oracle at 1e-10, FP32 against
+    FP64 on the same points."""
+    print("case: long bridge %s %d days" % (kind, days), flush=True)
+    if kind == "bs":
+        eqs, corr, cs = [Equity(spot=100, volatility=25.0)], None, [Vanilla(0, days, 110.0)]
+    elif kind == "heston":
+        eqs = [Equity(spot=100, vol_kind=abi.VOL_HESTON, init_vol=20, long_vol=22, kappa=1.2, vol_of_vol=0.4,
+                      rho_spot_var=-0.6)]
+        corr, cs = None, [Barrier(0, days, 100.0, abi.DOWN_OUT, 70.0, discrete=True, monitoring_period_days=5)]
+    else:
+        eqs, c = configs.lsv_assets(2)
+        corr, cs = c.tolist(), [Vanilla(-1, days, 100.0)]
+    spec = BookSpec(mcl=McConfig(paths=1 << 10, max_day_step=1), equities=eqs, correlation=corr, contracts=cs,
+                    rainbows=[RainbowUdl([0, 1], best=False)] if kind == "lsv2" else [], rate_values=[2.0])
+    b = BuiltRequest(spec)
+    g, o = gpu.price(b), oracle.price(b)
+    assert rel(g.premium, o.premium) < REL, (g.premium, o.premium)
+    spec.precision = abi.FP32
+    g32 = gpu.price(BuiltRequest(spec))
+    # a knock-out monitored 400 times may flip on one of the 1024 paths in FP32
+    # (one payoff / 1024, about 2e-3 of the premium); otherwise 1e-4
+    tol = 5e-3 if kind == "heston" else 1e-4
+    assert abs(g32.premium - g.premium) < tol * max(abs(g.premium), 1.0), (g32.premium, g.premium)
