@@ -35,6 +35,10 @@ UNITS = [
     ("mcl_sweep_f64.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_sweep_f32.hip", "hipcc", []),
     ("mcl_sweep_f32_uniform.hip", "hipcc", []),
+    # five-equity uniform books: ILP-first scheduling (same occupancy, better
+    # interleaving of the equities' steps; the one-equity kernels lose
+    # occupancy under it and stay in the unit above)
+    ("mcl_sweep_f32_uniform5.hip", "hipcc", ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]),
     ("mcl_util.hip", "hipcc", []),
     ("mcl_cv.hip", "hipcc", []),
     ("mcl_pathwise.hip", "hipcc", ["-ffp-contract=off"]),
