@@ -524,6 +524,9 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         B.db[b].sob_slots = RingSlots( B.step_ops[b], lev_group );
     }
     B.lds_bytes = LdsBytes( B, lev_group );
+    if ( B.lds_bytes + kSweepStaticLds > (size_t)160 * 1024 )
+        return SetError( THOTH_ERR_UNSUPPORTED, "books need more than the 160 KiB of LDS of a workgroup "
+                                                "(fewer steps, equities or leverage points)" );
     // grid partials [R][blocks][nm][2 nk][2] and their sums [nm][2 nk][2], per book
     B.grid_off.assign( n_books, 0 );
     B.grid_out_off.assign( n_books, 0 );
@@ -724,6 +727,8 @@ int SetLevGroup( DeviceBatch& B, int G )
 {
     if ( G < 1 || G > 16 )
         return SetError( THOTH_ERR_INVALID, "leverage group out of range" );
+    if ( LdsBytes( B, G ) + kSweepStaticLds > (size_t)160 * 1024 )
+        return SetError( THOTH_ERR_UNSUPPORTED, "steps per LDS barrier beyond the LDS of a workgroup" );
     for ( size_t b = 0; b < B.db.size(); b++ )
     {
         B.db[b].lev_group = G;

@@ -116,6 +116,8 @@ int Autotune( GridPlan& P )
     for ( int g : lg )
     {
         int rc = SetLevGroup( P.batch, g );
+        if ( rc == THOTH_ERR_UNSUPPORTED )
+            continue;  // this G does not fit the LDS of a workgroup
         if ( rc != THOTH_OK )
             return rc;
         P.lev_group = g;
