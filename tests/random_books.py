@@ -1,0 +1,62 @@
+"""Seeded random books for the device/oracle parity sweep: 1-4 equities of
+mixed models (Black-Scholes, Heston, LSV on a SABR surface), equicorrelated,
+with vanillas and barriers (every barrier type, discrete or continuous,
+absolute or relative strikes) on single equities and best/worst-of
+rainbows, short maturities so the oracle's LSV calibration stays fast."""
+import numpy as np
+
+from thoth_amd import abi
+from thoth_amd.book import Barrier, BookSpec, Equity, McConfig, RainbowUdl, Sabr, Vanilla
+
+
+def random_book(seed, paths=1 << 10):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 5))
+    eqs = []
+    for _ in range(n):
+        kind = int(rng.integers(0, 3))
+        spot = float(rng.uniform(50, 150))
+        div = float(rng.choice([0.0, rng.uniform(0.0, 3.0)]))
+        if kind == 0:
+            eqs.append(Equity(spot=spot, volatility=float(rng.uniform(10, 40)), dividend_yield=div))
+            continue
+        e = Equity(spot=spot, vol_kind=abi.VOL_HESTON if kind == 1 else abi.VOL_LSV,
+                   init_vol=float(rng.uniform(12, 30)), long_vol=float(rng.uniform(12, 30)),
+                   kappa=float(rng.uniform(0.5, 3.0)), vol_of_vol=float(rng.uniform(0.2, 0.8)),
+                   rho_spot_var=float(rng.uniform(-0.9, -0.1)), dividend_yield=div)
+        if kind == 2:
+            a = float(rng.uniform(0.18, 0.30))
+            e.surface = Sabr([0.1, 0.5], [a, a * 1.05], [1.0, 1.0], [float(rng.uniform(-0.6, -0.3))] * 2,
+                             [float(rng.uniform(0.4, 0.7))] * 2)
+        eqs.append(e)
+    rho = float(rng.uniform(-0.2, 0.7)) if n > 1 else 0.0
+    corr = [[1.0 if i == j else rho for j in range(n)] for i in range(n)]
+    rainbows = []
+    if n > 1 and rng.random() < 0.6:
+        k = int(rng.integers(2, n + 1))
+        rainbows.append(RainbowUdl(sorted(rng.choice(n, size=k, replace=False).tolist()), best=bool(rng.random() < 0.5)))
+    contracts = []
+    for _ in range(int(rng.integers(1, 5))):
+        udl = -1 if rainbows and rng.random() < 0.4 else int(rng.integers(0, n))
+        ref = 100.0 if udl < 0 else eqs[udl].spot
+        days = int(rng.integers(5, 46))
+        absolute = bool(rng.random() < 0.7)
+        strike = float(rng.uniform(0.8, 1.2)) * (ref if absolute else 100.0)
+        call = bool(rng.random() < 0.5)
+        if rng.random() < 0.5:
+            contracts.append(Vanilla(udl, days, strike, is_call=call, is_absolute_strike=absolute))
+            continue
+        btype = int(rng.integers(0, 4))
+        up = btype in (abi.UP_OUT, abi.UP_IN)
+        level = ref * float(rng.uniform(1.03, 1.25) if up else rng.uniform(0.75, 0.97))
+        discrete = udl < 0 or rng.random() < 0.6
+        contracts.append(Barrier(udl, days, strike, btype, level, is_call=call, discrete=discrete,
+                                 monitoring_period_days=int(rng.integers(1, 6)), is_absolute_strike=absolute))
+    lsv = any(e.vol_kind == abi.VOL_LSV for e in eqs)
+    return BookSpec(mcl=McConfig(paths=paths, max_day_step=int(rng.integers(1, 4)),
+                                 sobol_skip=int(rng.integers(0, 5000))),
+                    equities=eqs, contracts=contracts, correlation=corr,
+                    rate_days=[0, 30], rate_values=[float(rng.uniform(0, 4)), float(rng.uniform(0, 4))],
+                    rainbows=rainbows, correlate_stochastic=bool(rng.random() < 0.3),
+                    leverage_surface=(40, 8) if lsv and rng.random() < 0.5 else None,
+                    greeks=int(rng.choice([0, abi.GREEK_DELTA | abi.GREEK_RHO])))

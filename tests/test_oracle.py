@@ -361,3 +361,14 @@ def test_lsv_surface_reprices_its_target(oracle):
         ref = math.exp(-0.02 * T) * (F * norm.cdf(d1) - K * norm.cdf(d1 - v * math.sqrt(T)))
         print("K %.0f surface %.4f +- %.4f target %.4f" % (K, p, t, ref))
         assert abs(p - ref) < 6 * t + 0.10
+
+
+def test_random_books_are_valid_requests(oracle):
+    """The seeded random books of the device parity sweep are requests the
+    oracle prices (no refusals), with finite premiums."""
+    import math
+    from random_books import random_book
+    from thoth_amd.book import BuiltRequest
+    for seed in range(8):
+        r = oracle.price(BuiltRequest(random_book(seed)))
+        assert math.isfinite(r.premium) and r.premium >= 0.0
