@@ -51,8 +51,8 @@ def test_sample_pricer_fp64_matches_oracle(gpu, oracle, sample, pricer):
 def test_simple_call_sample_at_its_own_paths(gpu):
     text = (SAMPLES / "simple_call.yaml").read_text()
     res = yf.price(text)["simple_call_result"]
-    for key in ("premium", "premium_trust", "delta", "gamma", "vega", "rho", "simple_call_premium",
-                "simple_call_premium_trust", "simple_call_delta"):
+    for key in ("premium", "premium_trust", "delta", "gamma", "vega", "rho", "theta", "simple_call_premium",
+                "simple_call_premium_trust", "simple_call_delta", "simple_call_theta"):
         assert key in res, key
     S, K, r, v, T = 100.0, 100.0, 0.08, 0.30, 1.0
     d1 = (math.log(S / K) + (r + v * v / 2) * T) / (v * math.sqrt(T))
@@ -74,3 +74,18 @@ def test_lsv_sample_reprices_its_sabr_target(gpu, oracle):
     ref = math.exp(-r * T) * (F * norm.cdf(d1) - 100.0 * norm.cdf(d1 - vol * math.sqrt(T)))
     print("lsv sample: device %.6f +- %.6f, Black(SABR) %.6f" % (res["premium"], res["premium_trust"], ref))
     assert abs(res["premium"] - ref) <= 6 * res["premium_trust"] + 0.10
+
+
+def test_theta_matches_the_oracle_roll(gpu, oracle):
+    """Theta of the vanilla and barrier samples: the device's one-day roll
+    against the oracle's, each premium at 1e-10."""
+    objs = yf.load((SAMPLES / "matrix.yaml").read_text())
+    for name in ("vanilla_mono_eu_put_mcl", "barrier_mono_discrete_call_mcl"):
+        t0 = yf.translate(objs, name, paths=1 << 12)
+        t1 = yf.translate(objs, name, paths=1 << 12, roll_days=1)
+        t0.spec.greeks = t1.spec.greeks = 0
+        b0, b1 = BuiltRequest(t0.spec), BuiltRequest(t1.spec)
+        dev = yf.result_block(t0, gpu.price(b0), gpu.price(b1))["theta"]
+        ora = yf.result_block(t0, oracle.price(b0), oracle.price(b1))["theta"]
+        P = abs(oracle.price(b0).premium)
+        assert abs(dev - ora) <= 2e-10 * P, (name, dev, ora)

@@ -61,7 +61,7 @@ def test_simple_call_translation():
     t = yf.translate(_objs("simple_call"), "simple_call_pricing")
     s = t.spec
     assert t.result == "simple_call_result" and t.contracts == ["simple_call"]
-    assert t.greeks == ["delta", "gamma", "vega", "rho"] and t.not_computed == ["theta"]
+    assert t.greeks == ["delta", "gamma", "vega", "rho", "theta"] and t.not_computed == []
     assert s.greeks == abi.GREEK_DELTA | abi.GREEK_GAMMA | abi.GREEK_VEGA | abi.GREEK_RHO
     assert (s.mcl.paths, s.mcl.max_day_step, s.mcl.min_day_step, s.mcl.use_sobol) == (50000, 1, -1, True)
     (e,) = s.equities
@@ -103,3 +103,24 @@ def test_simple_call_prices_black_scholes(oracle):
 def test_unsupported_is_a_value_error():
     with pytest.raises(ValueError):
         yf.translate(_objs("matrix"), "vanilla_mono_am_put_mcl")
+
+
+def test_theta_is_the_one_day_roll(oracle):
+    """Theta as the reference computes it (tasks/pricer.cpp:371-384): the
+    premium with the valuation date one day later (spot held) minus the
+    premium.  For the Black-Scholes call sample the payoff reads the bridge
+    endpoint alone (the first Sobol dimension in both runs), so the MC
+    difference is the analytic one-day decay up to the integration error."""
+    objs = _objs("simple_call")
+    t0 = yf.translate(objs, "simple_call_pricing", paths=1 << 13)
+    t1 = yf.translate(objs, "simple_call_pricing", paths=1 << 13, roll_days=1)
+    assert t1.spec.contracts[0].maturity_days == 364 and list(t1.spec.rate_days) == [-1, 3652]
+    t0.spec.greeks = t1.spec.greeks = 0
+    r0, r1 = oracle.price(BuiltRequest(t0.spec)), oracle.price(BuiltRequest(t1.spec))
+    theta = yf.result_block(t0, r0, r1)["theta"]
+
+    def bs(T):
+        d1 = (math.log(100 / 100) + (0.08 + 0.045) * T) / (0.3 * math.sqrt(T))
+        return 100 * norm.cdf(d1) - 100 * math.exp(-0.08 * T) * norm.cdf(d1 - 0.3 * math.sqrt(T))
+    ref = bs(364 / 365) - bs(1.0)
+    assert ref < 0 and abs(theta - ref) < 2e-3, (theta, ref)

@@ -23,8 +23,8 @@ Anything outside it raises `Unsupported` with the reason: the caller keeps
 its CPU engine for that pricer, as the reference's gate does
 (tasks/pricer_mcl_gpu_gate.cpp:83-106).  Contract nominals are ignored, as
 the reference's vanilla and barrier contracts ignore them.  The indicators
-priced are premium, delta, gamma, vega and rho; theta is reported as not
-computed.
+priced are premium, delta, gamma, vega, rho and theta (a second sweep one
+calendar day later, spot held).
 """
 import datetime
 from dataclasses import dataclass, field
@@ -105,10 +105,14 @@ def _flat_matrix(corr, n):
     return [flat[i * n:(i + 1) * n] for i in range(n)]
 
 
-def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, paths: int = 0) -> Translation:
-    """The device request of one `!mcl_pricer` (paths > 0 overrides its path count)."""
+def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, paths: int = 0,
+              roll_days: int = 0) -> Translation:
+    """The device request of one `!mcl_pricer` (paths > 0 overrides its path
+    count; roll_days moves the valuation date forward, spot held, as the
+    reference's theta roll does, tasks/pricer.cpp:371-384)."""
     p = _obj(objs, pricer, "mcl_pricer")
     today = p["today"] if isinstance(p["today"], datetime.date) else datetime.date.fromisoformat(str(p["today"]))
+    today += datetime.timedelta(days=roll_days)
     ccy_name = p["currency"]
     ccy = _obj(objs, ccy_name, "currency")
     for key in ("discount_rate", "rate_model"):
@@ -227,29 +231,33 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
         c_names.append(cname)
 
     indicators = [str(x) for x in p.get("indicators", ["premium"])]
-    greeks = [g for g in indicators if g in _GREEKS]
-    not_computed = [g for g in indicators if g != "premium" and g not in _GREEKS]
+    greeks = [g for g in indicators if g in _GREEKS or g == "theta"]
+    not_computed = [g for g in indicators if g != "premium" and g not in greeks]
     n = len(equities)
     spec = BookSpec(mcl=cfg, equities=equities, contracts=contracts,
                     correlation=[[correl(eq_names[i], eq_names[j]) for j in range(n)] for i in range(n)],
                     rate_days=rate_days, rate_values=rate_values, rainbows=rainbows,
-                    greeks=sum(_GREEKS[g] for g in greeks), precision=precision)
+                    greeks=sum(_GREEKS.get(g, 0) for g in greeks), precision=precision)
     return Translation(spec, c_names, str(p.get("result", pricer)), greeks, not_computed)
 
 
-def result_block(t: Translation, r) -> dict:
+def result_block(t: Translation, r, rolled=None) -> dict:
     """The reference's result fields (tasks/pricer.cpp:536-596) from an engine
     result: premium, premium_trust, the requested Greeks, and per contract
-    <name>_premium, <name>_premium_trust and <name>_<greek>."""
+    <name>_premium, <name>_premium_trust and <name>_<greek>.  Theta is the
+    premium one calendar day later minus the premium (`rolled`, the result of
+    the request rolled by a day, tasks/pricer.cpp:371-384)."""
     out = {"premium": r.premium, "premium_trust": r.premium_trust}
-    for g in t.greeks:
-        out[g] = getattr(r, g)
     per = {"delta": r.contract_delta, "gamma": r.contract_gamma, "vega": r.contract_vega, "rho": r.contract_rho}
+    greeks = [g for g in t.greeks if g != "theta" or rolled is not None]  # no roll given: no theta
+    for g in greeks:
+        out[g] = rolled.premium - r.premium if g == "theta" else getattr(r, g)
     for c, name in enumerate(t.contracts):
         out[name + "_premium"] = r.contract_premium[c]
         out[name + "_premium_trust"] = r.contract_trust[c]
-        for g in t.greeks:
-            out[name + "_" + g] = per[g][c]
+        for g in greeks:
+            out[name + "_" + g] = (rolled.contract_premium[c] - r.contract_premium[c] if g == "theta"
+                                   else per[g][c])
     return out
 
 
@@ -262,5 +270,10 @@ def price(text: str, pricer: str = None, precision: int = abi.FP64, paths: int =
     out = {}
     for name in ([pricer] if pricer else mcl_pricers(objs)):
         t = translate(objs, name, precision, paths)
-        out[t.result] = result_block(t, mcl.price(BuiltRequest(t.spec)))
+        rolled = None
+        if "theta" in t.greeks:
+            t1 = translate(objs, name, precision, paths, roll_days=1)
+            t1.spec.greeks = 0
+            rolled = mcl.price(BuiltRequest(t1.spec))
+        out[t.result] = result_block(t, mcl.price(BuiltRequest(t.spec)), rolled)
     return out
