@@ -183,11 +183,20 @@ __global__ void __launch_bounds__( 256 ) cv_grad_kernel( const uint2* __restrict
     for ( int k = 0; k < 13; k++ )
         G.b[k] = f32x4{ 0.f, 0.f, 0.f, 0.f };
     const int j0 = chunk * dpc, j1 = min( J, j0 + dpc );
+    // the next date's features and increments are loaded while this date's
+    // forward and backward run (one wave per SIMD: nothing else hides them)
+    bf16x4 fx = load_feat( feat, (size_t)j0 * N + p, lane );
+    f32x4 dx = load_dm( dm, (size_t)j0 * N + p, lane );
     for ( int j = j0; j < j1; j++ )
     {
-        const size_t row = (size_t)j * N + p;
-        (void)mlp_forward_lds( wimg, bias, load_feat( feat, row, lane ), lane, act );
-        const f32x4 d = load_dm( dm, row, lane );
+        const bf16x4 f = fx;
+        const f32x4 d = dx;
+        if ( j + 1 < j1 )
+        {
+            fx = load_feat( feat, (size_t)( j + 1 ) * N + p, lane );
+            dx = load_dm( dm, (size_t)( j + 1 ) * N + p, lane );
+        }
+        (void)mlp_forward_lds( wimg, bias, f, lane, act );
         mlp_backward_lds( wbwd, act, f32x4{ g * d[0], g * d[1], g * d[2], g * d[3] }, lane, G );
     }
     // bias partials: sum the 16 sample columns of each lane group
