@@ -14,7 +14,8 @@
  * whose conditional mean given the path up to t_{i-1} is exactly zero.  CV
  * has zero mean for ANY weights, so P - CV is an unbiased estimator of the
  * premium P.  Networks: 11 -> 64 -> 64
- * -> 64 -> 5, SiLU, BF16 operands with FP32 accumulation on the matrix cores,
+ * -> 64 -> 5, smoothstep-gated linear units a(z) = z t^2 (3 - 2t) with
+ * t = clamp(z/6 + 1/2, 0, 1), BF16 operands with FP32 accumulation on the matrix cores,
  * FP32 master weights, Adam.  Plain Monte-Carlo is reported beside every
  * control-variate estimate as the yardstick of bias and variance reduction.
  */

@@ -3,7 +3,8 @@ its training loss, used to check the device training kernels' forward pass,
 gradients and Adam update.
 
 Network (no reference counterpart; the reference has no control variates):
-    x (11) -> Linear 64 -> SiLU -> Linear 64 -> SiLU -> Linear 64 -> SiLU -> Linear 5
+    x (11) -> Linear 64 -> a -> Linear 64 -> a -> Linear 64 -> a -> Linear 5
+    a(z) = z s(t), t = clip(z/6 + 1/2, 0, 1), s(t) = t^2 (3 - 2t)
 Loss of one minibatch of B paths with J hedge dates each:
     CV_p = sum_j f(x_{p,j}) . dM_{p,j},   L = mean_p (P_p - CV_p)^2
 Adam (Kingma & Ba 2015): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
@@ -52,20 +53,23 @@ def bf16_bits(x):
     return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def act(z):
+    """smoothstep-gated linear unit a(z) = z s(t), t = clip(z/6 + 1/2, 0, 1),
+    s = t^2 (3 - 2t)."""
+    t = np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+    return z * t * t * (3.0 - 2.0 * t)
 
 
-def dsilu(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def dact(z):
+    t = np.clip(z / 6.0 + 0.5, 0.0, 1.0)
+    return t * t * (3.0 - 2.0 * t) + z * t * (1.0 - t)
 
 
 def forward(params, x, matrix_cores=False):
     """params: list of (W [in,out], b [out]); x [N, 11] -> (f [N, 5], cache).
 
     matrix_cores=True rounds where the BF16 matrix-core kernels round: the
-    weights and every hidden activation SiLU(z) to bfloat16 (products and sums
+    weights and every hidden activation a(z) to bfloat16 (products and sums
     stay exact here, FP32 on the device)."""
     h = x
     cache = []
@@ -73,7 +77,7 @@ def forward(params, x, matrix_cores=False):
     for li, (W, b) in enumerate(params):
         z = h @ rnd(W) + b
         cache.append((h, z))
-        h = rnd(silu(z)) if li < len(params) - 1 else z
+        h = rnd(act(z)) if li < len(params) - 1 else z
     return h, cache
 
 
@@ -95,7 +99,7 @@ def loss_and_grads(params, x, dm, payoff, matrix_cores=False):
     for li in range(len(params) - 1, -1, -1):
         h, z = cache[li]
         if li < len(params) - 1:
-            g = g * dsilu(z)
+            g = g * dact(z)
         W, _ = params[li]
         grads[li] = (h.T @ rnd(g), g.sum(axis=0))
         g = rnd(g) @ rnd(W).T

@@ -97,3 +97,19 @@ def test_matrix_core_emulation_is_close_to_exact():
     assert abs(l1 - l0) <= 2e-2 * l0
     a, b = mo.pack(g0), mo.pack(g1)
     assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(a)
+
+
+def test_activation_closed_forms():
+    # a(z) = z t^2 (3 - 2t), t = clip(z/6 + 1/2): zero below -3, identity above 3,
+    # a(0) = 0 with slope 1/2, a(3/2) = 3/2 * (3/4)^2 * 3/2 = 81/64, continuous
+    # value and slope at the joins
+    z = np.array([-10.0, -3.0, 0.0, 1.5, 3.0, 10.0])
+    assert np.array_equal(mo.act(z), [0.0, 0.0, 0.0, 81.0 / 64.0, 3.0, 10.0])
+    assert np.array_equal(mo.dact(z), [0.0, 0.0, 0.5, 0.5625 * 1.5 + 1.5 * 0.75 * 0.25, 1.0, 1.0])
+    for j in (-3.0, 3.0):
+        e = 1e-7
+        assert abs(mo.act(j + e) - mo.act(j - e)) < 3e-7
+        assert abs(mo.dact(j + e) - mo.dact(j - e)) < 1e-6
+    zz = np.linspace(-5, 5, 101)
+    fd = (mo.act(zz + 1e-6) - mo.act(zz - 1e-6)) / 2e-6
+    assert np.allclose(fd, mo.dact(zz), atol=1e-5)  # h * the jump of a'' at +-3

@@ -1,7 +1,7 @@
 // mcl_mlp.hpp -- the hedge-ratio MLP on CDNA4 matrix cores (no reference
 // counterpart: the reference engine has no control variates).
 //
-//   f(x) = W4 s(W3 s(W2 s(W1 x + b1) + b2) + b3) + b4,   s = SiLU,
+//   f(x) = W4 s(W3 s(W2 s(W1 x + b1) + b2) + b3) + b4,   s = smoothstep-gated linear unit,
 //   x: 11 features (5 log-moneyness, 5 scaled vols, t/T) padded to 16,
 //   hidden width 64, 5 outputs (hedge ratios) padded to 16.
 //
@@ -78,12 +78,23 @@ __device__ __forceinline__ bf16x4 pack_bf16( f32x4 v )
     return __builtin_bit_cast( bf16x4, u32x2_t{ pack2_bf16( v[0], v[1] ), pack2_bf16( v[2], v[3] ) } );
 }
 
-__device__ __forceinline__ float silu( float z ) { return z * __builtin_amdgcn_rcpf( 1.0f + __expf( -z ) ); }
+// Smoothstep-gated linear unit, the hidden activation:
+//   a(z) = z s(t),  t = clamp(z / 6 + 1/2, 0, 1),  s(t) = t^2 (3 - 2t),
+// C1, 0 for z <= -3, z for z >= 3, a'(0) = 1/2 as SiLU's; five VALU
+// operations and no transcendental (SiLU needs an exp and a reciprocal).
+//   a'(z) = s(t) + z t (1 - t)   (s'(t) = 6 t (1 - t), dt/dz = 1/6)
+__device__ __forceinline__ float gate_t( float z ) { return __builtin_amdgcn_fmed3f( fmaf( z, 1.0f / 6.0f, 0.5f ), 0.0f, 1.0f ); }
 
-__device__ __forceinline__ float dsilu( float z )
+__device__ __forceinline__ float gate_act( float z )
 {
-    const float s = __builtin_amdgcn_rcpf( 1.0f + __expf( -z ) );
-    return s * ( 1.0f + z * ( 1.0f - s ) );
+    const float t = gate_t( z );
+    return z * ( t * t * fmaf( -2.0f, t, 3.0f ) );
+}
+
+__device__ __forceinline__ float gate_dact( float z )
+{
+    const float t = gate_t( z );
+    return fmaf( z * t, 1.0f - t, t * t * fmaf( -2.0f, t, 3.0f ) );
 }
 
 // A operand of tile (layer L, out-tile ot, k-tile kt) from a swizzled image:
@@ -110,7 +121,7 @@ __device__ __forceinline__ f32x4 bias_frag( const float* bias, int L, int ot, in
 struct MlpTile
 {
     f32x4 z[3][4];   // pre-activations of the hidden layers (D layout)
-    bf16x4 h[3][4];  // SiLU(z) as BF16 B operands
+    bf16x4 h[3][4];  // a(z) as BF16 B operands
     f32x4 out;       // outputs 0..15 (D layout; 0..4 meaningful)
 };
 
@@ -124,7 +135,7 @@ __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bia
         f32x4 c = bias_frag( bias, 0, ot, lane );
         c = mfma16( weight_frag( wimg, 0, ot, 0, lane ), x, c );
         t.z[0][ot] = c;
-        t.h[0][ot] = pack_bf16( f32x4{ silu( c[0] ), silu( c[1] ), silu( c[2] ), silu( c[3] ) } );
+        t.h[0][ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
     }
 #pragma unroll
     for ( int L = 1; L < 3; L++ )
@@ -137,7 +148,7 @@ __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bia
             for ( int kt = 0; kt < 4; kt++ )
                 c = mfma16( weight_frag( wimg, L, ot, kt, lane ), t.h[L - 1][kt], c );
             t.z[L][ot] = c;
-            t.h[L][ot] = pack_bf16( f32x4{ silu( c[0] ), silu( c[1] ), silu( c[2] ), silu( c[3] ) } );
+            t.h[L][ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
         }
     }
     f32x4 c = bias_frag( bias, 3, 0, lane );
@@ -156,10 +167,10 @@ __device__ __forceinline__ f32x4 mlp_infer( const uint2* wimg, const float* bias
         f32x4 c1 = mfma16( weight_frag( wimg, 0, 1, 0, lane ), x, bias_frag( bias, 0, 1, lane ) );
         f32x4 c2 = mfma16( weight_frag( wimg, 0, 2, 0, lane ), x, bias_frag( bias, 0, 2, lane ) );
         f32x4 c3 = mfma16( weight_frag( wimg, 0, 3, 0, lane ), x, bias_frag( bias, 0, 3, lane ) );
-        h0 = pack_bf16( f32x4{ silu( c0[0] ), silu( c0[1] ), silu( c0[2] ), silu( c0[3] ) } );
-        h1 = pack_bf16( f32x4{ silu( c1[0] ), silu( c1[1] ), silu( c1[2] ), silu( c1[3] ) } );
-        h2 = pack_bf16( f32x4{ silu( c2[0] ), silu( c2[1] ), silu( c2[2] ), silu( c2[3] ) } );
-        h3 = pack_bf16( f32x4{ silu( c3[0] ), silu( c3[1] ), silu( c3[2] ), silu( c3[3] ) } );
+        h0 = pack_bf16( f32x4{ gate_act( c0[0] ), gate_act( c0[1] ), gate_act( c0[2] ), gate_act( c0[3] ) } );
+        h1 = pack_bf16( f32x4{ gate_act( c1[0] ), gate_act( c1[1] ), gate_act( c1[2] ), gate_act( c1[3] ) } );
+        h2 = pack_bf16( f32x4{ gate_act( c2[0] ), gate_act( c2[1] ), gate_act( c2[2] ), gate_act( c2[3] ) } );
+        h3 = pack_bf16( f32x4{ gate_act( c3[0] ), gate_act( c3[1] ), gate_act( c3[2] ), gate_act( c3[3] ) } );
     }
     // hidden layers 2 and 3: one output tile at a time (fewest live registers)
     auto unit_tile = [&]( int L, int ot ) __attribute__( ( always_inline ) ) -> bf16x4
@@ -169,7 +180,7 @@ __device__ __forceinline__ f32x4 mlp_infer( const uint2* wimg, const float* bias
         c = mfma16( weight_frag( wimg, L, ot, 1, lane ), h1, c );
         c = mfma16( weight_frag( wimg, L, ot, 2, lane ), h2, c );
         c = mfma16( weight_frag( wimg, L, ot, 3, lane ), h3, c );
-        return pack_bf16( f32x4{ silu( c[0] ), silu( c[1] ), silu( c[2] ), silu( c[3] ) } );
+        return pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
     };
 #pragma unroll
     for ( int L = 1; L < 3; L++ )
@@ -251,7 +262,7 @@ struct MlpGrad
 //   dW_L^T += dZ_L^T H_{L-1}   (MFMA over the 16 samples of the tile, both
 //                               operands transposed through LDS),
 //   db_L   += row sums of dZ_L^T (per-lane partials, reduced by the caller),
-//   dH_{L-1}^T = W_L dZ_L^T,  dZ_{L-1} = dH_{L-1} * SiLU'(Z_{L-1}).
+//   dH_{L-1}^T = W_L dZ_L^T,  dZ_{L-1} = dH_{L-1} * a'(Z_{L-1}).
 __device__ __forceinline__ void mlp_backward( const uint2* wbwd, const MlpTile& t, bf16x4 x, f32x4 dout, short* buf,
                                               int lane, MlpGrad& g )
 {
@@ -288,7 +299,7 @@ __device__ __forceinline__ void mlp_backward( const uint2* wbwd, const MlpTile& 
             for ( int ot = 0; ot < no; ot++ )
                 c = mfma16( weight_frag_bwd( wbwd, L, it, ot, lane ), pack_bf16( dz[ot] ), c );
             const f32x4 z = t.z[L - 1][it];
-            nd[it] = f32x4{ c[0] * dsilu( z[0] ), c[1] * dsilu( z[1] ), c[2] * dsilu( z[2] ), c[3] * dsilu( z[3] ) };
+            nd[it] = f32x4{ c[0] * gate_dact( z[0] ), c[1] * gate_dact( z[1] ), c[2] * gate_dact( z[2] ), c[3] * gate_dact( z[3] ) };
         }
 #pragma unroll
         for ( int it = 0; it < 4; it++ )
@@ -358,7 +369,7 @@ __device__ __forceinline__ f32x4 mlp_forward_lds( const uint2* wimg, const float
         f32x4 c = bias_frag( bias, 0, ot, lane );
         c = mfma16( weight_frag( wimg, 0, ot, 0, lane ), x, c );
         act.z[ot][lane] = c;
-        h[ot] = pack_bf16( f32x4{ silu( c[0] ), silu( c[1] ), silu( c[2] ), silu( c[3] ) } );
+        h[ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
         store_rows( act.in[1 + ot], h[ot], lane );
     }
 #pragma unroll
@@ -373,7 +384,7 @@ __device__ __forceinline__ f32x4 mlp_forward_lds( const uint2* wimg, const float
             for ( int kt = 0; kt < 4; kt++ )
                 c = mfma16( weight_frag( wimg, L, ot, kt, lane ), h[kt], c );
             act.z[4 * L + ot][lane] = c;
-            hn[ot] = pack_bf16( f32x4{ silu( c[0] ), silu( c[1] ), silu( c[2] ), silu( c[3] ) } );
+            hn[ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
             store_rows( act.in[1 + 4 * L + ot], hn[ot], lane );
         }
 #pragma unroll
@@ -425,7 +436,7 @@ __device__ __forceinline__ void mlp_backward_lds( const uint2* wbwd, MlpActLds& 
             for ( int ot = 0; ot < no; ot++ )
                 c = mfma16( weight_frag_bwd( wbwd, L, it, ot, lane ), pack_bf16( dz[ot] ), c );
             const f32x4 z = act.z[4 * ( L - 1 ) + it][lane];
-            nd[it] = f32x4{ c[0] * dsilu( z[0] ), c[1] * dsilu( z[1] ), c[2] * dsilu( z[2] ), c[3] * dsilu( z[3] ) };
+            nd[it] = f32x4{ c[0] * gate_dact( z[0] ), c[1] * gate_dact( z[1] ), c[2] * gate_dact( z[2] ), c[3] * gate_dact( z[3] ) };
         }
 #pragma unroll
         for ( int it = 0; it < 4; it++ )

@@ -850,7 +850,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         const uint2* wimg = reinterpret_cast<const uint2*>( cvl );
         const float* wb = reinterpret_cast<const float*>( cvl + kCvImgBytes );
         // two sample tiles per iteration: their independent MFMA chains and
-        // SiLUs interleave (no extra registers at this kernel's peak)
+        // activations interleave (no extra registers at this kernel's peak)
 #pragma unroll 2
         for ( int t = 0; t < 4; t++ )
         {
