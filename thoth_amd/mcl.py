@@ -256,7 +256,7 @@ def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_see
 # ---- learned martingale control variate (include/thoth_mcl_cv.h) ----------
 
 def cv_config(hedge_dates=97, train_paths=1 << 18, train_sobol_skip=1 << 24, adam_steps=2000, batch_paths=32768,
-              learning_rate=1e-3, init_seed=7, eval_mode=abi.CV_IN_LOOP, hedge_spacing=2.0):
+              learning_rate=2e-3, init_seed=7, eval_mode=abi.CV_IN_LOOP, hedge_spacing=2.0):
     """Training/evaluation settings; the training paths come from a Sobol block
     (points train_sobol_skip+1 ..) disjoint from the evaluation replicates."""
     return abi.CvConfig(hedge_dates, train_paths, train_sobol_skip, adam_steps, batch_paths, learning_rate, init_seed,
