@@ -14,9 +14,10 @@
  * whose conditional mean given the path up to t_{i-1} is exactly zero.  CV
  * has zero mean for ANY weights, so P - CV is an unbiased estimator of the
  * premium P.  Networks: 11 -> 64 -> 64
- * -> 64 -> 5, smoothstep-gated linear units a(z) = z t^2 (3 - 2t) with
- * t = clamp(z/6 + 1/2, 0, 1), BF16 operands with FP32 accumulation on the matrix cores,
- * FP32 master weights, Adam.  Plain Monte-Carlo is reported beside every
+ * -> 64 -> 5 with SiLU hidden units a(z) = z / (1 + e^-z) (THOTH_CV_SILU) or
+ * smoothstep-gated linear units a(z) = z t^2 (3 - 2t), t = clamp(z/6 + 1/2, 0, 1)
+ * (THOTH_CV_GATE: no transcendental), BF16 operands with FP32 accumulation on
+ * the matrix cores, FP32 master weights, Adam.  Plain Monte-Carlo is reported beside every
  * control-variate estimate as the yardstick of bias and variance reduction.
  */
 #ifndef THOTH_MCL_CV_H
@@ -38,6 +39,9 @@ typedef struct thoth_cv_weights {
     float w[THOTH_CV_WEIGHTS];
 } thoth_cv_weights;
 
+/* hidden activation of the hedge network */
+enum { THOTH_CV_SILU = 0, THOTH_CV_GATE = 1 };
+
 typedef struct thoth_cv_config {
     int32_t hedge_dates;       /* J: hedge ratios are re-evaluated at J dates (hedge_spacing) */
     int64_t train_paths;       /* training paths (Sobol points train_sobol_skip+1 ..) */
@@ -50,6 +54,7 @@ typedef struct thoth_cv_config {
     float hedge_spacing;       /* p in [1, 8]: hedge date j at step m - round(m (1 - j/J)^p), at least one
                                   step apart; 1 = uniform, 2 = uniform in sqrt(T - t) (denser toward
                                   maturity, where the discrete-hedging error concentrates) */
+    int32_t activation;        /* THOTH_CV_SILU or THOTH_CV_GATE (training and pricing must agree) */
 } thoth_cv_config;
 
 /* pricing with a trained network: the hedge network evaluated on the matrix
@@ -73,6 +78,13 @@ typedef struct thoth_cv_result {
     double cv_mean, cv_halfwidth;
     double plain_var, cv_var;             /* pooled per-path variances of P and of P - CV */
     double sweep_ms, eval_ms;             /* device time: FP32 sweep (with in-loop inference) + separate inference */
+    /* paired difference plain - (P - CV), i.e. the mean of CV over the replicates, and the
+     * 95 % Student-t half-width of that mean: zero mean is the unbiasedness of the estimator */
+    double cv_term_mean, cv_term_halfwidth;
+    /* optional caller-owned arrays of `replicates` entries (NULL: not written):
+     * the replicate means of P and of P - CV */
+    double* replicate_plain_means;
+    double* replicate_cv_means;
 } thoth_cv_result;
 
 /* Train the hedge network for the base book of req (FP32 sweep).  w is the
@@ -96,9 +108,10 @@ int thoth_cv_samples(const thoth_mcl_request* req, int32_t hedge_dates, int64_t 
 
 /* Loss mean_p (P_p - sum_j f(x_pj).dm_pj)^2 and its gradient (same layout as
  * thoth_cv_weights) for B paths of host data: feat [J][B][16] BF16, dm [J][B][8],
- * pay [B]; B multiple of 64.  The device training step's forward/backward. */
+ * pay [B]; B multiple of 64; activation THOTH_CV_SILU or THOTH_CV_GATE.  The
+ * device training step's forward/backward. */
 int thoth_cv_loss_grad(const thoth_cv_weights* w, const uint16_t* feat, const float* dm, const float* pay, int32_t B,
-                       int32_t J, double* loss, float* grad);
+                       int32_t J, int32_t activation, double* loss, float* grad);
 
 #ifdef __cplusplus
 }

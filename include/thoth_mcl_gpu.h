@@ -71,6 +71,17 @@ int thoth_gpu_price_european_gbm(double forward, double strike, double t, double
  * No reference counterpart as an entry point; exposed for QMC users and tests. */
 int thoth_gpu_sobol_points(int32_t dim, int64_t skip, int64_t n, uint32_t* out);
 
+/* The FP32 special functions of the production (FP32) sweep evaluated on the
+ * device for n inputs (no reference counterpart; exposed so that their error
+ * bounds are checked on the hardware that runs them):
+ *   fn 0: the inverse normal of the Sobol coordinate in[i] (u = in[i] 2^-32,
+ *         u = 0 read as 2^-32), as the generic kernels evaluate it;
+ *   fn 1: the same through the uniform-book kernels' split form (central
+ *         polynomial for every lane, tail lanes redone);
+ *   fn 2: erfc of the float whose bits are in[i] (the QE exponential branch).
+ * n in [1, 2^28]; out[i] receives the result. */
+int thoth_gpu_special_f32(int32_t fn, int64_t n, const uint32_t* in, float* out);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* thoth_last_error(void);
 

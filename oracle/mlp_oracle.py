@@ -4,7 +4,8 @@ gradients and Adam update.
 
 Network (no reference counterpart; the reference has no control variates):
     x (11) -> Linear 64 -> a -> Linear 64 -> a -> Linear 64 -> a -> Linear 5
-    a(z) = z s(t), t = clip(z/6 + 1/2, 0, 1), s(t) = t^2 (3 - 2t)
+    a = SiLU, a(z) = z / (1 + e^-z)   ("silu", the specified network), or
+    a(z) = z s(t), t = clip(z/6 + 1/2, 0, 1), s(t) = t^2 (3 - 2t)   ("gate")
 Loss of one minibatch of B paths with J hedge dates each:
     CV_p = sum_j f(x_{p,j}) . dM_{p,j},   L = mean_p (P_p - CV_p)^2
 Adam (Kingma & Ba 2015): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
@@ -53,19 +54,34 @@ def bf16_bits(x):
     return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
 
 
-def act(z):
-    """smoothstep-gated linear unit a(z) = z s(t), t = clip(z/6 + 1/2, 0, 1),
+def act(z, kind="gate"):
+    """Hidden activation: "silu" a(z) = z / (1 + e^-z); "gate" the
+    smoothstep-gated linear unit a(z) = z s(t), t = clip(z/6 + 1/2, 0, 1),
     s = t^2 (3 - 2t)."""
+    if kind == "silu":
+        return z * _sigmoid(z)
     t = np.clip(z / 6.0 + 0.5, 0.0, 1.0)
     return z * t * t * (3.0 - 2.0 * t)
 
 
-def dact(z):
+def dact(z, kind="gate"):
+    if kind == "silu":
+        s = _sigmoid(z)
+        return s * (1.0 + z * (1.0 - s))
     t = np.clip(z / 6.0 + 0.5, 0.0, 1.0)
     return t * t * (3.0 - 2.0 * t) + z * t * (1.0 - t)
 
 
-def forward(params, x, matrix_cores=False):
+def _sigmoid(z):
+    # 1 / (1 + e^-z) without overflow for large |z|
+    e = np.exp(-np.abs(z))
+    return np.where(z >= 0, 1.0 / (1.0 + e), e / (1.0 + e))
+
+
+ACTIVATIONS = {0: "silu", 1: "gate"}  # THOTH_CV_SILU, THOTH_CV_GATE
+
+
+def forward(params, x, matrix_cores=False, activation="silu"):
     """params: list of (W [in,out], b [out]); x [N, 11] -> (f [N, 5], cache).
 
     matrix_cores=True rounds where the BF16 matrix-core kernels round: the
@@ -77,18 +93,18 @@ def forward(params, x, matrix_cores=False):
     for li, (W, b) in enumerate(params):
         z = h @ rnd(W) + b
         cache.append((h, z))
-        h = rnd(act(z)) if li < len(params) - 1 else z
+        h = rnd(act(z, activation)) if li < len(params) - 1 else z
     return h, cache
 
 
-def loss_and_grads(params, x, dm, payoff, matrix_cores=False):
+def loss_and_grads(params, x, dm, payoff, matrix_cores=False, activation="silu"):
     """x [B, J, 11], dm [B, J, 5], payoff [B] -> (loss, grads like params).
 
     matrix_cores=True also rounds the back-propagated error to bfloat16 where
     it enters a matrix product (weight gradient, error of the layer below);
     the bias gradients use the unrounded error, as on the device."""
     B, J, _ = x.shape
-    f, cache = forward(params, x.reshape(B * J, -1), matrix_cores)
+    f, cache = forward(params, x.reshape(B * J, -1), matrix_cores, activation)
     cv = (f.reshape(B, J, 5) * dm).sum(axis=(1, 2))
     r = payoff - cv
     loss = np.mean(r * r)
@@ -99,7 +115,7 @@ def loss_and_grads(params, x, dm, payoff, matrix_cores=False):
     for li in range(len(params) - 1, -1, -1):
         h, z = cache[li]
         if li < len(params) - 1:
-            g = g * dact(z)
+            g = g * dact(z, activation)
         W, _ = params[li]
         grads[li] = (h.T @ rnd(g), g.sum(axis=0))
         g = rnd(g) @ rnd(W).T

@@ -6,6 +6,8 @@ import sys
 from pathlib import Path
 
 import numpy as np
+import pytest
+from scipy.special import expit
 
 sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
 import mlp_oracle as mo  # noqa: E402
@@ -19,11 +21,12 @@ def sample(rng, B=8, J=3):
     return rng.normal(0, 1, (B, J, 11)), rng.normal(0, 1, (B, J, 5)), rng.normal(2, 1, B)
 
 
-def test_gradients_match_central_differences():
+@pytest.mark.parametrize("activation", ["silu", "gate"])
+def test_gradients_match_central_differences(activation):
     rng = np.random.default_rng(11)
     params = random_params(rng)
     x, dm, pay = sample(rng)
-    _, grads = mo.loss_and_grads(params, x, dm, pay)
+    _, grads = mo.loss_and_grads(params, x, dm, pay, activation=activation)
     h = 1e-6
     for li in range(4):
         for which in (0, 1):
@@ -33,9 +36,9 @@ def test_gradients_match_central_differences():
                 idx = tuple(rng.integers(0, s) for s in p.shape)
                 old = p[idx]
                 p[idx] = old + h
-                lp, _ = mo.loss_and_grads(params, x, dm, pay)
+                lp, _ = mo.loss_and_grads(params, x, dm, pay, activation=activation)
                 p[idx] = old - h
-                lm, _ = mo.loss_and_grads(params, x, dm, pay)
+                lm, _ = mo.loss_and_grads(params, x, dm, pay, activation=activation)
                 p[idx] = old
                 fd = (lp - lm) / (2 * h)
                 assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), (li, which, idx, fd, g[idx])
@@ -97,6 +100,21 @@ def test_matrix_core_emulation_is_close_to_exact():
     assert abs(l1 - l0) <= 2e-2 * l0
     a, b = mo.pack(g0), mo.pack(g1)
     assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(a)
+
+
+def test_silu_closed_forms():
+    # SiLU a(z) = z sigma(z) with scipy's logistic function; a(0) = 0, a'(0) = 1/2,
+    # a(z) -> z, a'(z) -> 1 for large z and -> 0 for very negative z (no overflow)
+    z = np.concatenate([np.linspace(-40, 40, 8001), [-800.0, 800.0]])
+    assert np.allclose(mo.act(z, "silu"), z * expit(z), rtol=1e-15, atol=1e-300)
+    assert mo.act(np.array([0.0]), "silu")[0] == 0.0 and mo.dact(np.array([0.0]), "silu")[0] == 0.5
+    assert mo.act(np.array([800.0]), "silu")[0] == 800.0 and mo.dact(np.array([800.0]), "silu")[0] == 1.0
+    assert abs(mo.act(np.array([-800.0]), "silu")[0]) < 1e-300 and mo.dact(np.array([-800.0]), "silu")[0] == 0.0
+    # the derivative against central differences, and a(z) - a(-z) = z (sigma(z) + sigma(-z) = 1)
+    zz = np.linspace(-8, 8, 161)
+    fd = (mo.act(zz + 1e-6, "silu") - mo.act(zz - 1e-6, "silu")) / 2e-6
+    assert np.allclose(fd, mo.dact(zz, "silu"), rtol=0, atol=1e-8)
+    assert np.allclose(mo.act(zz, "silu") - mo.act(-zz, "silu"), zz, rtol=0, atol=1e-14)
 
 
 def test_activation_closed_forms():

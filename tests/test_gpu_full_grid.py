@@ -1,0 +1,141 @@
+"""FP64 device parity against the CPU oracle on the workloads' full grids (at
+path counts the oracle finishes in seconds):
+
+* the 5-asset LSV worst-of basket: 5 LSV equities, 365 two-day steps over 2y,
+  the 100 x 50 leverage surface, premium, delta and vega (every bump scenario
+  recalibrated), in the reference's uncorrelated mode and in the correlated
+  (Cholesky) mode;
+* the vanilla grids: all 5 underlyings, 12 monthly maturities x 41 strikes,
+  calls and puts, per-cell premium, trust, delta and vega, against the oracle
+  pricing every cell as a `!vanilla` contract;
+* the Heston down-and-out call on its 252-step uniform grid with bump Greeks,
+  and the pathwise kernel on that grid: its delta equal to the central
+  difference of its own estimator, its Greeks within their bias of the bump
+  Greeks.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from thoth_amd import abi, configs
+from thoth_amd.book import BookSpec, BuiltRequest, McConfig, Vanilla
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_lsv_basket_full_grid_fp64(gpu, oracle):
+    spec = configs.lsv_worst_of_basket(paths=1 << 12, precision=abi.FP64,
+                                       greeks=abi.GREEK_DELTA | abi.GREEK_VEGA)
+    b = BuiltRequest(spec)
+    days, t, _ = gpu.grid(b)
+    assert len(t) == 366 and days[-1] == 730
+    g = gpu.price(b)
+    o = oracle.price(b)
+    print("premium %.12f / %.12f, delta %.12f / %.12f, vega %.12f / %.12f"
+          % (g.premium, o.premium, g.delta, o.delta, g.vega, o.vega))
+    assert rel(g.premium, o.premium) < REL
+    assert rel(g.premium_trust, o.premium_trust) < 1e-9
+    # Greeks are premium differences over the bump: premium-scaled bounds
+    for name in ("delta", "vega"):
+        assert abs(getattr(g, name) - getattr(o, name)) <= REL * o.premium * 10, name
+
+
+def test_lsv_basket_full_grid_correlated_fp64(gpu, oracle):
+    spec = configs.lsv_worst_of_basket(paths=1 << 12, precision=abi.FP64, correlate_stochastic=True)
+    b = BuiltRequest(spec)
+    g = gpu.price(b)
+    o = oracle.price(b)
+    print("correlated: premium %.12f / %.12f" % (g.premium, o.premium))
+    assert rel(g.premium, o.premium) < REL
+    # the correlation moves the worst-of: not the uncorrelated premium
+    u = oracle.price(BuiltRequest(configs.lsv_worst_of_basket(paths=1 << 12, precision=abi.FP64)))
+    assert abs(u.premium - o.premium) > 10 * REL * o.premium
+
+
+def test_vanilla_grids_full_fp64(gpu, oracle):
+    books = configs.lsv_vanilla_grids(paths=1 << 12, precision=abi.FP64)
+    plan = gpu.GridPlan([BuiltRequest(s) for s, _ in books], [gpu.vanilla_grid(**g) for _, g in books],
+                        autotune=False)
+    res, _ = plan.run()
+    worst = 0.0
+    for (spec, grid), r in zip(books, res):
+        eq = spec.equities[0]
+        mats, pcts = grid["maturity_days"], grid["strikes"]
+        assert len(mats) == 12 and len(pcts) == 41
+        contracts = []
+        for d in mats:
+            for p in pcts:
+                k = p * eq.spot / 100.0
+                contracts += [Vanilla(0, d, k, is_call=True), Vanilla(0, d, k, is_call=False)]
+        ref = BookSpec(mcl=spec.mcl, equities=[eq], contracts=contracts, rate_days=spec.rate_days,
+                       rate_values=spec.rate_values, greeks=spec.greeks, precision=abi.FP64,
+                       leverage_surface=spec.leverage_surface)
+        o = oracle.price(BuiltRequest(ref))
+        for j in range(len(mats)):
+            for k in range(len(pcts)):
+                for put, name in ((0, "call"), (1, "put")):
+                    c = 2 * (j * len(pcts) + k) + put
+                    P = abs(o.contract_premium[c]) + 1e-10 * eq.spot
+                    for key, ok in (("", o.contract_premium[c]), ("_trust", o.contract_trust[c])):
+                        gv = r[name + key][j, k]
+                        # deep out-of-the-money cells: absolute floor at 1e-10 of the spot
+                        err = abs(gv - ok) / max(abs(ok), 1e-10 * eq.spot)
+                        worst = max(worst, err)
+                        assert err <= 1e-9, (name + key, j, k, gv, ok)
+                    # bump Greeks are premium differences over the bump (delta: 1 % of the
+                    # spot, vega: one vol point): their rounding scales with the premium
+                    for key, ok, h in (("_delta", o.contract_delta[c], 0.01 * eq.spot),
+                                       ("_vega", o.contract_vega[c], 1.0)):
+                        gv = r[name + key][j, k]
+                        assert abs(gv - ok) <= 1e-9 * P / h, (name + key, j, k, gv, ok)
+    print("5 x 984 cells: worst relative premium / trust difference to the oracle %.2e" % worst)
+
+
+def test_heston_252_step_grid_with_greeks_fp64(gpu, oracle):
+    spec = configs.heston_down_out_call(paths=1 << 12, precision=abi.FP64, uniform_steps=252,
+                                        greeks=abi.GREEK_DELTA | abi.GREEK_GAMMA | abi.GREEK_VEGA)
+    b = BuiltRequest(spec)
+    _, t, _ = gpu.grid(b)
+    assert len(t) == 253 and abs(t[1] - 1 / 252) < 1e-15
+    g = gpu.price(b)
+    o = oracle.price(b)
+    assert rel(g.premium, o.premium) < REL
+    for name, h in (("delta", 1.0), ("gamma", 1.0), ("vega", 1.0)):
+        assert abs(getattr(g, name) - getattr(o, name)) <= REL * o.premium / h * 10, name
+
+
+def _heston252(paths, spot=100.0, vol=20.0, precision=abi.FP64):
+    spec = configs.heston_down_out_call(paths=paths, precision=precision, uniform_steps=252)
+    spec.equities[0].spot = spot
+    spec.equities[0].init_vol = vol
+    spec.equities[0].long_vol = vol
+    return BuiltRequest(spec)
+
+
+def test_pathwise_on_252_step_grid(gpu):
+    n = 1 << 16
+    base = gpu.price_pathwise(_heston252(n))
+    h = 1e-3
+    fd = (gpu.price_pathwise(_heston252(n, spot=100 + h)).premium
+          - gpu.price_pathwise(_heston252(n, spot=100 - h)).premium) / (2 * h)
+    print("252 steps: pathwise delta %.10f, central difference of its estimator %.10f" % (base.delta, fd))
+    assert abs(base.delta - fd) <= 1e-5 * abs(fd)
+    # against the bump Greeks of the plain engine (finite-difference bias of the 1 % bump)
+    n2 = 1 << 18
+    pw = gpu.price_pathwise(_heston252(n2))
+    spec = configs.heston_down_out_call(paths=n2, precision=abi.FP64, uniform_steps=252,
+                                        greeks=abi.GREEK_DELTA | abi.GREEK_VEGA)
+    bump = gpu.price(BuiltRequest(spec))
+    print("premium %.6f +- %.1e / plain %.6f +- %.1e; delta %.6f / %.6f; vega %.6f / %.6f"
+          % (pw.premium, pw.premium_trust, bump.premium, bump.premium_trust, pw.delta, bump.delta, pw.vega,
+             bump.vega))
+    assert abs(pw.premium - bump.premium) <= 4 * math.hypot(pw.premium_trust, bump.premium_trust)
+    assert abs(pw.delta - bump.delta) <= 4 * pw.delta_trust + 0.02 * abs(bump.delta)
+    assert abs(pw.vega - bump.vega) <= 4 * pw.vega_trust + 0.05 * abs(bump.vega)

@@ -116,10 +116,12 @@ class CvWeights(C.Structure):
 class CvConfig(C.Structure):
     _fields_ = [("hedge_dates", _i32), ("train_paths", _i64), ("train_sobol_skip", _i64),
                 ("adam_steps", _i32), ("batch_paths", _i32), ("learning_rate", C.c_float),
-                ("init_seed", C.c_uint64), ("eval_mode", _i32), ("hedge_spacing", C.c_float)]
+                ("init_seed", C.c_uint64), ("eval_mode", _i32), ("hedge_spacing", C.c_float),
+                ("activation", _i32)]
 
 
 CV_IN_LOOP, CV_TWO_PHASE = 0, 1
+CV_SILU, CV_GATE = 0, 1
 
 
 class CvTrainReport(C.Structure):
@@ -129,7 +131,8 @@ class CvTrainReport(C.Structure):
 class CvResult(C.Structure):
     _fields_ = [("replicates", _i32), ("paths", _i64), ("plain_mean", _d), ("plain_halfwidth", _d),
                 ("cv_mean", _d), ("cv_halfwidth", _d), ("plain_var", _d), ("cv_var", _d),
-                ("sweep_ms", _d), ("eval_ms", _d)]
+                ("sweep_ms", _d), ("eval_ms", _d), ("cv_term_mean", _d), ("cv_term_halfwidth", _d),
+                ("replicate_plain_means", C.POINTER(_d)), ("replicate_cv_means", C.POINTER(_d))]
 
 
 class RqmcResult(C.Structure):

@@ -13,6 +13,7 @@
 #include "mcl_device.hpp"
 #include "mcl_internal.hpp"
 #include "mcl_launch.hpp"
+#include "thoth_mcl_cv.h"
 
 namespace thoth
 {
@@ -463,6 +464,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             if ( rc != THOTH_OK )
                 return rc;
             B.cv_mode = cv->mode;
+            B.cv_act = cv->act;
         }
         if ( grids && grids[b].nm > 0 )
         {
@@ -664,14 +666,16 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
     HIP_OK( hipEventCreate( &e1 ) );
     HIP_OK( hipEventRecord( e0, s ) );
     const dim3 grid( B.nblocks, B.n_books, B.replicates );
+    // kernel control-variate mode: in-loop inference of a gated network is mode 3
+    const int kcv = B.cv_mode == 2 && B.cv_act == THOTH_CV_GATE ? 3 : B.cv_mode;
     bool ok = B.precision == 0
                   ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
                                       B.first, B.last, B.dshifts )
                   : ( B.kind && launch_sweep_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
                                                           DeviceDirections(), B.q0, B.first, B.last, B.dshifts,
-                                                          B.cv_mode ) ) ||
+                                                          kcv ) ) ||
                         launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                          B.first, B.last, B.dshifts, B.cv_mode );
+                                          B.first, B.last, B.dshifts, kcv );
     if ( !ok )
     {
         (void)hipEventDestroy( e0 );

@@ -16,9 +16,12 @@
 
 #include "mcl_launch.hpp"
 #include "mcl_mlp.hpp"
+#include "thoth_mcl_cv.h"
 
 namespace thoth
 {
+
+static_assert( THOTH_CV_SILU == kActSilu && THOTH_CV_GATE == kActGate, "activation codes of the C-ABI" );
 
 namespace
 {
@@ -74,6 +77,7 @@ __global__ void mlp_pack_kernel( const float* __restrict__ w, uint16_t* __restri
 // Forward of every sample: per path Y = P - CV; block partial sums
 // (sum Y, sum Y^2, sum P, sum P^2) in FP64 for each replicate.
 // One wave = 16 paths (one sample column tile), looping over the J dates.
+template <int ACT>
 __global__ void __launch_bounds__( 256 ) cv_eval_kernel( const uint2* __restrict__ wimg, const float* __restrict__ bias,
                                                         const uint16_t* __restrict__ feat, const float* __restrict__ dm,
                                                         const float* __restrict__ pay, int64_t N, int J,
@@ -89,7 +93,7 @@ __global__ void __launch_bounds__( 256 ) cv_eval_kernel( const uint2* __restrict
     for ( int j = 0; j < J; j++ )
     {
         const size_t row = ( (size_t)r * J + j ) * N + pp;
-        cv += mlp_dot( mlp_infer( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
+        cv += mlp_dot( mlp_infer<ACT>( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
     }
     const float P = pay[(size_t)r * N + pp];
     double y = ( valid && lane < 16 ) ? (double)P - (double)cv : 0.0;
@@ -119,6 +123,7 @@ __global__ void __launch_bounds__( 256 ) cv_eval_kernel( const uint2* __restrict
 // chunks.  Wave (tile, chunk) takes 16 paths of the minibatch
 // [ (step B) mod N, + B ) and the dates of its chunk; cvpart[chunk][b] =
 // sum over those dates of f(x) . dm.
+template <int ACT>
 __global__ void __launch_bounds__( 256 ) cv_part_kernel( const uint2* __restrict__ wimg, const float* __restrict__ bias,
                                                         const uint16_t* __restrict__ feat, const float* __restrict__ dm,
                                                         int64_t N, int J, int B, int dpc, const int* __restrict__ step,
@@ -136,7 +141,7 @@ __global__ void __launch_bounds__( 256 ) cv_part_kernel( const uint2* __restrict
     for ( int j = j0; j < j1; j++ )
     {
         const size_t row = (size_t)j * N + p;
-        cv += mlp_dot( mlp_infer( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
+        cv += mlp_dot( mlp_infer<ACT>( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
     }
     if ( lane < 16 )
         cvpart[(size_t)chunk * B + b] = cv;
@@ -153,6 +158,7 @@ constexpr int kGradGroups = kCvGradGroups;  // first-level groups of the gradien
 constexpr size_t kCvGradLds = 4 * sizeof( MlpActLds );
 static_assert( kCvGradLds >= kGradN * sizeof( float ), "gradient reduction fits in the activation area" );
 
+template <int ACT>
 __global__ void __launch_bounds__( 256 ) cv_grad_kernel( const uint2* __restrict__ wimg, const uint2* __restrict__ wbwd,
                                                         const float* __restrict__ bias, const uint16_t* __restrict__ feat,
                                                         const float* __restrict__ dm, const float* __restrict__ pay,
@@ -196,8 +202,8 @@ __global__ void __launch_bounds__( 256 ) cv_grad_kernel( const uint2* __restrict
             fx = load_feat( feat, (size_t)( j + 1 ) * N + p, lane );
             dx = load_dm( dm, (size_t)( j + 1 ) * N + p, lane );
         }
-        (void)mlp_forward_lds( wimg, bias, f, lane, act );
-        mlp_backward_lds( wbwd, act, f32x4{ g * d[0], g * d[1], g * d[2], g * d[3] }, lane, G );
+        (void)mlp_forward_lds<ACT>( wimg, bias, f, lane, act );
+        mlp_backward_lds<ACT>( wbwd, act, f32x4{ g * d[0], g * d[1], g * d[2], g * d[3] }, lane, G );
     }
     // bias partials: sum the 16 sample columns of each lane group
     constexpr int kBiasTile[4] = { 0, 4, 8, 12 };
@@ -311,18 +317,26 @@ void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bw
     hipLaunchKernelGGL( mlp_pack_kernel, dim3( ( kMlpWSize + 255 ) / 256 ), dim3( 256 ), 0, s, w, fwd, bwd );
 }
 
-void launch_cv_eval( hipStream_t s, const uint16_t* fwd, const float* bias, const uint16_t* feat, const float* dm,
-                     const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks )
+void launch_cv_eval( hipStream_t s, int act, const uint16_t* fwd, const float* bias, const uint16_t* feat,
+                     const float* dm, const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks )
 {
     const uint32_t nb = (uint32_t)( ( N + 63 ) / 64 );
     *nblocks = nb;
-    hipLaunchKernelGGL( cv_eval_kernel, dim3( nb, R ), dim3( 256 ), 0, s, reinterpret_cast<const uint2*>( fwd ), bias,
-                        feat, dm, pay, N, J, partial );
+    if ( act == kActGate )
+        hipLaunchKernelGGL( cv_eval_kernel<kActGate>, dim3( nb, R ), dim3( 256 ), 0, s,
+                            reinterpret_cast<const uint2*>( fwd ), bias, feat, dm, pay, N, J, partial );
+    else
+        hipLaunchKernelGGL( cv_eval_kernel<kActSilu>, dim3( nb, R ), dim3( 256 ), 0, s,
+                            reinterpret_cast<const uint2*>( fwd ), bias, feat, dm, pay, N, J, partial );
 }
 
 hipError_t cv_train_init()
 {
-    return hipFuncSetAttribute( reinterpret_cast<const void*>( cv_grad_kernel ),
+    hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( cv_grad_kernel<kActSilu> ),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCvGradLds );
+    if ( e != hipSuccess )
+        return e;
+    return hipFuncSetAttribute( reinterpret_cast<const void*>( cv_grad_kernel<kActGate> ),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCvGradLds );
 }
 
@@ -341,17 +355,29 @@ CvTrainLayout cv_train_layout( int B, int J )
     return l;
 }
 
-bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd, float* w,
-                           float* m, float* v, const uint16_t* feat, const float* dm, const float* pay, int64_t N,
-                           int* step, float* cvpart, float* gpart, float* gsum, float* loss_part, float lr )
+template <int ACT>
+static void launch_cv_forward_backward( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd,
+                                        const uint16_t* bwd, const float* w, const uint16_t* feat, const float* dm,
+                                        const float* pay, int64_t N, const int* step, float* cvpart, float* gpart,
+                                        float* loss_part )
+{
+    hipLaunchKernelGGL( cv_part_kernel<ACT>, dim3( l.blocks ), dim3( 256 ), 0, s, reinterpret_cast<const uint2*>( fwd ),
+                        w + kMlpWSize, feat, dm, N, l.J, l.B, l.dpc, step, cvpart );
+    hipLaunchKernelGGL( cv_grad_kernel<ACT>, dim3( l.blocks ), dim3( 256 ), kCvGradLds, s,
+                        reinterpret_cast<const uint2*>( fwd ), reinterpret_cast<const uint2*>( bwd ), w + kMlpWSize,
+                        feat, dm, pay, N, l.J, l.B, l.dpc, l.chunks, step, cvpart, gpart, loss_part );
+}
+
+bool launch_cv_train_step( hipStream_t s, int act, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd,
+                           float* w, float* m, float* v, const uint16_t* feat, const float* dm, const float* pay,
+                           int64_t N, int* step, float* cvpart, float* gpart, float* gsum, float* loss_part, float lr )
 {
     if ( l.blocks < 1 || l.blocks * 4 != l.waves )
         return false;
-    hipLaunchKernelGGL( cv_part_kernel, dim3( l.blocks ), dim3( 256 ), 0, s, reinterpret_cast<const uint2*>( fwd ),
-                        w + kMlpWSize, feat, dm, N, l.J, l.B, l.dpc, step, cvpart );
-    hipLaunchKernelGGL( cv_grad_kernel, dim3( l.blocks ), dim3( 256 ), kCvGradLds, s,
-                        reinterpret_cast<const uint2*>( fwd ), reinterpret_cast<const uint2*>( bwd ), w + kMlpWSize,
-                        feat, dm, pay, N, l.J, l.B, l.dpc, l.chunks, step, cvpart, gpart, loss_part );
+    if ( act == kActGate )
+        launch_cv_forward_backward<kActGate>( s, l, fwd, bwd, w, feat, dm, pay, N, step, cvpart, gpart, loss_part );
+    else
+        launch_cv_forward_backward<kActSilu>( s, l, fwd, bwd, w, feat, dm, pay, N, step, cvpart, gpart, loss_part );
     hipLaunchKernelGGL( cv_gsum_kernel, dim3( ( kGradN + 255 ) / 256, kGradGroups ), dim3( 256 ), 0, s, gpart, l.blocks,
                         gsum );
     hipLaunchKernelGGL( cv_adam_kernel, dim3( ( kGradN + 255 ) / 256 ), dim3( 256 ), 0, s, w, m, v, gsum, step, lr,

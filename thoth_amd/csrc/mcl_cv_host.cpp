@@ -79,7 +79,8 @@ int CheckConfig( const thoth_cv_config& c )
 {
     if ( c.hedge_dates < 1 || !( c.hedge_spacing >= 1.0f && c.hedge_spacing <= 8.0f ) || c.batch_paths < 64 ||
          c.batch_paths % 64 || c.adam_steps < 0 || c.train_paths < c.batch_paths ||
-         c.train_sobol_skip < 0 || !( c.learning_rate > 0 ) )
+         c.train_sobol_skip < 0 || !( c.learning_rate > 0 ) ||
+         ( c.activation != THOTH_CV_SILU && c.activation != THOTH_CV_GATE ) )
         return SetError( THOTH_ERR_INVALID, "control-variate config out of range" );
     return THOTH_OK;
 }
@@ -191,7 +192,8 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_train( cons
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIP_OK( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
-        const bool launched = launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, cv.feat, cv.dm, cv.pay, N, step,
+        const bool launched = launch_cv_train_step( s, cfg->activation, lay, fwd, bwd, dw, dm_, dv, cv.feat, cv.dm,
+                                                    cv.pay, N, step,
                                                     cvpart, gpart, gsum, loss_part, cfg->learning_rate );
         HIP_OK( hipStreamEndCapture( s, &graph ) );
         if ( !launched )
@@ -282,6 +284,18 @@ void FinishCi( const std::vector<double>& sums, int R, int64_t N, thoth_cv_resul
     };
     ci( my, res->cv_mean, res->cv_halfwidth );
     ci( mp, res->plain_mean, res->plain_halfwidth );
+    // paired: plain - (P - CV) per replicate is the replicate's mean of CV
+    std::vector<double> md( R );
+    for ( int r = 0; r < R; r++ )
+        md[r] = mp[r] - my[r];
+    ci( md, res->cv_term_mean, res->cv_term_halfwidth );
+    for ( int r = 0; r < R; r++ )
+    {
+        if ( res->replicate_plain_means )
+            res->replicate_plain_means[r] = mp[r];
+        if ( res->replicate_cv_means )
+            res->replicate_cv_means[r] = my[r];
+    }
     const double n = (double)N * R;
     res->cv_var = ( sy2 - sy * sy / n ) / ( n - 1 );
     res->plain_var = ( sp2 - sp * sp / n ) / ( n - 1 );
@@ -309,6 +323,7 @@ int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const
     launch_mlp_pack( s, dw, fwd, bwd );
     CvSpec cv;
     cv.mode = 2;
+    cv.act = cfg.activation;
     cv.J = cfg.hedge_dates;
     cv.spacing = cfg.hedge_spacing;
     cv.N = req.mcl.paths;
@@ -368,7 +383,7 @@ int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, con
     HIP_OK( hipEventCreate( &e1 ) );
     HIP_OK( hipEventRecord( e0, s ) );
     uint32_t nbl = 0;
-    launch_cv_eval( s, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part, &nbl );
+    launch_cv_eval( s, cfg.activation, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part, &nbl );
     HIP_OK( hipGetLastError() );
     HIP_OK( hipEventRecord( e1, s ) );
     std::vector<double> hp( (size_t)R * nb * 4 );
@@ -437,10 +452,13 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_price( cons
 extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_loss_grad( const thoth_cv_weights* w,
                                                                             const uint16_t* feat, const float* dm,
                                                                             const float* pay, int32_t B, int32_t J,
-                                                                            double* loss, float* grad )
+                                                                            int32_t activation, double* loss,
+                                                                            float* grad )
 {
     if ( !w || !feat || !dm || !pay || B < 64 || B % 64 || J < 1 )
         return SetError( THOTH_ERR_INVALID, "B must be a positive multiple of 64" );
+    if ( activation != THOTH_CV_SILU && activation != THOTH_CV_GATE )
+        return SetError( THOTH_ERR_INVALID, "unknown activation" );
     std::lock_guard<std::mutex> lk( DeviceMutex() );
     int rc = EnsureDevice();
     if ( rc != THOTH_OK )
@@ -474,7 +492,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_cv_loss_grad( 
     HIP_OK( hipMemsetAsync( step, 0, sizeof( int ), s ) );
     launch_mlp_pack( s, dw, fwd, bwd );
     // gradient only: the training step with a zero learning rate leaves w unchanged
-    if ( !launch_cv_train_step( s, lay, fwd, bwd, dw, dm_, dv, dfeat, ddm, dpay, B, step, cvpart, gpart, gsum,
+    if ( !launch_cv_train_step( s, activation, lay, fwd, bwd, dw, dm_, dv, dfeat, ddm, dpay, B, step, cvpart, gpart, gsum,
                                 loss_part, 0.0f ) )
         return SetError( THOTH_ERR_INVALID, "batch layout not supported" );
     HIP_OK( hipGetLastError() );

@@ -309,6 +309,32 @@ __attribute__( ( visibility( "default" ) ) ) int thoth_gpu_sobol_points( int32_t
     return THOTH_OK;
 }
 
+__attribute__( ( visibility( "default" ) ) ) int thoth_gpu_special_f32( int32_t fn, int64_t n, const uint32_t* in,
+                                                                       float* out )
+{
+    std::lock_guard<std::mutex> lk( DeviceMutex() );
+    int rc = EnsureDevice();
+    if ( rc != THOTH_OK )
+        return rc;
+    if ( fn < 0 || fn > 2 || n < 1 || n > ( (int64_t)1 << 28 ) || !in || !out )
+        return Fail( THOTH_ERR_INVALID, "fn in [0, 2], n in [1, 2^28], non-null buffers" );
+    char* d = nullptr;
+    const size_t bytes = (size_t)n * sizeof( uint32_t );
+    HIP_OK( hipMalloc( &d, 2 * bytes ) );
+    uint32_t* d_in = reinterpret_cast<uint32_t*>( d );
+    float* d_out = reinterpret_cast<float*>( d + bytes );
+    hipError_t e = hipMemcpyAsync( d_in, in, bytes, hipMemcpyHostToDevice, g_dev.stream );
+    const bool ok = e == hipSuccess && launch_special_f32( g_dev.stream, fn, n, d_in, d_out );
+    if ( ok )
+        e = hipMemcpyAsync( out, d_out, bytes, hipMemcpyDeviceToHost, g_dev.stream );
+    if ( ok && e == hipSuccess )
+        e = hipStreamSynchronize( g_dev.stream );
+    (void)hipFree( d );
+    if ( !ok || e != hipSuccess )
+        return Fail( THOTH_ERR_DEVICE, "special-function kernel failed" );
+    return THOTH_OK;
+}
+
 __attribute__( ( visibility( "default" ) ) ) int thoth_gpu_price_european_gbm( double forward, double strike, double t, double vol, double df, int is_call,
                                   long long paths, uint64_t seed, double* premium, double* trust )
 {

@@ -50,6 +50,7 @@ struct DeviceBatch
     int n_books = 0, depth = 1, na_max = 1, precision = 0;
     int kind = 0;  // uniform-book kernel: 1 all LSV, 2 all Heston (see sweep_block), 0 generic
     int cv_mode = 0;  // 0 none, 1 control-variate samples, 2 in-loop hedge inference
+    int cv_act = 0;   // hidden activation of the in-loop network: THOTH_CV_SILU or THOTH_CV_GATE
     bool pathwise = false;  // pathwise-Greek sweep: three estimators per block
     // vanilla grids: per-(replicate, block) partials and their per-book sums
     // [nm][2 nk][2] (device), offsets per book; grid_cells[b] = nm * 2 nk
@@ -80,6 +81,7 @@ struct DeviceBatch
 struct CvSpec
 {
     int mode = 1;
+    int act = 0;  // hidden activation of the network (mode 2): THOTH_CV_SILU or THOTH_CV_GATE
     int J = 0;
     float spacing = 1.0f;  // hedge date j at step m - round(m (1 - j/J)^spacing)
     int64_t N = 0;

@@ -41,6 +41,8 @@ namespace thoth
 constexpr int kMaxStackDepth = 16;
 bool launch_sweep_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
+// cv_mode: the kernel's control-variate mode (0 none, 1 training samples,
+// 2 in-loop inference of a SiLU network, 3 of a smoothstep-gated network)
 bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts,
                        int cv_mode = 0 );
@@ -58,8 +60,9 @@ bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipS
                                     const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first,
                                     uint32_t last, const int2* items, int n_items, uint32_t ntiles, int* counter );
 void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bwd );
-void launch_cv_eval( hipStream_t s, const uint16_t* fwd, const float* bias, const uint16_t* feat, const float* dm,
-                     const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks );
+// act: the hidden activation of the network (kActSilu 0, kActGate 1, mcl_mlp.hpp)
+void launch_cv_eval( hipStream_t s, int act, const uint16_t* fwd, const float* bias, const uint16_t* feat,
+                     const float* dm, const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks );
 // one Adam step of the hedge network: minibatch of B paths x J dates split
 // into `chunks` date chunks of dpc dates; one gradient slice per block
 struct CvTrainLayout
@@ -71,9 +74,9 @@ hipError_t cv_train_init();  // once per process before the first step (LDS limi
 // buffers: cvpart [chunks * B], gpart [blocks * THOTH_CV_WEIGHTS], gsum [kCvGradGroups * THOTH_CV_WEIGHTS],
 // loss_part [blocks]
 constexpr int kCvGradGroups = 32;
-bool launch_cv_train_step( hipStream_t s, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd, float* w,
-                           float* m, float* v, const uint16_t* feat, const float* dm, const float* pay, int64_t N,
-                           int* step, float* cvpart, float* gpart, float* gsum, float* loss_part, float lr );
+bool launch_cv_train_step( hipStream_t s, int act, const CvTrainLayout& l, const uint16_t* fwd, const uint16_t* bwd,
+                           float* w, float* m, float* v, const uint16_t* feat, const float* dm, const float* pay,
+                           int64_t N, int* step, float* cvpart, float* gpart, float* gsum, float* loss_part, float lr );
 bool launch_persistent_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                            const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
                            int n_items, uint32_t ntiles, int* counter );
@@ -89,4 +92,5 @@ void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slo
 void launch_grid_sum( hipStream_t s, const double* part, const long long* table, int n_books, int max_per_slot,
                       int nslots, double* out );
 bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t first, uint32_t last, uint32_t* out );
+bool launch_special_f32( hipStream_t s, int fn, int64_t n, const uint32_t* in, float* out );
 }  // namespace thoth

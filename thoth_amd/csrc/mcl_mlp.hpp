@@ -1,7 +1,7 @@
 // mcl_mlp.hpp -- the hedge-ratio MLP on CDNA4 matrix cores (no reference
 // counterpart: the reference engine has no control variates).
 //
-//   f(x) = W4 s(W3 s(W2 s(W1 x + b1) + b2) + b3) + b4,   s = smoothstep-gated linear unit,
+//   f(x) = W4 s(W3 s(W2 s(W1 x + b1) + b2) + b3) + b4,   s = SiLU (or the smoothstep-gated linear unit),
 //   x: 11 features (5 log-moneyness, 5 scaled vols, t/T) padded to 16,
 //   hidden width 64, 5 outputs (hedge ratios) padded to 16.
 //
@@ -97,6 +97,52 @@ __device__ __forceinline__ float gate_dact( float z )
     return fmaf( z * t, 1.0f - t, t * t * fmaf( -2.0f, t, 3.0f ) );
 }
 
+// SiLU a(z) = z / (1 + e^-z), a'(z) = s (1 + z (1 - s)) with s = 1 / (1 + e^-z):
+// one exponential and one reciprocal per unit (e^-z = inf for z < -88 gives
+// s = 0, a = -0, a' = 0)
+__device__ __forceinline__ float silu_s( float z ) { return __builtin_amdgcn_rcpf( 1.0f + __expf( -z ) ); }
+__device__ __forceinline__ float silu_act( float z ) { return z * silu_s( z ); }
+__device__ __forceinline__ float silu_dact( float z )
+{
+    const float s = silu_s( z );
+    return fmaf( z * s, 1.0f - s, s );
+}
+
+// hidden activation of a network: ACT = kActSilu or kActGate
+enum
+{
+    kActSilu = 0,
+    kActGate = 1
+};
+template <int ACT>
+__device__ __forceinline__ float act_f( float z )
+{
+    if constexpr ( ACT == kActGate )
+        return gate_act( z );
+    else
+        return silu_act( z );
+}
+template <int ACT>
+__device__ __forceinline__ float dact_f( float z )
+{
+    if constexpr ( ACT == kActGate )
+        return gate_dact( z );
+    else
+        return silu_dact( z );
+}
+// a(c) of a D-layout tile packed as the next layer's BF16 B operand
+template <int ACT>
+__device__ __forceinline__ bf16x4 act_pack( f32x4 c )
+{
+    return pack_bf16( f32x4{ act_f<ACT>( c[0] ), act_f<ACT>( c[1] ), act_f<ACT>( c[2] ), act_f<ACT>( c[3] ) } );
+}
+template <int ACT>
+__device__ __forceinline__ f32x4 dact_mul( f32x4 c, f32x4 z )
+{
+    return f32x4{ c[0] * dact_f<ACT>( z[0] ), c[1] * dact_f<ACT>( z[1] ), c[2] * dact_f<ACT>( z[2] ),
+                  c[3] * dact_f<ACT>( z[3] ) };
+}
+
 // A operand of tile (layer L, out-tile ot, k-tile kt) from a swizzled image:
 // image[(tile * 64 + lane)] holds the lane's 4 BF16 values.
 __device__ __forceinline__ bf16x4 weight_frag( const uint2* img, int L, int ot, int kt, int lane )
@@ -127,6 +173,7 @@ struct MlpTile
 
 // forward of one tile; x: B operand of the padded features (lane: features
 // 4 (lane/16) .. +3 of sample lane % 16)
+template <int ACT>
 __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bias, bf16x4 x, int lane, MlpTile& t )
 {
 #pragma unroll
@@ -135,7 +182,7 @@ __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bia
         f32x4 c = bias_frag( bias, 0, ot, lane );
         c = mfma16( weight_frag( wimg, 0, ot, 0, lane ), x, c );
         t.z[0][ot] = c;
-        t.h[0][ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
+        t.h[0][ot] = act_pack<ACT>( c );
     }
 #pragma unroll
     for ( int L = 1; L < 3; L++ )
@@ -148,7 +195,7 @@ __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bia
             for ( int kt = 0; kt < 4; kt++ )
                 c = mfma16( weight_frag( wimg, L, ot, kt, lane ), t.h[L - 1][kt], c );
             t.z[L][ot] = c;
-            t.h[L][ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
+            t.h[L][ot] = act_pack<ACT>( c );
         }
     }
     f32x4 c = bias_frag( bias, 3, 0, lane );
@@ -159,6 +206,7 @@ __device__ __forceinline__ void mlp_forward( const uint2* wimg, const float* bia
 }
 
 // forward of one tile keeping nothing for a backward pass: the output tile
+template <int ACT>
 __device__ __forceinline__ f32x4 mlp_infer( const uint2* wimg, const float* bias, bf16x4 x, int lane )
 {
     bf16x4 h0, h1, h2, h3;
@@ -167,10 +215,10 @@ __device__ __forceinline__ f32x4 mlp_infer( const uint2* wimg, const float* bias
         f32x4 c1 = mfma16( weight_frag( wimg, 0, 1, 0, lane ), x, bias_frag( bias, 0, 1, lane ) );
         f32x4 c2 = mfma16( weight_frag( wimg, 0, 2, 0, lane ), x, bias_frag( bias, 0, 2, lane ) );
         f32x4 c3 = mfma16( weight_frag( wimg, 0, 3, 0, lane ), x, bias_frag( bias, 0, 3, lane ) );
-        h0 = pack_bf16( f32x4{ gate_act( c0[0] ), gate_act( c0[1] ), gate_act( c0[2] ), gate_act( c0[3] ) } );
-        h1 = pack_bf16( f32x4{ gate_act( c1[0] ), gate_act( c1[1] ), gate_act( c1[2] ), gate_act( c1[3] ) } );
-        h2 = pack_bf16( f32x4{ gate_act( c2[0] ), gate_act( c2[1] ), gate_act( c2[2] ), gate_act( c2[3] ) } );
-        h3 = pack_bf16( f32x4{ gate_act( c3[0] ), gate_act( c3[1] ), gate_act( c3[2] ), gate_act( c3[3] ) } );
+        h0 = act_pack<ACT>( c0 );
+        h1 = act_pack<ACT>( c1 );
+        h2 = act_pack<ACT>( c2 );
+        h3 = act_pack<ACT>( c3 );
     }
     // hidden layers 2 and 3: one output tile at a time (fewest live registers)
     auto unit_tile = [&]( int L, int ot ) __attribute__( ( always_inline ) ) -> bf16x4
@@ -180,7 +228,7 @@ __device__ __forceinline__ f32x4 mlp_infer( const uint2* wimg, const float* bias
         c = mfma16( weight_frag( wimg, L, ot, 1, lane ), h1, c );
         c = mfma16( weight_frag( wimg, L, ot, 2, lane ), h2, c );
         c = mfma16( weight_frag( wimg, L, ot, 3, lane ), h3, c );
-        return pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
+        return act_pack<ACT>( c );
     };
 #pragma unroll
     for ( int L = 1; L < 3; L++ )
@@ -263,6 +311,7 @@ struct MlpGrad
 //                               operands transposed through LDS),
 //   db_L   += row sums of dZ_L^T (per-lane partials, reduced by the caller),
 //   dH_{L-1}^T = W_L dZ_L^T,  dZ_{L-1} = dH_{L-1} * a'(Z_{L-1}).
+template <int ACT>
 __device__ __forceinline__ void mlp_backward( const uint2* wbwd, const MlpTile& t, bf16x4 x, f32x4 dout, short* buf,
                                               int lane, MlpGrad& g )
 {
@@ -299,7 +348,7 @@ __device__ __forceinline__ void mlp_backward( const uint2* wbwd, const MlpTile& 
             for ( int ot = 0; ot < no; ot++ )
                 c = mfma16( weight_frag_bwd( wbwd, L, it, ot, lane ), pack_bf16( dz[ot] ), c );
             const f32x4 z = t.z[L - 1][it];
-            nd[it] = f32x4{ c[0] * gate_dact( z[0] ), c[1] * gate_dact( z[1] ), c[2] * gate_dact( z[2] ), c[3] * gate_dact( z[3] ) };
+            nd[it] = dact_mul<ACT>( c, z );
         }
 #pragma unroll
         for ( int it = 0; it < 4; it++ )
@@ -356,6 +405,7 @@ __device__ __forceinline__ void wave_lds_sync()
 
 // forward of one tile keeping what the backward pass needs in the wave's LDS
 // block; returns the output tile (D layout)
+template <int ACT>
 __device__ __forceinline__ f32x4 mlp_forward_lds( const uint2* wimg, const float* bias, bf16x4 x, int lane,
                                                   MlpActLds& act )
 {
@@ -369,7 +419,7 @@ __device__ __forceinline__ f32x4 mlp_forward_lds( const uint2* wimg, const float
         f32x4 c = bias_frag( bias, 0, ot, lane );
         c = mfma16( weight_frag( wimg, 0, ot, 0, lane ), x, c );
         act.z[ot][lane] = c;
-        h[ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
+        h[ot] = act_pack<ACT>( c );
         store_rows( act.in[1 + ot], h[ot], lane );
     }
 #pragma unroll
@@ -384,7 +434,7 @@ __device__ __forceinline__ f32x4 mlp_forward_lds( const uint2* wimg, const float
             for ( int kt = 0; kt < 4; kt++ )
                 c = mfma16( weight_frag( wimg, L, ot, kt, lane ), h[kt], c );
             act.z[4 * L + ot][lane] = c;
-            hn[ot] = pack_bf16( f32x4{ gate_act( c[0] ), gate_act( c[1] ), gate_act( c[2] ), gate_act( c[3] ) } );
+            hn[ot] = act_pack<ACT>( c );
             store_rows( act.in[1 + 4 * L + ot], hn[ot], lane );
         }
 #pragma unroll
@@ -400,6 +450,7 @@ __device__ __forceinline__ f32x4 mlp_forward_lds( const uint2* wimg, const float
 }
 
 // mlp_backward with the activations read back from the wave's LDS block
+template <int ACT>
 __device__ __forceinline__ void mlp_backward_lds( const uint2* wbwd, MlpActLds& act, f32x4 dout, int lane, MlpGrad& g )
 {
     constexpr int kBiasTile[4] = { 0, 4, 8, 12 };
@@ -436,7 +487,7 @@ __device__ __forceinline__ void mlp_backward_lds( const uint2* wbwd, MlpActLds& 
             for ( int ot = 0; ot < no; ot++ )
                 c = mfma16( weight_frag_bwd( wbwd, L, it, ot, lane ), pack_bf16( dz[ot] ), c );
             const f32x4 z = act.z[4 * ( L - 1 ) + it][lane];
-            nd[it] = f32x4{ c[0] * gate_dact( z[0] ), c[1] * gate_dact( z[1] ), c[2] * gate_dact( z[2] ), c[3] * gate_dact( z[3] ) };
+            nd[it] = dact_mul<ACT>( c, z );
         }
 #pragma unroll
         for ( int it = 0; it < 4; it++ )

@@ -42,8 +42,10 @@ __device__ __forceinline__ float fast_div( float a, float b ) { return a * __bui
 __device__ __forceinline__ float fast_log( float x ) { return __builtin_amdgcn_logf( x ) * 0.693147180559945309f; }
 
 // erfc(x) by the Chebyshev-fitted form of Numerical Recipes in C (2nd ed.,
-// section 6.2, "erfcc"): fractional error below 1.2e-7 for every x, one
-// reciprocal and one exponential.
+// section 6.2, "erfcc"): fractional error below 1.2e-7 for every x in exact
+// arithmetic, one reciprocal and one exponential; in FP32 on the MI355X at
+// most 2.0e-6 relative for |x| <= 4.5 (the arguments z / sqrt 2 of the QE
+// exponential branch), from the FP32 exponential of -x^2 + P(t).
 __device__ __forceinline__ float erfc_fast( float x )
 {
     const float z = fabsf( x );
@@ -171,7 +173,8 @@ __device__ __forceinline__ double normal_f64( uint32_t x )
 // FP32 inverse normal from the 32-bit integer coordinate, in the erfinv form
 // of M. Giles ("Approximating the erfinv function", GPU Computing Gems 2011)
 // with coefficients fitted by thoth_amd/csrc/fit_inverse_normal.py to
-// Phi^-1 directly (max relative error 3.0e-7 over every 32-bit input):
+// Phi^-1 directly (max relative error 3.0e-7 in the fit's FP32 emulation,
+// 3.14e-7 measured on the MI355X, whose v_log_f32 is not correctly rounded):
 //   x = 2u - 1, w = -log(4u(1-u)), Phi^-1(u) = x P(w).
 // 4u(1-u) is formed from the integers u*2^32 and (1-u)*2^32, so it keeps its
 // precision in both tails; one log per normal.  The central polynomial
@@ -486,8 +489,9 @@ struct BridgeWalk
 static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 4, "hedge-network LDS area" );
 
 // CV: 0 plain sweep; 1 also writes the control-variate training samples;
-// 2 evaluates the hedge network at every hedge date inside the stepping loop
-// and accumulates P - CV in an extra result slot after the contracts.
+// 2 (SiLU network) and 3 (smoothstep-gated network) evaluate the hedge network
+// at every hedge date inside the stepping loop and accumulate P - CV in an
+// extra result slot after the contracts.
 // One block of 256 consecutive Sobol indices (Sobol block bq) of book B,
 // replicate z (RQMC), partial-sum slot (z, tile) of ntiles per replicate.
 // KIND: 0 any book; 1 (2) every equity of every book is LSV (Heston) with
@@ -528,7 +532,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     // in-loop inference kernels][Sobol ring: nslots x NF factors x 32 words]
     // [FP32 LSV: 2G x NA x lev_points leverage layers][in-loop hedge inference
     // or grid area, 16-byte aligned]
-    constexpr bool block_table = CV != 2;
+    constexpr bool block_table = CV < 2;
     extern __shared__ uint32_t lds[];
     const int nslots = B.sob_slots;
     const int tab_words = block_table ? ( m * F + 3 ) & ~3 : 0;
@@ -546,7 +550,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
     constexpr bool cv_on = CV != 0 && !F64;
     constexpr bool cv_samples = CV == 1 && !F64;
-    constexpr bool cv_infer = CV == 2 && !F64;
+    constexpr bool cv_infer = CV >= 2 && !F64;
+    constexpr int cv_act = CV == 3 ? kActGate : kActSilu;
     [[maybe_unused]] char* cvl = nullptr;
     if constexpr ( cv_infer )
     {
@@ -856,7 +861,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         {
             const uint2 xw = *reinterpret_cast<const uint2*>(
                 reinterpret_cast<const uint16_t*>( wl ) + ( 16 * t + ( lane & 15 ) ) * 16 + 4 * ( lane >> 4 ) );
-            const f32x4 out = mlp_infer( wimg, wb, bits_bf16x4( xw ), lane );
+            const f32x4 out = mlp_infer<cv_act>( wimg, wb, bits_bf16x4( xw ), lane );
             const int row = 16 * t + ( lane & 15 );
             if ( lane < 16 )
                 *reinterpret_cast<float4*>( hbuf + row * 8 ) = float4{ out[0], out[1], out[2], out[3] };
@@ -1457,7 +1462,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 
 // one Sobol block per workgroup: grid (blocks, books, replicates)
 template <class R, int NA, int DEPTH, int CV = 0, int KIND = 0>
-__global__ void __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( CV == 2 ? 2 : 1 ) ) )
+__global__ void __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( CV >= 2 ? 2 : 1 ) ) )
 sweep_kernel( const DevBook* __restrict__ books, const uint32_t* __restrict__ V, uint32_t bq0, uint32_t first_idx,
               uint32_t last_idx, const uint32_t* __restrict__ shifts )
 {

@@ -13,7 +13,8 @@ static void launch_one( dim3 grid, hipStream_t s, size_t lds, const DevBook* boo
 }
 
 // the same sweep with the learned control variate (at most 5 equities):
-// CV = 1 writes the training samples, CV = 2 runs the hedge network in the loop
+// CV = 1 writes the training samples, CV = 2 (SiLU) and 3 (smoothstep gate)
+// run the hedge network in the loop
 template <int CV>
 struct CvLaunch
 {
@@ -37,6 +38,8 @@ bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, 
         THOTH_DISPATCH_SWEEP( CvLaunch<1>::one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     if ( cv_mode == 2 )
         THOTH_DISPATCH_SWEEP( CvLaunch<2>::one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
+    if ( cv_mode == 3 )
+        THOTH_DISPATCH_SWEEP( CvLaunch<3>::one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, lds, books, V, bq0, first, last, shifts );
 }
 template <int NA, int D>

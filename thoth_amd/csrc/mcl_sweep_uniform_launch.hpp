@@ -49,6 +49,8 @@ inline bool by_cv( int cv_mode, int depth, dim3 grid, hipStream_t s, size_t lds,
             return by_depth<NA, 1, KIND>( depth, grid, s, lds, books, V, bq0, first, last, shifts );
         if ( cv_mode == 2 )
             return by_depth<NA, 2, KIND>( depth, grid, s, lds, books, V, bq0, first, last, shifts );
+        if ( cv_mode == 3 )
+            return by_depth<NA, 3, KIND>( depth, grid, s, lds, books, V, bq0, first, last, shifts );
     }
     return false;
 }

@@ -40,6 +40,41 @@ bool launch_sobol_points( hipStream_t s, const uint32_t* V, int dim, uint32_t fi
     return hipGetLastError() == hipSuccess;
 }
 
+// The FP32 special functions of the production sweep evaluated on given
+// inputs, one lane per input, so that their error bounds can be checked on the
+// device itself: fn 0 normal_f32 of the 32-bit Sobol coordinate in[i]; fn 1
+// the split form of the uniform-book kernels (central polynomial for every
+// lane, the tail lanes redone by normal_f32); fn 2 erfc_fast of the float
+// whose bits are in[i].
+__global__ void __launch_bounds__( 256 ) special_f32_kernel( int fn, int64_t n, const uint32_t* __restrict__ in,
+                                                             float* __restrict__ out )
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if ( i >= n )
+        return;
+    const uint32_t x = in[i];
+    float r;
+    if ( fn == 0 )
+        r = normal_f32( x );
+    else if ( fn == 1 )
+    {
+        bool tail;
+        r = normal_f32_central( x, tail );
+        if ( tail )
+            r = normal_f32( x );
+    }
+    else
+        r = erfc_fast( __uint_as_float( x ) );
+    out[i] = r;
+}
+
+bool launch_special_f32( hipStream_t s, int fn, int64_t n, const uint32_t* in, float* out )
+{
+    hipLaunchKernelGGL( special_f32_kernel, dim3( (uint32_t)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, fn, n, in,
+                        out );
+    return hipGetLastError() == hipSuccess;
+}
+
 // zero the persistent sweep's work counter (a kernel, so that the graph that
 // replays the sweep holds no memset node)
 __global__ void counter_reset_kernel( int* counter ) { *counter = 0; }

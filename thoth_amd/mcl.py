@@ -40,6 +40,7 @@ def lib():
         L.thoth_gpu_price_european_gbm.argtypes = [C.c_double] * 5 + [C.c_int, C.c_longlong, C.c_uint64,
                                                                         P(C.c_double), P(C.c_double)]
         L.thoth_gpu_sobol_points.argtypes = [C.c_int32, C.c_int64, C.c_int64, P(C.c_uint32)]
+        L.thoth_gpu_special_f32.argtypes = [C.c_int32, C.c_int64, P(C.c_uint32), P(C.c_float)]
         L.thoth_mcl_sweep.argtypes = [C.c_int32, P(abi.Book), P(abi.SweepCfg), P(abi.BookOut)]
         L.thoth_mcl_price.argtypes = [P(abi.Request), P(abi.Result)]
         L.thoth_mcl_prepare.argtypes = [P(abi.Request), P(C.c_void_p)]
@@ -60,7 +61,7 @@ def lib():
         L.thoth_cv_price.argtypes = [P(abi.Request), P(abi.CvConfig), P(abi.CvWeights), C.c_int32, C.c_uint64,
                                      P(abi.CvResult)]
         L.thoth_cv_loss_grad.argtypes = [P(abi.CvWeights), P(C.c_uint16), P(C.c_float), P(C.c_float), C.c_int32,
-                                         C.c_int32, P(C.c_double), P(C.c_float)]
+                                         C.c_int32, C.c_int32, P(C.c_double), P(C.c_float)]
         L.thoth_cv_samples.argtypes = [P(abi.Request), C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_uint64,
                                        P(C.c_uint16), P(C.c_float), P(C.c_float)]
         _lib = L
@@ -94,6 +95,17 @@ def price_european_gbm(forward, strike, t, vol, df, is_call, paths, seed=0):
 def sobol_points(dim, n, skip=0):
     out = np.zeros((n, dim), np.uint32)
     _check(lib().thoth_gpu_sobol_points(dim, skip, n, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
+
+
+def special_f32(fn, x):
+    """Device FP32 special function of the production sweep on uint32 inputs x:
+    fn 0 / 1 inverse normal of the Sobol coordinate (generic / split form),
+    fn 2 erfc of the float whose bits are x."""
+    x = np.ascontiguousarray(x, np.uint32)
+    out = np.zeros(x.size, np.float32)
+    _check(lib().thoth_gpu_special_f32(fn, x.size, x.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                       out.ctypes.data_as(C.POINTER(C.c_float))))
     return out
 
 
@@ -256,11 +268,20 @@ def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_see
 # ---- learned martingale control variate (include/thoth_mcl_cv.h) ----------
 
 def cv_config(hedge_dates=97, train_paths=1 << 18, train_sobol_skip=1 << 24, adam_steps=2000, batch_paths=32768,
-              learning_rate=2e-3, init_seed=7, eval_mode=abi.CV_IN_LOOP, hedge_spacing=2.0):
+              learning_rate=1e-3, init_seed=7, eval_mode=abi.CV_IN_LOOP, hedge_spacing=2.0, activation=abi.CV_SILU):
     """Training/evaluation settings; the training paths come from a Sobol block
-    (points train_sobol_skip+1 ..) disjoint from the evaluation replicates."""
+    (points train_sobol_skip+1 ..) disjoint from the evaluation replicates.
+    The defaults are the specified network: SiLU hidden units, Adam at
+    learning rate 1e-3, 2000 steps on 2^18 engine paths."""
     return abi.CvConfig(hedge_dates, train_paths, train_sobol_skip, adam_steps, batch_paths, learning_rate, init_seed,
-                        eval_mode, hedge_spacing)
+                        eval_mode, hedge_spacing, activation)
+
+
+def cv_config_gated(**kw):
+    """The smoothstep-gated variant (no transcendental in the network) at
+    learning rate 2e-3."""
+    kw.setdefault("learning_rate", 2e-3)
+    return cv_config(activation=abi.CV_GATE, **kw)
 
 
 def cv_train(built, cfg):
@@ -271,13 +292,21 @@ def cv_train(built, cfg):
     return w, rep
 
 
-def cv_price(built, cfg, weights, replicates=16, rqmc_seed=1):
+def cv_price(built, cfg, weights, replicates=16, rqmc_seed=1, per_replicate=False):
     """Plain and control-variate RQMC estimates of the base book premium
-    (built.req.mcl.paths paths per replicate); returns abi.CvResult."""
+    (built.req.mcl.paths paths per replicate); returns abi.CvResult, or with
+    per_replicate=True (abi.CvResult, plain replicate means, P - CV replicate means)."""
     res = abi.CvResult()
+    pm = np.zeros(replicates)
+    cm = np.zeros(replicates)
+    if per_replicate:
+        res.replicate_plain_means = pm.ctypes.data_as(C.POINTER(C.c_double))
+        res.replicate_cv_means = cm.ctypes.data_as(C.POINTER(C.c_double))
     _check(lib().thoth_cv_price(C.byref(built.req), C.byref(cfg), C.byref(weights), replicates, rqmc_seed,
                                 C.byref(res)))
-    return res
+    res.replicate_plain_means = None
+    res.replicate_cv_means = None
+    return (res, pm, cm) if per_replicate else res
 
 
 def cv_samples(built, hedge_dates, paths, sobol_skip=0, replicates=0, rqmc_seed=0):
@@ -294,7 +323,7 @@ def cv_samples(built, hedge_dates, paths, sobol_skip=0, replicates=0, rqmc_seed=
     return feat, dm, pay
 
 
-def cv_loss_grad(weights, feat, dm, pay):
+def cv_loss_grad(weights, feat, dm, pay, activation=abi.CV_SILU):
     """Device loss and gradient for host samples: feat uint16 [J, B, 16] (BF16
     bits), dm float32 [J, B, 8], pay float32 [B]; returns (loss, grad[10448])."""
     feat = np.ascontiguousarray(feat, np.uint16)
@@ -306,7 +335,7 @@ def cv_loss_grad(weights, feat, dm, pay):
     grad = np.zeros(abi.CV_WEIGHTS, np.float32)
     _check(lib().thoth_cv_loss_grad(C.byref(weights), feat.ctypes.data_as(C.POINTER(C.c_uint16)),
                                     dm.ctypes.data_as(C.POINTER(C.c_float)), pay.ctypes.data_as(C.POINTER(C.c_float)),
-                                    B, J, C.byref(loss), grad.ctypes.data_as(C.POINTER(C.c_float))))
+                                    B, J, activation, C.byref(loss), grad.ctypes.data_as(C.POINTER(C.c_float))))
     return loss.value, grad
 
 
