@@ -137,7 +137,7 @@ int oracle_path_noise( int n_times, const double* t, int factors, int64_t skip, 
     return 0;
 }
 
-double oracle_sabr_vol( const thoth_sabr_surface* s, double K, double F, double day )
+static Sabr SabrOf( const thoth_sabr_surface* s )
 {
     Sabr sb;
     for ( int i = 0; i < s->n; i++ )
@@ -148,7 +148,18 @@ double oracle_sabr_vol( const thoth_sabr_surface* s, double K, double F, double 
         sb.rho.push_back( s->rho[i] );
         sb.nu.push_back( s->nu[i] );
     }
-    return sb.ImplicitVol( K, F, day );
+    return sb;
+}
+
+double oracle_sabr_vol( const thoth_sabr_surface* s, double K, double F, double day )
+{
+    return SabrOf( s ).ImplicitVol( K, F, day );
+}
+
+// the Hagan expansion alone (no wings) at a maturity T in years
+double oracle_sabr_hagan( const thoth_sabr_surface* s, double K, double F, double T )
+{
+    return SabrOf( s ).HaganVol( K, F, T );
 }
 
 double oracle_local_vol( const thoth_mcl_request* req, int asset, double K, double day )

@@ -37,6 +37,8 @@ def lib():
     L.oracle_path_noise.argtypes = [C.c_int, P(C.c_double), C.c_int, C.c_int64, C.c_int64, P(C.c_double)]
     L.oracle_sabr_vol.argtypes = [P(abi.SabrSurface), C.c_double, C.c_double, C.c_double]
     L.oracle_sabr_vol.restype = C.c_double
+    L.oracle_sabr_hagan.argtypes = [P(abi.SabrSurface), C.c_double, C.c_double, C.c_double]
+    L.oracle_sabr_hagan.restype = C.c_double
     L.oracle_local_vol.argtypes = [P(abi.Request), C.c_int, C.c_double, C.c_double]
     L.oracle_local_vol.restype = C.c_double
     L.oracle_cholesky.argtypes = [C.c_int, P(C.c_double)]
@@ -108,6 +110,11 @@ def path_noise(t, factors, paths, skip=0):
 
 def sabr_vol(surface, K, F, day):
     return lib().oracle_sabr_vol(C.byref(surface), K, F, day)
+
+
+def sabr_hagan(surface, K, F, T):
+    """The Hagan expansion alone (no wings) at maturity T in years."""
+    return lib().oracle_sabr_hagan(C.byref(surface), K, F, T)
 
 
 def local_vol(built, asset, K, day):
