@@ -1,10 +1,18 @@
-"""Benchmark of the device Monte-Carlo sweep on one MI355X.
+"""Benchmark of the device Monte-Carlo engine on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--paths P] [--workload NAME]
+    python bench.py [--steps K] [--warmup W] [--paths P] [--leg NAME ...]
 
-A step is one device sweep of the workload's book over `paths` Sobol paths
-(the books are prepared and resident in HBM before the timed region: the LSV
-leverage calibration, grid and packing are setup).  Prints ONE JSON line.
+A step is one device sweep of the 5-asset LSV worst-of basket (2^20 Sobol
+paths, 365 two-day steps over 2y, 100 x 50 leverage surface, FP32 stepping
+with FP64 accumulation), the book prepared and resident in HBM before the
+timed region (grid, LSV leverage calibration and packing are setup).  Prints
+ONE JSON line: paths/s of that sweep with its roofline, beside the other
+workloads (legs): the basket with Cholesky-correlated spots and in FP64, the
+Heston down-and-out call with bump and pathwise Greeks, the vanilla grids in
+one persistent graph-replayed launch, the learned control variate as
+specified, the time to a 1bp confidence interval measured on the host clock,
+and the CPU oracle timed on this host.  `--leg` runs only the named legs
+(for profiling); the JSON line then holds those legs alone.
 """
 import argparse
 import json
@@ -15,6 +23,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -23,60 +33,101 @@ from thoth_amd.book import BuiltRequest  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (256 CU x 128 lanes x 2 x 2.4 GHz)
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense matrix cores
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic FLOPs per path (FMA = 2; add/mul/min/max/transcendental = 1;
-# integer Sobol work not counted): per LSV asset-step 2 normals x 35 + QE 22 +
-# Andersen 21 + leverage lookup 9 + Cholesky share 6 = 128; worst-of payoff 16;
-# Heston + discrete barrier step 105; one-step BS path 35 (FP64).
-FLOPS = {
-    "lsv_worst_of_basket": lambda steps, assets: steps * assets * 128 + 16,
-    "heston_down_out_call": lambda steps, assets: steps * 105,
-    "black_scholes_call": lambda steps, assets: 35,
-}
+# Algorithmic FLOPs (FMA = 2; add/mul/min/max/transcendental = 1; integer Sobol
+# work not counted), charged only for work the kernel does:
+#   LSV asset-step: 2 normals x 35 + QE 22 + Andersen 21 + leverage lookup 9 = 122,
+#     + 6 (the asset's share of a 5 x 5 lower Cholesky, 15 FMA / 5) when the
+#     spots are correlated; worst-of payoff 16 per path;
+#   Heston + discrete barrier step: 2 x 35 + 22 + 11 + 2 = 105;
+#   vanilla-grid payoffs: 82 cells x (sub + max + FMA) = 328 per maturity;
+#   one-step Black-Scholes path: 35 (FP64).
+LSV_STEP = 122
+CHOLESKY_SHARE = 6
+WORST_OF = 16
+HESTON_BARRIER_STEP = 105
+GRID_MATURITY = 328
+MLP_MACS = 9216            # 11 -> 64 -> 64 -> 64 -> 5 per evaluation
+T975_15 = 2.131449545559323  # Student t, 15 degrees of freedom, two-sided 95 %
 
 
-def workload(name, paths):
-    if name == "lsv_worst_of_basket":
-        return configs.lsv_worst_of_basket(paths=paths, precision=abi.FP32)
-    if name == "heston_down_out_call":
-        return configs.heston_down_out_call(paths=paths, precision=abi.FP32)
-    if name == "black_scholes_call":
-        return configs.black_scholes_call(paths=paths)
-    raise SystemExit("unknown workload %s" % name)
+def basket(paths, precision=abi.FP32, correlated=False, greeks=0):
+    return BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=precision,
+                                                    correlate_stochastic=correlated, greeks=greeks))
 
 
-def cpu_baseline(name, sample_paths):
+def timed_plan(built, steps, warmup):
+    """Sweeps of a prepared plan: (setup s, host-clock s per sweep list, device ms list, last result)."""
+    t0 = time.perf_counter()
+    plan = mcl.Plan(built)
+    setup = time.perf_counter() - t0
+    for _ in range(warmup):
+        plan.run()
+    wall, dev = [], []
+    res = None
+    for _ in range(steps):
+        t = time.perf_counter()
+        res = plan.run()
+        wall.append(time.perf_counter() - t)
+        dev.append(res.sweep_ms)
+    return setup, wall, dev, res
+
+
+def roofline(flops, kernel_ms, peak, kernel):
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "flops_per_launch": flops}
+
+
+def basket_leg(paths, steps, warmup, precision=abi.FP32, correlated=False):
+    """The basket sweep (uniform LSV kernel; correlated: its Cholesky form; FP64: the oracle-matched kernel)."""
+    b = basket(paths, precision, correlated)
+    setup, wall, dev, res = timed_plan(b, steps, warmup)
+    _, t, _ = mcl.grid(b)
+    m = len(t) - 1
+    na = len(b.spec.equities)
+    per_step = LSV_STEP + (CHOLESKY_SHARE if correlated else 0)
+    flops = paths * (m * na * per_step + WORST_OF)
+    kernel_ms = sum(dev) / len(dev)
+    fp32 = precision == abi.FP32
+    kernel = ("sweep_kernel<float, 5, 8, 0, %d>" % (3 if correlated else 1)) if fp32 else "sweep_kernel<double, 5, 8>"
+    return {"paths": paths, "steps": m, "assets": na, "correlated_spots": correlated,
+            "precision": "fp32 stepping, fp64 accumulation" if fp32 else "fp64",
+            "paths_per_s": paths * len(wall) / sum(wall), "ms_per_sweep": 1e3 * sum(wall) / len(wall),
+            "kernel_ms": kernel_ms, "setup_s": setup, "premium": res.premium, "premium_trust": res.premium_trust,
+            "roofline": roofline(flops, kernel_ms, FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS, kernel)}
+
+
+def cpu_baseline(sample_paths):
     """The CPU FP64 oracle (a restatement of the reference engine, which is
-    single-threaded) timed on this host on a bounded sample of the workload."""
+    single-threaded) timed on this host on a bounded sample of the basket."""
     try:
-        sys.path.insert(0, str(ROOT))
         from oracle import pyoracle
         pyoracle.lib()
     except Exception as e:  # the oracle is built on the CPU side; report if absent
         return {"value": None, "unit": "paths/s", "cores": 1, "kind": "port",
                 "sample": "oracle unavailable: %s" % e}
-    spec = workload(name, sample_paths)
-    spec.precision = abi.FP64
-    sec, _ = pyoracle.sweep_seconds(BuiltRequest(spec))
+    sec, _ = pyoracle.sweep_seconds(basket(sample_paths, abi.FP64))
     return {"value": sample_paths / sec, "unit": "paths/s", "cores": 1, "kind": "port",
-            "sample": "%d paths of the same book, path sweep only (grid, LSV calibration excluded), "
-                      "%.1f s on %s" % (sample_paths, sec, platform.processor() or platform.machine())}
+            "sample": "%d paths of the same book (5-asset LSV worst-of, 365 steps), path sweep only (grid, LSV "
+                      "calibration excluded), %.1f s on %s" % (sample_paths, sec,
+                                                                platform.processor() or platform.machine())}
 
 
 def _oracle_block(args):
     """One CPU worker of the all-cores baseline: the oracle's sweep of a
     disjoint Sobol block (run_cluster.cpp:308-339 splits paths the same way)."""
-    name, paths, skip = args
+    paths, skip = args
     from oracle import pyoracle
-    spec = workload(name, paths)
-    spec.precision = abi.FP64
+    spec = configs.lsv_worst_of_basket(paths=paths, precision=abi.FP64)
     spec.mcl.sobol_skip = skip
     sec, _ = pyoracle.sweep_seconds(BuiltRequest(spec))
     return sec
 
 
-def cpu_baseline_all_cores(name, paths_per_core):
+def cpu_baseline_all_cores(paths_per_core):
     """The same oracle on every host core this process may use (at most 16, the
     host share of one GPU), one process per core on disjoint Sobol blocks;
     throughput = all paths / the slowest worker's sweep time."""
@@ -88,7 +139,7 @@ def cpu_baseline_all_cores(name, paths_per_core):
     cores = max(1, min(16, cores))
     try:
         with mp.get_context("spawn").Pool(cores) as pool:
-            secs = pool.map(_oracle_block, [(name, paths_per_core, k * paths_per_core) for k in range(cores)])
+            secs = pool.map(_oracle_block, [(paths_per_core, k * paths_per_core) for k in range(cores)])
     except Exception as e:
         return {"value": None, "unit": "paths/s", "cores": cores, "kind": "port", "sample": "failed: %s" % e}
     return {"value": cores * paths_per_core / max(secs), "unit": "paths/s", "cores": cores, "kind": "port",
@@ -127,18 +178,17 @@ def _profile_counter(counter, cmd, kernel, workdir):
     return _counter_kib(files[0], kernel) if files else None
 
 
-def hbm_traffic(workload_name, paths, kernel="sweep_kernel"):
-    """HBM bytes per launch of the sweep kernel: rocprofv3 FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch) of a one-step run of this workload, each
-    scaled by bytes / counter of a device-to-device copy of known size."""
+def hbm_traffic(paths, kernel="sweep_kernel"):
+    """HBM bytes per launch of the basket sweep: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) of a one-step run, each scaled by bytes /
+    counter of a device-to-device copy of known size."""
     import shutil
     import tempfile
     if not shutil.which("rocprofv3"):
         return None, "rocprofv3 not found"
     probe_mib = 256
     me = [sys.executable, str(Path(__file__).resolve())]
-    run = me + ["--workload", workload_name, "--paths", str(paths), "--steps", "1", "--warmup", "0",
-                "--no-cpu", "--no-t1bp", "--no-grid", "--no-traffic", "--no-heston"]
+    run = me + ["--leg", "c3", "--paths", str(paths), "--steps", "1", "--warmup", "0"]
     probe = me + ["--copy-probe", str(probe_mib)]
     detail = {}
     total = 0.0
@@ -154,70 +204,110 @@ def hbm_traffic(workload_name, paths, kernel="sweep_kernel"):
     return total, detail
 
 
-def time_to_1bp(replicates, paths, seed=101, target=0.01, other_regimes=((512, 1 << 14),)):
-    """Wall time to a 95 % half-width of 1bp of notional (0.01 on the rebased
-    100 notional) on the 5-asset LSV worst-of basket, plain Monte-Carlo in the
-    FP64 oracle-matched mode against the FP32 learned control variate.  Each
-    estimator is run at R randomised-QMC replicates of `paths` Sobol points
-    (the same digital shifts for all) and its device time is projected to the
-    target by the 1/sqrt(N) law of the half-width.
-
-    The regime matters for plain MC, whose RQMC gain grows with the points per
-    replicate: at the target it needs about 26 replicates of 4096 points (a CI
-    needs at least 16 replicates, so larger replicates overshoot the target),
-    and 4096 points per replicate is the default.  The learned control variate
-    has no such gain (its residual is the high-frequency hedging error).  The
-    same comparison at other (replicates, points) is reported beside it."""
-    out = {"target_halfwidth": target, "replicates": replicates, "paths_per_replicate": paths,
-           "projection": "device ms x (halfwidth / target)^2"}
-
-    def entry(mean, hw, ms, **kw):
-        e = {"premium": mean, "halfwidth": hw, "ms": ms, "projected_ms": ms * (hw / target) ** 2}
-        e.update(kw)
-        return e
-
-    cfg = mcl.cv_config()
-    w, rep = mcl.cv_train(BuiltRequest(configs.lsv_worst_of_basket(paths=paths, precision=abi.FP32)), cfg)
-    train_ms = rep.data_ms + rep.train_ms
-
-    def regime(R, N):
-        res = {}
-        b64 = BuiltRequest(configs.lsv_worst_of_basket(paths=N, precision=abi.FP64))
-        r64 = mcl.price_rqmc(b64, R, seed)
-        res["plain_fp64"] = entry(r64.mean, r64.halfwidth, r64.sweep_ms, per_path_var=r64.var)
-        b32 = BuiltRequest(configs.lsv_worst_of_basket(paths=N, precision=abi.FP32))
-        r32 = mcl.price_rqmc(b32, R, seed)
-        res["plain_fp32"] = entry(r32.mean, r32.halfwidth, r32.sweep_ms, per_path_var=r32.var)
-        rcv = mcl.cv_price(b32, cfg, w, R, seed)
-        res["learned_cv_fp32"] = entry(rcv.cv_mean, rcv.cv_halfwidth, rcv.sweep_ms + rcv.eval_ms,
-                                       per_path_var=rcv.cv_var, variance_ratio=rcv.plain_var / rcv.cv_var)
-        res["speedup_vs_plain_fp64"] = res["plain_fp64"]["projected_ms"] / res["learned_cv_fp32"]["projected_ms"]
-        return res
-
-    out.update(regime(replicates, paths))
-    out["learned_cv_fp32"].update(
-        train_ms=train_ms,
-        training={"paths": cfg.train_paths, "adam_steps": cfg.adam_steps, "batch_paths": cfg.batch_paths,
-                  "hedge_dates": cfg.hedge_dates, "hedge_spacing": cfg.hedge_spacing,
-                  "learning_rate": cfg.learning_rate, "loss_first": rep.loss_first, "loss_last": rep.loss_last})
-    out["speedup_vs_plain_fp64_incl_training"] = out["plain_fp64"]["projected_ms"] / (
-        out["learned_cv_fp32"]["projected_ms"] + train_ms)
-    # one trained network serves every re-pricing of the book: pricings at the
-    # target after which the training has paid for itself
-    out["training_break_even_pricings"] = train_ms / (
-        out["plain_fp64"]["projected_ms"] - out["learned_cv_fp32"]["projected_ms"])
-    out["other_regimes"] = []
-    for R, N in other_regimes:
-        r = regime(R, N)
-        out["other_regimes"].append({"replicates": R, "paths_per_replicate": N,
-                                     "speedup_vs_plain_fp64": r["speedup_vs_plain_fp64"],
-                                     "plain_fp64_halfwidth": r["plain_fp64"]["halfwidth"],
-                                     "learned_cv_halfwidth": r["learned_cv_fp32"]["halfwidth"],
-                                     "plain_fp64_ms": r["plain_fp64"]["ms"],
-                                     "learned_cv_ms": r["learned_cv_fp32"]["ms"]})
-    return out
+# ---- learned control variate and the time to a 1bp confidence interval ---------
+def learned_cv(cfg, eval_replicates=256, eval_paths=1 << 12, seed=2026):
+    """Train the hedge network on 2^18 engine paths (points of a disjoint Sobol
+    block) and evaluate it on 2^20 fresh paths (random digital shifts of Sobol
+    points 0 .. eval_paths - 1), against plain MC on the same paths."""
+    b = basket(eval_paths, abi.FP32)
+    w, rep = mcl.cv_train(b, cfg)
+    r = mcl.cv_price(b, cfg, w, eval_replicates, seed)
+    # the in-loop sweep: 365 x 5 LSV asset-steps per path plus the network at J dates
+    J = cfg.hedge_dates
+    n = eval_replicates * eval_paths
+    out = {"activation": "silu" if cfg.activation == abi.CV_SILU else "smoothstep-gated linear",
+           "network": "11 -> 64 -> 64 -> 64 -> 5, BF16 weights, FP32 accumulation",
+           "adam": {"learning_rate": round(cfg.learning_rate, 6), "steps": rep.steps, "batch_paths": cfg.batch_paths},
+           "train_paths": cfg.train_paths, "hedge_dates": J, "hedge_spacing": cfg.hedge_spacing,
+           "train_ms": rep.train_ms, "train_data_ms": rep.data_ms, "loss_first": rep.loss_first,
+           "loss_last": rep.loss_last,
+           "eval_paths": n, "plain_mean": r.plain_mean, "plain_halfwidth": r.plain_halfwidth,
+           "cv_mean": r.cv_mean, "cv_halfwidth": r.cv_halfwidth, "variance_ratio": r.plain_var / r.cv_var,
+           "paired_cv_term": {"mean": r.cv_term_mean, "halfwidth": r.cv_term_halfwidth},
+           "sweep_ms": r.sweep_ms, "paths_per_s": n / (r.sweep_ms * 1e-3),
+           "inference_tflops_bf16": n * J * 2 * MLP_MACS / (r.sweep_ms * 1e-3) / 1e12}
+    return w, rep, out
 
 
+def _hw16(means):
+    """Half-widths of consecutive groups of 16 replicate means."""
+    g = np.asarray(means).reshape(-1, 16)
+    return T975_15 * g.std(axis=1, ddof=1) / 4.0
+
+
+def time_to_1bp(cfg, w, train_ms, target=0.01, batch=64, seed=101):
+    """Host-clock time to a 95 % half-width of 1bp of notional (0.01 on the
+    rebased 100 notional) on the basket, for plain MC in the FP64
+    oracle-matched mode, plain MC in FP32 and the learned control variate
+    (FP32), each a randomised-QMC estimate over 16 random digital shifts.
+
+    For each estimator the paths per replicate N are the smallest of the grid
+    256 ceil(2^(k/4)) whose expected 16-replicate half-width (the RMS over 4
+    independent groups of 16 shifts) is at most the target, searched upward
+    from 2^(3/2) below the 1/sqrt(N) prediction of a 4096-point pilot.  Then:
+    * single: one pricing of 16 replicates at that N, alone on the GPU: the
+      median host-clock time of 5 runs with the book resident (launch, sweep,
+      inference, reduction; the book setup is reported apart);
+    * batched: `batch` pricings (16 replicates each) in one launch, the
+      host-clock time per pricing and the half-widths each pricing reached.
+    The training time of the network is reported beside, with the number of
+    pricings after which it has paid for itself."""
+    b64 = lambda n: basket(n, abi.FP64)
+    b32 = lambda n: basket(n, abi.FP32)
+    runs = {
+        "plain_fp64": lambda n, R, s: mcl.price_rqmc(b64(n), R, s, per_replicate=True),
+        "plain_fp32": lambda n, R, s: mcl.price_rqmc(b32(n), R, s, per_replicate=True),
+        "learned_cv_fp32": lambda n, R, s: mcl.cv_price(b32(n), cfg, w, R, s, per_replicate=True),
+    }
+
+    def result(name, out):
+        if name == "learned_cv_fp32":
+            r, _, cm = out
+            return r, cm, r.run_wall_ms, r.sweep_ms, r.setup_ms
+        r, m = out
+        return r, m, r.run_wall_ms, r.sweep_ms, r.setup_ms
+
+    grid_n = lambda k: 256 * math.ceil(2 ** (k / 4))
+    res = {"target_halfwidth": target, "replicates": 16}
+    for name, run in runs.items():
+        def expected_hw(n):
+            _, means, _, _, _ = result(name, run(n, 64, seed))
+            return float(np.sqrt(np.mean(_hw16(means) ** 2)))
+        pilot = 4096
+        k = max(0, math.ceil(4 * math.log2(pilot / 256 * (expected_hw(pilot) / target) ** 2)) - 6)
+        while True:
+            n = grid_n(k)
+            hw_expected = expected_hw(n)
+            if hw_expected <= target or n >= 1 << 22:
+                break
+            k += 1
+        walls = []
+        for k in range(5):
+            r, means, wall, dev, setup = result(name, run(n, 16, seed + 1 + k))
+            walls.append(wall)
+        single = {"paths_per_replicate": n, "paths": 16 * n, "expected_halfwidth": hw_expected,
+                  "halfwidth": float(_hw16(means)[0]), "wall_ms": float(np.median(walls)), "device_ms": dev,
+                  "book_setup_ms": setup}
+        r, means, wall, dev, _ = result(name, run(n, 16 * batch, seed + 100))
+        hws = _hw16(means)
+        batched = {"pricings": batch, "wall_ms_per_pricing": wall / batch, "device_ms_per_pricing": dev / batch,
+                   "halfwidth_mean": float(hws.mean()), "halfwidth_max": float(hws.max()),
+                   "fraction_at_target": float(np.mean(hws <= target))}
+        res[name] = {"single": single, "batched": batched}
+    cv, p64, p32 = res["learned_cv_fp32"], res["plain_fp64"], res["plain_fp32"]
+    for regime, key in (("single", "wall_ms"), ("batched", "wall_ms_per_pricing")):
+        t64, t32, tcv = p64[regime][key], p32[regime][key], cv[regime][key]
+        res["speedup_" + regime] = {
+            "learned_cv_fp32_vs_plain_fp64": t64 / tcv,
+            "learned_cv_fp32_vs_plain_fp32": t32 / tcv,
+            "plain_fp32_vs_plain_fp64": t64 / t32,
+            "training_break_even_pricings_vs_plain_fp64": train_ms / (t64 - tcv) if t64 > tcv else None,
+        }
+    res["training_ms"] = train_ms
+    return res
+
+
+# ---- other workloads ---------------------------------------------------------------
 def heston_barrier(paths=1 << 20, steps=252, repeats=3):
     """The single-asset Heston down-and-out call (v0 = theta = 0.04, kappa
     1.5, xi 0.5, rho -0.7, K 100, H 80 monitored daily) on 252 uniform steps:
@@ -242,8 +332,11 @@ def heston_barrier(paths=1 << 20, steps=252, repeats=3):
                                   "gamma": pw[-1].gamma, "vega": pw[-1].vega, "vega_trust": pw[-1].vega_trust,
                                   "ms": min(p.sweep_ms for p in pw)}}
     out.update(res["fp32"])
-    out["paths_per_s_bump"] = paths * res["fp32"]["bump"]["books"] / (res["fp32"]["bump"]["ms"] * 1e-3)
+    books = res["fp32"]["bump"]["books"]
+    out["paths_per_s_bump"] = paths * books / (res["fp32"]["bump"]["ms"] * 1e-3)
     out["paths_per_s_pathwise"] = paths / (res["fp32"]["pathwise"]["ms"] * 1e-3)
+    out["bump"]["roofline"] = roofline(books * paths * steps * HESTON_BARRIER_STEP, res["fp32"]["bump"]["ms"],
+                                       FP32_PEAK_TFLOPS, "sweep_kernel<float, 1, 8, 0, 2>")
     out["fp32_minus_fp64"] = {
         kind: {k: res["fp32"][kind][k] - res["fp64"][kind][k] for k in ("premium", "delta", "gamma", "vega")}
         for kind in ("bump", "pathwise")}
@@ -265,18 +358,19 @@ def vanilla_grids(paths, replays=5):
     t = plan.tuning()
     dev = sum(ms) / len(ms)
     n_books = 3 * len(books)
-    # per book-path: 365 LSV steps x 128 FLOP + 12 maturities x 328 (82 cells)
-    flops = n_books * paths * (365 * 128 + 12 * 328)
-    achieved = flops / (dev * 1e-3) / 1e12
+    # per book-path: 365 LSV steps (one equity: no Cholesky) + 12 maturities x 82 cells
+    flops = n_books * paths * (365 * LSV_STEP + 12 * GRID_MATURITY)
     return {"workload": "lsv_vanilla_grids", "underlyings": len(books), "books": n_books,
             "cells_per_underlying": 2 * len(configs.GRID_STRIKES) * len(configs.GRID_MATURITIES),
             "paths_per_book": paths, "replays": replays, "device_ms": dev,
             "book_paths_per_s": n_books * paths / (dev * 1e-3),
-            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": achieved / FP32_PEAK_TFLOPS},
+            "roofline": roofline(flops, dev, FP32_PEAK_TFLOPS, "sweep_persistent<float, 1, 8, 1>"),
             "autotuned": {"workgroups_per_cu": t.blocks_per_cu, "leverage_layers_per_barrier": t.lev_group,
                           "candidates": t.candidates, "best_ms": t.device_ms},
             "setup_s": setup}
+
+
+LEGS = ("c3", "c3corr", "c3fp64", "c2", "c4", "c5", "t1bp", "cpu", "traffic")
 
 
 def main():
@@ -285,18 +379,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--paths", type=int, default=1 << 20)
-    ap.add_argument("--workload", default="lsv_worst_of_basket")
+    ap.add_argument("--leg", action="append", choices=LEGS,
+                    help="run only these legs (repeatable); default: all")
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-t1bp", action="store_true", help="skip the time-to-1bp comparison")
-    ap.add_argument("--t1bp-replicates", type=int, default=2048)
-    ap.add_argument("--t1bp-paths", type=int, default=1 << 12)
-    ap.add_argument("--no-grid", action="store_true", help="skip the vanilla-grid batch")
-    ap.add_argument("--grid-paths", type=int, default=1 << 18)
-    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
-    ap.add_argument("--no-heston", action="store_true", help="skip the Heston barrier (bump and pathwise Greeks)")
     ap.add_argument("--cpu-all-cores-paths", type=int, default=1 << 14,
                     help="paths per core of the all-cores CPU baseline (0 skips it)")
+    ap.add_argument("--grid-paths", type=int, default=1 << 18)
+    ap.add_argument("--no-gated", action="store_true", help="skip the smoothstep-gated network variant")
+    ap.add_argument("--adam-steps", type=int, default=2000,
+                    help="Adam steps of the hedge-network training (2000 as specified; fewer only for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -313,74 +404,60 @@ def main():
         print("bench.py: this build is single-GPU (one MI355X); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-
-    spec = workload(args.workload, args.paths)
-    built = BuiltRequest(spec)
-    t0 = time.perf_counter()
-    plan = mcl.Plan(built)
-    setup_s = time.perf_counter() - t0
-    for _ in range(args.warmup):
-        plan.run()
-    wall, dev = [], []
-    res = None
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        res = plan.run()
-        wall.append(time.perf_counter() - t)
-        dev.append(res.sweep_ms)
-    total = sum(wall)
-    ms_step = 1e3 * total / args.steps
-    value = args.paths * args.steps / total
-    kernel_ms = sum(dev) / len(dev)
-    _, t, _ = mcl.grid(built)
-    steps = len(t) - 1
-    flops = args.paths * FLOPS[args.workload](steps, len(spec.equities))
-    fp32 = spec.precision == abi.FP32
-    peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
-    line = {
-        "metric": "paths/s & time-to-1bp CI: LSV basket, learned-CV FP32 vs plain MC, 1 MI355X",
-        "value": value,
-        "unit": "paths/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if fp32 else "f64",
-        "data": "synthetic (seeded books, Joe-Kuo Sobol paths)",
-        "config": {"workload": args.workload, "paths": args.paths, "steps": steps,
-                   "assets": len(spec.equities), "precision": "fp32 stepping, fp64 accumulation" if fp32 else "fp64"},
-        "kernel_ms": kernel_ms,
-        "setup_s": setup_s,
-        "premium": res.premium,
-        "premium_trust": res.premium_trust,
-        "roofline": {"bound": "valu", "kernel": "sweep_kernel<%s, %d, ...>" % ("float" if fp32 else "double",
-                                                                            len(spec.equities)),
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None},
-        "device": mcl.device_info(),
-    }
-    if not args.no_grid:
+    legs = set(args.leg or LEGS)
+    line = {}
+    if "c3" in legs:
+        c3 = basket_leg(args.paths, args.steps, args.warmup)
+        line.update({
+            "metric": "paths/s & time-to-1bp CI: LSV basket, learned-CV FP32 vs plain MC, 1 MI355X",
+            "value": c3["paths_per_s"],
+            "unit": "paths/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": c3["ms_per_sweep"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic (seeded books, Joe-Kuo Sobol paths)",
+            "config": {"workload": "lsv_worst_of_basket", "paths": args.paths, "steps": c3["steps"],
+                       "assets": c3["assets"], "leverage_surface": "100 spot x 50 time",
+                       "precision": c3["precision"], "correlated_spots": False},
+            "kernel_ms": c3["kernel_ms"],
+            "setup_s": c3["setup_s"],
+            "premium": c3["premium"],
+            "premium_trust": c3["premium_trust"],
+            "roofline": dict(c3["roofline"], traffic=None),
+            "device": mcl.device_info(),
+        })
+    if "c3corr" in legs:
+        line["basket_correlated"] = basket_leg(args.paths, max(3, args.steps // 2), 1, correlated=True)
+    if "c3fp64" in legs:
+        line["basket_fp64"] = basket_leg(args.paths, 3, 1, precision=abi.FP64)
+    if "c4" in legs:
         line["vanilla_grids"] = vanilla_grids(args.grid_paths)
-    if not args.no_heston:
+    if "c2" in legs:
         line["heston_barrier"] = heston_barrier()
-    if not args.no_t1bp and args.workload == "lsv_worst_of_basket":
-        line["time_to_1bp"] = time_to_1bp(args.t1bp_replicates, args.t1bp_paths)
-    if not args.no_traffic:
-        traffic, detail = hbm_traffic(args.workload, args.paths)
+    if "c5" in legs or "t1bp" in legs:
+        cfg = mcl.cv_config(adam_steps=args.adam_steps)
+        w, rep, c5 = learned_cv(cfg)
+        line["learned_cv"] = c5
+        if not args.no_gated:
+            line["learned_cv_gated"] = learned_cv(mcl.cv_config_gated(adam_steps=args.adam_steps))[2]
+        if "t1bp" in legs:
+            line["time_to_1bp"] = time_to_1bp(cfg, w, rep.train_ms + rep.data_ms)
+    if "traffic" in legs and "roofline" in line:
+        traffic, detail = hbm_traffic(args.paths)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_detail"] = detail
         if traffic:
-            line["roofline"]["hbm_gbs"] = traffic / (kernel_ms * 1e-3) / 1e9
+            line["roofline"]["hbm_gbs"] = traffic / (line["kernel_ms"] * 1e-3) / 1e9
             line["roofline"]["hbm_frac"] = line["roofline"]["hbm_gbs"] / HBM_PEAK_GBS
-    if not args.no_cpu:
-        cb = cpu_baseline(args.workload, args.cpu_sample)
-        line["cpu_baseline"] = cb
+    if "cpu" in legs:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample)
         if args.cpu_all_cores_paths:
-            line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(args.workload, args.cpu_all_cores_paths)
+            line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(args.cpu_all_cores_paths)
     print(json.dumps(line))
 
 

@@ -179,6 +179,9 @@ typedef struct thoth_rqmc_result {
     double halfwidth;    /* 95 % confidence half-width of the mean */
     double var;          /* pooled per-path variance */
     double sweep_ms;     /* device time of the sweep */
+    double setup_ms;     /* host clock: grid, leverage calibration, packing and upload */
+    double run_wall_ms;  /* host clock: launch, sweep, reduction and interval, the book resident */
+    double* replicate_means; /* optional caller-owned array of `replicates` entries (NULL: not written) */
 } thoth_rqmc_result;
 int thoth_mcl_price_rqmc(const thoth_mcl_request* req, int32_t replicates, uint64_t rqmc_seed, thoth_rqmc_result* res);
 

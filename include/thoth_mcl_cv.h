@@ -85,6 +85,9 @@ typedef struct thoth_cv_result {
      * the replicate means of P and of P - CV */
     double* replicate_plain_means;
     double* replicate_cv_means;
+    /* host clock: book setup (grid, leverage calibration, packing, network upload), and
+     * the pricing itself with the book resident (launches, sweep, inference, reduction) */
+    double setup_ms, run_wall_ms;
 } thoth_cv_result;
 
 /* Train the hedge network for the base book of req (FP32 sweep).  w is the

@@ -132,12 +132,13 @@ class CvResult(C.Structure):
     _fields_ = [("replicates", _i32), ("paths", _i64), ("plain_mean", _d), ("plain_halfwidth", _d),
                 ("cv_mean", _d), ("cv_halfwidth", _d), ("plain_var", _d), ("cv_var", _d),
                 ("sweep_ms", _d), ("eval_ms", _d), ("cv_term_mean", _d), ("cv_term_halfwidth", _d),
-                ("replicate_plain_means", C.POINTER(_d)), ("replicate_cv_means", C.POINTER(_d))]
+                ("replicate_plain_means", C.POINTER(_d)), ("replicate_cv_means", C.POINTER(_d)),
+                ("setup_ms", _d), ("run_wall_ms", _d)]
 
 
 class RqmcResult(C.Structure):
     _fields_ = [("replicates", _i32), ("paths", _i64), ("mean", _d), ("halfwidth", _d), ("var", _d),
-                ("sweep_ms", _d)]
+                ("sweep_ms", _d), ("setup_ms", _d), ("run_wall_ms", _d), ("replicate_means", C.POINTER(_d))]
 
 
 # include/thoth_mcl_grid.h

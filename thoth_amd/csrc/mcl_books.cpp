@@ -410,12 +410,12 @@ static int SlotsOf( const DeviceBatch& B, int b )
 
 DeviceBatch::~DeviceBatch()
 {
-    if ( dbase )
+    if ( dbase && !external )
         (void)hipFree( dbase );
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv, const GridSpec* grids, bool pathwise )
+                  const CvSpec* cv, const GridSpec* grids, bool pathwise, char* ext_base, size_t ext_cap )
 {
     B.pathwise = pathwise;
     int rc = EnsureDevice();
@@ -478,7 +478,8 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     }
     // FP32 books whose equities are all LSV (or all Heston) with vol-of-vol
     // > 1e-12, uncorrelated stochastic spots and the same equity count run
-    // the uniform-book kernel (no per-equity model branches)
+    // the uniform-book kernel (no per-equity model branches); all-LSV books
+    // with correlated stochastic spots run its correlated form (kind 3)
     B.kind = 0;
     if ( cfg->precision == 1 )
     {
@@ -490,7 +491,10 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             {
                 const DevAsset& A = d.asset[a];
                 int k = A.model == THOTH_MODEL_LSV ? 1 : A.model == THOTH_MODEL_HESTON ? 2 : 0;
-                if ( !( A.xi > 1e-12 ) || d.corr_stoch || d.n_assets != B.na_max )
+                // correlated stochastic spots: the correlated LSV kind (3) only
+                if ( d.corr_stoch && d.n_assets > 1 )
+                    k = k == 1 ? 3 : 0;
+                if ( !( A.xi > 1e-12 ) || d.n_assets != B.na_max )
                     k = 0;
                 kind = kind < 0 || kind == k ? k : 0;
             }
@@ -592,9 +596,22 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
     const size_t total =
         arena_bytes + books_bytes + ( B.part_total + B.grid_total + B.grid_out_total ) * sizeof( double );
-    hipError_t e = hipMalloc( &B.dbase, total );
-    if ( e != hipSuccess )
-        return SetError( THOTH_ERR_DEVICE, std::string( "hipMalloc: " ) + hipGetErrorString( e ) );
+    if ( ext_base )
+    {
+        if ( total > ext_cap )
+            return SetError( THOTH_ERR_UNSUPPORTED,
+                             "the books need " + std::to_string( total ) + " bytes of device memory, more than the " +
+                                 std::to_string( ext_cap ) + " bytes of the no-allocation scratch (fewer paths, "
+                                 "replicates, dates or leverage points, or the allocating entry point)" );
+        B.dbase = ext_base;
+        B.external = true;
+    }
+    else
+    {
+        hipError_t e = hipMalloc( &B.dbase, total );
+        if ( e != hipSuccess )
+            return SetError( THOTH_ERR_DEVICE, std::string( "hipMalloc: " ) + hipGetErrorString( e ) );
+    }
     char* dbase = B.dbase;
     B.dpart = reinterpret_cast<double*>( dbase + arena_bytes + books_bytes );
     B.dgrid = B.dpart + B.part_total;
@@ -661,9 +678,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
 {
     hipStream_t s = Stream();
-    hipEvent_t e0, e1;
-    HIP_OK( hipEventCreate( &e0 ) );
-    HIP_OK( hipEventCreate( &e1 ) );
+    hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
     HIP_OK( hipEventRecord( e0, s ) );
     const dim3 grid( B.nblocks, B.n_books, B.replicates );
     // kernel control-variate mode: in-loop inference of a gated network is mode 3
@@ -677,11 +692,7 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
                         launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
                                           B.first, B.last, B.dshifts, kcv );
     if ( !ok )
-    {
-        (void)hipEventDestroy( e0 );
-        (void)hipEventDestroy( e1 );
         return SetError( THOTH_ERR_UNSUPPORTED, "no kernel instantiation for this book shape" );
-    }
     HIP_OK( hipGetLastError() );
     HIP_OK( hipEventRecord( e1, s ) );
     std::vector<double> part( B.part_total );
@@ -689,8 +700,6 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
     HIP_OK( hipStreamSynchronize( s ) );
     float ms = 0;
     HIP_OK( hipEventElapsedTime( &ms, e0, e1 ) );
-    (void)hipEventDestroy( e0 );
-    (void)hipEventDestroy( e1 );
     for ( int b = 0; b < B.n_books; b++ )
       for ( int r = 0; r < B.replicates; r++ )
     {

@@ -307,6 +307,7 @@ void FinishCi( const std::vector<double>& sums, int R, int64_t N, thoth_cv_resul
 int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const thoth_cv_weights& w, int R,
                  uint64_t rqmc_seed, thoth_cv_result* res )
 {
+    const auto t0 = std::chrono::steady_clock::now();
     host::BookArrays A;
     host::Grid g;
     int rc = host::ResolveBaseBook( req, A, g );
@@ -337,6 +338,7 @@ int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const
     DeviceBatch B;
     if ( ( rc = PrepareBatch( 1, &A.book, &sc, B, &cv ) ) != THOTH_OK )
         return rc;
+    const auto t1 = std::chrono::steady_clock::now();
     std::vector<thoth_mcl_book_out> out( R );
     std::vector<CvSums> cs( R );
     if ( ( rc = RunBatch( B, out.data(), cs.data() ) ) != THOTH_OK )
@@ -350,6 +352,9 @@ int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const
         sums[4 * r + 3] = out[r].book_sum2;
     }
     FinishCi( sums, R, req.mcl.paths, res );
+    const auto t2 = std::chrono::steady_clock::now();
+    res->setup_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count();
+    res->run_wall_ms = std::chrono::duration<double, std::milli>( t2 - t1 ).count();
     res->sweep_ms = out[0].device_ms;
     res->eval_ms = 0;
     return THOTH_OK;

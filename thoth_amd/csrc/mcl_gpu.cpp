@@ -45,6 +45,7 @@ struct DeviceState
     std::string info;
     uint32_t* d_dirs = nullptr;  // [kSobolDims][32] directions, then the lane nibble tables
     hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = { nullptr, nullptr };  // sweep timing (calls are serialised)
 };
 DeviceState g_dev;
 
@@ -161,9 +162,13 @@ int EnsureDevice()
         }
     HIP_OK( hipMalloc( &g_dev.d_dirs, tables.size() * sizeof( uint32_t ) ) );
     HIP_OK( hipMemcpy( g_dev.d_dirs, tables.data(), tables.size() * sizeof( uint32_t ), hipMemcpyHostToDevice ) );
+    HIP_OK( hipEventCreate( &g_dev.ev[0] ) );
+    HIP_OK( hipEventCreate( &g_dev.ev[1] ) );
     g_dev.ok = true;
     return THOTH_OK;
 }
+
+hipEvent_t SweepEvent( int k ) { return g_dev.ev[k]; }
 
 std::mutex& DeviceMutex()
 {

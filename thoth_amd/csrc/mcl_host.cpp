@@ -587,6 +587,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_rqmc
         int rc = thoth::EnsureDevice();
         if ( rc != THOTH_OK )
             return rc;
+        const auto t0 = std::chrono::steady_clock::now();
         thoth::host::BookArrays A;
         thoth::host::Grid g;
         if ( ( rc = thoth::host::ResolveBaseBook( *req, A, g ) ) != THOTH_OK )
@@ -599,6 +600,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_rqmc
         thoth::DeviceBatch B;
         if ( ( rc = thoth::PrepareBatch( 1, &A.book, &sc, B ) ) != THOTH_OK )
             return rc;
+        const auto t1 = std::chrono::steady_clock::now();
         std::vector<thoth_mcl_book_out> out( replicates );
         if ( ( rc = thoth::RunBatch( B, out.data() ) ) != THOTH_OK )
             return rc;
@@ -622,6 +624,12 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_rqmc
         res->halfwidth = thoth::StudentT975( replicates - 1 ) * std::sqrt( v / replicates );
         res->var = ( s2 - s1 * s1 / n ) / ( n - 1 );
         res->sweep_ms = out[0].device_ms;
+        if ( res->replicate_means )
+            for ( int r = 0; r < replicates; r++ )
+                res->replicate_means[r] = out[r].book_sum / N;
+        const auto t2 = std::chrono::steady_clock::now();
+        res->setup_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count();
+        res->run_wall_ms = std::chrono::duration<double, std::milli>( t2 - t1 ).count();
         return THOTH_OK;
     }
     catch ( const thoth::host::Invalid& e )

@@ -30,6 +30,7 @@ BridgeProgram BuildBridgeProgram( const double* t, int n_dates );
 int EnsureDevice();
 std::mutex& DeviceMutex();  // serialises every public entry point
 hipStream_t Stream();
+hipEvent_t SweepEvent( int k );  // the two events that time a sweep (created at set-up)
 const uint32_t* DeviceDirections();
 int SetError( int code, const std::string& msg );
 double StudentT975( int df );  // two-sided 95 % quantile of Student's t
@@ -38,6 +39,7 @@ struct DevBook;
 struct DeviceBatch
 {
     char* dbase = nullptr;
+    bool external = false;  // dbase is caller-provided memory (not freed here)
     double* dpart = nullptr;
     const DevBook* dbooks = nullptr;
     const uint32_t* dshifts = nullptr;  // [replicates][kSobolDims] in RQMC mode
@@ -105,8 +107,12 @@ struct CvSums
 {
     double sum = 0, sum2 = 0;
 };
+// ext_base / ext_cap: device memory of the caller to hold the whole batch
+// (the no-allocation entry point's scratch) instead of a hipMalloc; a batch
+// that needs more than ext_cap bytes is refused before any device work
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, bool pathwise = false );
+                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, bool pathwise = false,
+                  char* ext_base = nullptr, size_t ext_cap = 0 );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
 // grid sums of every book, [nm][2 nk][2] each, from the last sweep (device reduction + copy)
 int ReduceGrids( const DeviceBatch& B, hipStream_t s );

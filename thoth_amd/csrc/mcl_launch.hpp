@@ -47,14 +47,15 @@ bool launch_sweep_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, 
                        const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts,
                        int cv_mode = 0 );
 // uniform books (every equity LSV: kind 1, or Heston: kind 2, vol-of-vol > 0,
-// uncorrelated stochastic spots, exactly na equities): false where no
-// specialised instantiation exists (the caller then runs the generic kernel)
+// uncorrelated stochastic spots, exactly na equities; kind 3: every equity LSV
+// with Cholesky-correlated spots): false where no specialised instantiation
+// exists (the caller then runs the generic kernel)
 bool launch_sweep_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
                                const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
                                const uint32_t* shifts, int cv_mode );
-// the five-equity LSV instantiations of the above (a unit of their own)
-bool launch_sweep_f32_uniform5( int cv_mode, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
-                                const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
+// the five-equity LSV instantiations of the above, kinds 1 and 3 (a unit of their own)
+bool launch_sweep_f32_uniform5( int kind, int cv_mode, int depth, dim3 grid, hipStream_t s, size_t lds,
+                                const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
                                 const uint32_t* shifts );
 bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
                                     const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first,

@@ -497,7 +497,8 @@ static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 
 // KIND: 0 any book; 1 (2) every equity of every book is LSV (Heston) with
 // vol-of-vol > 1e-12 and uncorrelated stochastic spots, and the book has
 // exactly NA equities -- the host checks this -- so the per-equity model
-// branches fold away and the equities' steps interleave in one basic block.
+// branches fold away and the equities' steps interleave in one basic block;
+// 3 as 1 with the spot noises Cholesky-correlated across the equities.
 template <class R, int NA, int DEPTH, int CV, int KIND = 0>
 __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* __restrict__ V, uint32_t bq,
                                              uint32_t first_idx, uint32_t last_idx,
@@ -506,6 +507,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
 {
     constexpr bool F64 = std::is_same_v<R, double>;
     constexpr bool uniform = KIND != 0;
+    constexpr bool lsv_kind = KIND == 1 || KIND == 3;  // every equity LSV
+    constexpr bool corr_kind = KIND == 3;              // correlated spot noises
     constexpr bool qe_phase = uniform;
     const int nas = uniform ? NA : B.n_assets;  // row length of the date-major per-step arrays
     static_assert( KIND == 0 || !F64, "uniform-book kernels are FP32" );
@@ -772,7 +775,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     // ---- equity state.  The uniform LSV kernel keeps log S in S (the leverage
     // is read at log S; the spot is exponentiated only where a payoff or the
     // control variate reads it); every other kernel keeps S.
-    constexpr bool log_spot = KIND == 1;
+    constexpr bool log_spot = lsv_kind;
     assetv S, Var, W;
 #pragma unroll
     for ( int a = 0; a < NAV; a++ )
@@ -1224,10 +1227,18 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             if constexpr ( !uniform )
                 if ( a >= na )
                     continue;
-            const int model = KIND == 1 ? 2 : KIND == 2 ? 1 : B.asset[a].model;
+            const int model = lsv_kind ? 2 : KIND == 2 ? 1 : B.asset[a].model;
             // correlated noise (correlated_noise_node.cpp:15-27, lower triangle, name order)
             R eps = nz[a];
-            if ( !uniform && na > 1 && ( model == 0 || B.corr_stoch ) )
+            if constexpr ( corr_kind )
+            {
+                // every spot correlated: the lower Cholesky row by scalar loads
+                eps = R( 0 );
+#pragma unroll
+                for ( int v = 0; v <= a; v++ )
+                    eps += nz[v] * as_const( B.cholf )[a * kMaxAssets + v];
+            }
+            else if ( !uniform && na > 1 && ( model == 0 || B.corr_stoch ) )
             {
                 eps = R( 0 );
 #pragma unroll

@@ -16,8 +16,8 @@ bool launch_sweep_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream
 {
     if ( kind == 1 && na == 1 )
         return by_cv<1, 1>( cv_mode, depth, grid, s, lds, books, V, bq0, first, last, shifts );
-    if ( kind == 1 && na == 5 )
-        return launch_sweep_f32_uniform5( cv_mode, depth, grid, s, lds, books, V, bq0, first, last, shifts );
+    if ( ( kind == 1 || kind == 3 ) && na == 5 )
+        return launch_sweep_f32_uniform5( kind, cv_mode, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     if ( kind == 2 && na == 1 )
         return by_cv<1, 2>( cv_mode, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     return false;

@@ -1,5 +1,6 @@
 // mcl_sweep_f32_uniform5.hip -- the FP32 uniform-book sweep instantiations
-// of five-equity LSV books (the basket workload), in a unit of their own so
+// of five-equity LSV books (the basket workload, uncorrelated as the
+// reference diffuses it, or with Cholesky-correlated spots), in a unit of their own so
 // that they are scheduled ILP-first (-mllvm -amdgpu-sched-strategy=max-ilp):
 // at two waves per SIMD either way the five equities' independent steps
 // interleave better (12.07 -> 11.79 ms on the basket), while the one-equity
@@ -8,10 +9,12 @@
 
 namespace thoth
 {
-bool launch_sweep_f32_uniform5( int cv_mode, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
-                                const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
+bool launch_sweep_f32_uniform5( int kind, int cv_mode, int depth, dim3 grid, hipStream_t s, size_t lds,
+                                const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
                                 const uint32_t* shifts )
 {
+    if ( kind == 3 )  // correlated spots: the plain sweep only
+        return uniform_launch::by_cv<5, 3>( cv_mode, depth, grid, s, lds, books, V, bq0, first, last, shifts );
     return uniform_launch::by_cv<5, 1>( cv_mode, depth, grid, s, lds, books, V, bq0, first, last, shifts );
 }
 }  // namespace thoth

@@ -116,12 +116,17 @@ def price(built):
     return res
 
 
-def price_rqmc(built, replicates=16, rqmc_seed=1):
+def price_rqmc(built, replicates=16, rqmc_seed=1, per_replicate=False):
     """Randomised-QMC premium of the base book with a 95 % confidence interval
-    (built.req.mcl.paths paths per replicate); returns abi.RqmcResult."""
+    (built.req.mcl.paths paths per replicate); returns abi.RqmcResult, or with
+    per_replicate=True (abi.RqmcResult, replicate means)."""
     res = abi.RqmcResult()
+    means = np.zeros(replicates)
+    if per_replicate:
+        res.replicate_means = means.ctypes.data_as(C.POINTER(C.c_double))
     _check(lib().thoth_mcl_price_rqmc(C.byref(built.req), replicates, rqmc_seed, C.byref(res)))
-    return res
+    res.replicate_means = None
+    return (res, means) if per_replicate else res
 
 
 def price_pathwise(built):
