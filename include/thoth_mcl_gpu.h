@@ -177,6 +177,23 @@ typedef struct thoth_mcl_book_out {
 int thoth_mcl_sweep(int32_t n_books, const thoth_mcl_book* books,
                     const thoth_mcl_sweep_cfg* cfg, thoth_mcl_book_out* out);
 
+/* No reference counterpart: thoth_mcl_sweep for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and the same results bit for bit;
+ * every device buffer the sweep needs (packed books, digital shifts, per-block
+ * partial sums) lives in one library-owned static device array of
+ * THOTH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  A batch
+ * that needs more is refused with THOTH_ERR_UNSUPPORTED before any device
+ * work.  The device is set up (stream, Sobol tables, the array's address) by
+ * the first call of any entry point of the library, thoth_gpu_available()
+ * included: make that call outside a capture.  The library serialises its
+ * calls; calls that overlapped on one GPU (from several threads bypassing the
+ * library's lock, or captured graphs replayed concurrently) would race on the
+ * one array, which keeps the last call's contents. */
+#define THOTH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int thoth_mcl_sweep_noalloc(int32_t n_books, const thoth_mcl_book* books,
+                            const thoth_mcl_sweep_cfg* cfg, thoth_mcl_book_out* out);
+
 #ifdef __cplusplus
 }
 #endif

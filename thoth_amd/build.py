@@ -43,6 +43,7 @@ UNITS = [
     ("mcl_cv.hip", "hipcc", []),
     ("mcl_pathwise.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_pathwise_f32.hip", "hipcc", []),
+    ("mcl_noalloc.hip", "hipcc", []),
     ("mcl_gpu.cpp", "g++", []),
     ("mcl_books.cpp", "g++", []),
     ("mcl_market.cpp", "g++", []),

@@ -164,6 +164,10 @@ int EnsureDevice()
     HIP_OK( hipMemcpy( g_dev.d_dirs, tables.data(), tables.size() * sizeof( uint32_t ), hipMemcpyHostToDevice ) );
     HIP_OK( hipEventCreate( &g_dev.ev[0] ) );
     HIP_OK( hipEventCreate( &g_dev.ev[1] ) );
+    // the device address of the no-allocation entry point's scratch array
+    int rc = NoAllocSetup();
+    if ( rc != THOTH_OK )
+        return rc;
     g_dev.ok = true;
     return THOTH_OK;
 }

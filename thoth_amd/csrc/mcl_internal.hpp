@@ -32,6 +32,9 @@ std::mutex& DeviceMutex();  // serialises every public entry point
 hipStream_t Stream();
 hipEvent_t SweepEvent( int k );  // the two events that time a sweep (created at set-up)
 const uint32_t* DeviceDirections();
+// set-up of the no-allocation entry point (mcl_noalloc.hip): the device
+// address of its static scratch array on the current device
+int NoAllocSetup();
 int SetError( int code, const std::string& msg );
 double StudentT975( int df );  // two-sided 95 % quantile of Student's t
 struct DevBook;

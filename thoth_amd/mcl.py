@@ -42,6 +42,7 @@ def lib():
         L.thoth_gpu_sobol_points.argtypes = [C.c_int32, C.c_int64, C.c_int64, P(C.c_uint32)]
         L.thoth_gpu_special_f32.argtypes = [C.c_int32, C.c_int64, P(C.c_uint32), P(C.c_float)]
         L.thoth_mcl_sweep.argtypes = [C.c_int32, P(abi.Book), P(abi.SweepCfg), P(abi.BookOut)]
+        L.thoth_mcl_sweep_noalloc.argtypes = [C.c_int32, P(abi.Book), P(abi.SweepCfg), P(abi.BookOut)]
         L.thoth_mcl_price.argtypes = [P(abi.Request), P(abi.Result)]
         L.thoth_mcl_prepare.argtypes = [P(abi.Request), P(C.c_void_p)]
         L.thoth_mcl_run.argtypes = [C.c_void_p, P(abi.Result)]
@@ -258,15 +259,20 @@ class BookArrays:
         self.book = b
 
 
-def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_seed=0):
-    """Run thoth_mcl_sweep over a list of BookArrays; returns the BookOut list
-    (len(books) * replicates entries in RQMC mode, book-major)."""
+NOALLOC_SCRATCH_BYTES = 64 << 20  # THOTH_NOALLOC_SCRATCH_BYTES
+
+
+def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_seed=0, noalloc=False):
+    """Run thoth_mcl_sweep (or, noalloc=True, thoth_mcl_sweep_noalloc, which
+    allocates no device memory) over a list of BookArrays; returns the BookOut
+    list (len(books) * replicates entries in RQMC mode, book-major)."""
     n = len(books)
     arr = (abi.Book * n)(*[b.book for b in books])
     cfg = abi.SweepCfg(paths=paths, sobol_skip=sobol_skip, precision=precision, block=0,
                        replicates=replicates, rqmc_seed=rqmc_seed)
     out = (abi.BookOut * (n * max(1, replicates)))()
-    _check(lib().thoth_mcl_sweep(n, arr, C.byref(cfg), out))
+    fn = lib().thoth_mcl_sweep_noalloc if noalloc else lib().thoth_mcl_sweep
+    _check(fn(n, arr, C.byref(cfg), out))
     return list(out)
 
 
