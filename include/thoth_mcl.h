@@ -136,6 +136,20 @@ typedef struct thoth_mcl_request {
      * does (leverage_surface.cpp:43-66). */
     int32_t leverage_spot_points;
     int32_t leverage_time_layers;
+    /* !currency discount_rate (reference: marketdata/currency.cpp:19-27, 129-142):
+     * the OIS / collateral curve the contract flows are discounted on, while the
+     * rate curve above drives the equity drifts; n = 0: the rate curve discounts
+     * (single curve).  The rho bump shifts both curves (tasks/pricer.cpp:202-217). */
+    thoth_curve discount;
+    /* term-structured !correlation_matrix (vol_correl/correlation.cpp:35-130,
+     * 528-620, 860-903): n_corr_pillars > 1 increasing pillar maturities (years)
+     * with one n_assets x n_assets matrix each (corr_matrices[p * n * n + i * n + j]);
+     * each entry is linear in time between pillars and flat outside, and the
+     * increments of step i are correlated by the Cholesky factor of the matrix
+     * averaged over [t_{i-1}, t_i].  0 or 1 pillar: the constant `correlation`. */
+    int32_t n_corr_pillars;
+    const double* corr_maturities;
+    const double* corr_matrices;
 } thoth_mcl_request;
 
 typedef struct thoth_mcl_result {

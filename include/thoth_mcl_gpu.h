@@ -145,6 +145,10 @@ typedef struct thoth_mcl_book {
     const double* c_level;       /* barrier level (continuity-corrected when continuous) */
     const int32_t* c_mon_offset; /* [n_contracts + 1] CSR into c_mon_index */
     const int32_t* c_mon_index;  /* monitored date indices */
+    /* optional [n_dates * n_assets * n_assets]: the lower Cholesky factor that
+     * correlates the increments into date i (term-structured correlation,
+     * correlation.cpp:860-903; date 0 unused); NULL: `chol` at every date */
+    const double* chol_dates;
 } thoth_mcl_book;
 
 typedef struct thoth_mcl_sweep_cfg {

@@ -387,6 +387,54 @@ double Curve::Value( double day ) const
 static inline double YearFraction( double from, double to ) { return ( to - from ) / NB_OF_DAYS_A_YEAR; }
 
 // ============================================================================
+// term-structured correlation (reference: vol_correl/correlation.cpp:528-620)
+// ============================================================================
+// Average of entry (i, j) over [a, b] (years): the entry is linear in time
+// between pillars and flat outside, integrated exactly (Correlation::AvgEntry
+// over EntryPrimitive); a degenerate window is the instantaneous value at a.
+double Market::AvgCorrelation( int i, int j, double a, double b ) const
+{
+    const size_t n = eq.size();
+    if ( !TermStructured() )
+        return correlation[i * n + j];
+    const std::vector<double>& m = corr_pillars;
+    auto entry = [&]( size_t p ) { return corr_mats[p][i * n + j]; };
+    auto primitive = [&]( double t )  // EntryPrimitive, correlation.cpp:539-559
+    {
+        if ( t <= m.front() )
+            return entry( 0 ) * t;
+        double F = entry( 0 ) * m.front();
+        for ( size_t p = 1; p < m.size(); p++ )
+        {
+            if ( t <= m[p] )
+            {
+                const double w = ( t - m[p - 1] ) / ( m[p] - m[p - 1] );
+                const double e_t = entry( p - 1 ) + w * ( entry( p ) - entry( p - 1 ) );
+                return F + ( entry( p - 1 ) + e_t ) / 2 * ( t - m[p - 1] );
+            }
+            F += ( entry( p - 1 ) + entry( p ) ) / 2 * ( m[p] - m[p - 1] );
+        }
+        return F + entry( m.size() - 1 ) * ( t - m.back() );
+    };
+    a = std::max( a, 0.0 );
+    b = std::max( b, a );
+    if ( b - a < 1e-12 )  // correlation.cpp:571-590
+    {
+        if ( a <= m.front() )
+            return entry( 0 );
+        if ( a >= m.back() )
+            return entry( m.size() - 1 );
+        for ( size_t p = 1; p < m.size(); p++ )
+            if ( a <= m[p] )
+            {
+                const double w = ( a - m[p - 1] ) / ( m[p] - m[p - 1] );
+                return entry( p - 1 ) + w * ( entry( p ) - entry( p - 1 ) );
+            }
+    }
+    return ( primitive( b ) - primitive( a ) ) / ( b - a );
+}
+
+// ============================================================================
 // SABR (reference: vol_correl/sabr_volatility.cpp:40-239)
 // ============================================================================
 double Sabr::Interp( const std::vector<double>& values, double T ) const
@@ -938,7 +986,30 @@ Market BuildMarket( const thoth_mcl_request& req )
         mk.rate.days.push_back( req.rate.days[i] );
         mk.rate.values.push_back( req.rate.values[i] * 0.01 );
     }
-    if ( req.correlation )
+    for ( int i = 0; i < req.discount.n; i++ )  // currency.cpp:19-27
+    {
+        mk.discount.days.push_back( req.discount.days[i] );
+        mk.discount.values.push_back( req.discount.values[i] * 0.01 );
+    }
+    mk.multi_curve = req.discount.n > 0;
+    const int n = req.n_assets;
+    if ( req.n_corr_pillars > 1 )
+    {
+        // one matrix per pillar (correlation.cpp:35-130); today's matrix is the
+        // instantaneous one at t = 0 (ExtractMatrix over the (0, 0) window)
+        for ( int p = 0; p < req.n_corr_pillars; p++ )
+        {
+            mk.corr_pillars.push_back( req.corr_maturities[p] );
+            mk.corr_mats.emplace_back( req.corr_matrices + (size_t)p * n * n, req.corr_matrices + (size_t)( p + 1 ) * n * n );
+        }
+        mk.correlation.assign( (size_t)n * n, 0.0 );
+        for ( int i = 0; i < n; i++ )
+            for ( int j = 0; j < n; j++ )
+                mk.correlation[i * n + j] = i == j ? 1.0 : mk.AvgCorrelation( i, j, 0.0, 0.0 );
+    }
+    else if ( req.n_corr_pillars == 1 )
+        mk.correlation.assign( req.corr_matrices, req.corr_matrices + n * n );
+    else if ( req.correlation )
         mk.correlation.assign( req.correlation, req.correlation + req.n_assets * req.n_assets );
     mk.lev_spot_points = req.leverage_spot_points;
     mk.lev_time_layers = req.leverage_time_layers;
@@ -1066,9 +1137,32 @@ ResolvedBook ResolveBook( const thoth_mcl_request& req, const Market& mk, const 
         for ( int i = 0; i < n; i++ )
             for ( int j = 0; j <= i; j++ )
                 b.chol[i * n + j] = full[i * n + j];
+        if ( mk.TermStructured() )
+        {
+            // Correlation::EnsureCholeskySchedule (correlation.cpp:860-903): the
+            // factor of date i is that of the matrix averaged over [t_{i-1}, t_i]
+            // (date 0: the instantaneous matrix at t_0)
+            for ( int d = 0; d < nd; d++ )
+            {
+                const double t1 = d == 0 ? g.t[0] : g.t[d - 1], t2 = g.t[d];
+                std::vector<double> step( (size_t)n * n, 0.0 );
+                for ( int i = 0; i < n; i++ )
+                    for ( int j = 0; j < n; j++ )
+                        step[i * n + j] = i == j ? 1.0 : mk.AvgCorrelation( i, j, t1, t2 );
+                if ( !CholeskyDecomposeLower( step, n ) )
+                    throw std::runtime_error( "step-average correlation matrix at index " + std::to_string( d ) +
+                                              " is not positive-definite" );
+                for ( int i = 0; i < n; i++ )
+                    for ( int j = i + 1; j < n; j++ )
+                        step[i * n + j] = 0;
+                b.chol_dates.push_back( std::move( step ) );
+            }
+        }
     }
+    // contract_node.cpp:33-47 over Currency::GetDiscountRateNode (currency.cpp:129-142):
+    // the OIS curve when the currency has one, else the rate curve
     for ( int i = 0; i < nd; i++ )
-        b.disc_rate.push_back( mk.rate.Value( g.days[i] ) );
+        b.disc_rate.push_back( mk.multi_curve ? mk.discount.Value( g.days[i] ) : mk.rate.Value( g.days[i] ) );
     for ( int r = 0; r < req.n_rainbows; r++ )
     {
         ResolvedBook::Rainbow rb;
@@ -1236,9 +1330,17 @@ BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, i
                 if ( n > 1 && ( !stoch || b.correlate_stochastic ) )
                 {
                     double x = 0;
-                    for ( int v = 0; v <= a; v++ )
-                        if ( b.chol[a * n + v] != 0 )  // CholeskyMayBeNonZero, correlation.cpp:923-933
-                            x += pg.Buffer( v )[i] * b.chol[a * n + v];
+                    if ( !b.chol_dates.empty() )
+                    {
+                        // term structure: every lower-triangle coefficient wired,
+                        // read at the date (correlation.cpp:906-933)
+                        for ( int v = 0; v <= a; v++ )
+                            x += pg.Buffer( v )[i] * b.chol_dates[i][a * n + v];
+                    }
+                    else
+                        for ( int v = 0; v <= a; v++ )
+                            if ( b.chol[a * n + v] != 0 )  // CholeskyMayBeNonZero, correlation.cpp:923-933
+                                x += pg.Buffer( v )[i] * b.chol[a * n + v];
                     corr[a] = x;
                 }
                 else
@@ -1418,7 +1520,8 @@ int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
     if ( req.request_greeks & THOTH_GREEK_RHO )
     {
         Market mk = base;
-        mk.rate.shift = GREEK_RATE_BUMP;
+        mk.rate.shift = GREEK_RATE_BUMP;  // projection AND discounting curves (pricer.cpp:344-359)
+        mk.discount.shift = GREEK_RATE_BUMP;
         BookSums s = run( mk );
         res->n_scenarios++;
         res->rho = ( s.Mean( -1 ) - p0 ) / GREEK_RATE_BUMP * 0.01;

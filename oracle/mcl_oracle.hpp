@@ -129,7 +129,15 @@ struct Market
 {
     std::vector<Equity> eq;
     Curve rate;
+    Curve discount;                                  // currency.cpp:19-27 discount_rate (OIS)
+    bool multi_curve = false;                        // HasDistinctDiscountCurve
     std::vector<double> correlation;
+    // term-structured correlation (correlation.cpp:35-130): pillar maturities
+    // (years) and one n x n matrix per pillar; IsTermStructured = > 1 pillar
+    std::vector<double> corr_pillars;
+    std::vector<std::vector<double>> corr_mats;
+    bool TermStructured() const { return corr_pillars.size() > 1; }
+    double AvgCorrelation( int i, int j, double a, double b ) const;  // Correlation::AvgEntry
     int lev_spot_points = 0, lev_time_layers = 0;    // 0, 0: one 201-point layer per diffusion date
     double today = 0;                                // days
     StochParams Stoch( int a ) const;                // StochasticParams()
@@ -176,6 +184,7 @@ struct ResolvedBook
     std::vector<double> spot, vol, v0, kappa, theta, xi, rho;
     std::vector<std::vector<double>> drift;          // [asset][date]
     std::vector<double> chol;
+    std::vector<std::vector<double>> chol_dates;     // per-date factors (term-structured correlation)
     bool correlate_stochastic = false;
     std::vector<LeverageLayers> lev;                 // per asset (LSV only)
     std::vector<double> disc_rate;

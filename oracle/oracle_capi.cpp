@@ -192,6 +192,45 @@ int oracle_grid( const thoth_mcl_request* req, int cap, double* days, double* t,
     }
 }
 
+// the spot/spot correlation of the request averaged over [a, b] years, n x n
+int oracle_avg_correlation( const thoth_mcl_request* req, double a, double b, double* out )
+{
+    try
+    {
+        Market mk = BuildMarket( *req );
+        const int n = req->n_assets;
+        for ( int i = 0; i < n; i++ )
+            for ( int j = 0; j < n; j++ )
+                out[i * n + j] = i == j ? 1.0 : mk.AvgCorrelation( i, j, a, b );
+        return 0;
+    }
+    catch ( const std::exception& e )
+    {
+        return -Fail( e );
+    }
+}
+
+// per-date lower Cholesky factors of a term-structured correlation [dates][n][n];
+// returns the date count (0 for a constant matrix)
+int oracle_chol_schedule( const thoth_mcl_request* req, int cap, double* out )
+{
+    try
+    {
+        Market mk = BuildMarket( *req );
+        Grid g = BuildGrid( *req, mk );
+        ResolvedBook b = ResolveBook( *req, mk, g );
+        const int n = req->n_assets;
+        const int nd = (int)b.chol_dates.size();
+        for ( int d = 0; d < nd && d < cap; d++ )
+            std::memcpy( out + (size_t)d * n * n, b.chol_dates[d].data(), sizeof( double ) * n * n );
+        return nd;
+    }
+    catch ( const std::exception& e )
+    {
+        return -Fail( e );
+    }
+}
+
 // leverage layers of an LSV asset on the request's grid (base market + optional spot / vol shift)
 int oracle_calibrate( const thoth_mcl_request* req, int asset, double spot, double vol_shift, double rate_shift, int cap,
                       double* t, double* ln_step, int64_t* offset, double* levels )

@@ -43,6 +43,8 @@ def lib():
     L.oracle_local_vol.restype = C.c_double
     L.oracle_cholesky.argtypes = [C.c_int, P(C.c_double)]
     L.oracle_grid.argtypes = [P(abi.Request), C.c_int] + [P(C.c_double)] * 3
+    L.oracle_avg_correlation.argtypes = [P(abi.Request), C.c_double, C.c_double, P(C.c_double)]
+    L.oracle_chol_schedule.argtypes = [P(abi.Request), C.c_int, P(C.c_double)]
     L.oracle_calibrate.argtypes = [P(abi.Request), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                    P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_double)]
     L.oracle_sweep.argtypes = [P(abi.Request), P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double),
@@ -115,6 +117,26 @@ def sabr_vol(surface, K, F, day):
 def sabr_hagan(surface, K, F, T):
     """The Hagan expansion alone (no wings) at maturity T in years."""
     return lib().oracle_sabr_hagan(C.byref(surface), K, F, T)
+
+
+def avg_correlation(built, a, b):
+    """The request's spot/spot correlation averaged over [a, b] years."""
+    n = built.req.n_assets
+    out = np.zeros((n, n))
+    rc = lib().oracle_avg_correlation(C.byref(built.req), a, b, _p(out))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return out
+
+
+def chol_schedule(built, cap=100000):
+    """Per-date lower Cholesky factors of a term-structured correlation, [dates, n, n]."""
+    n = built.req.n_assets
+    out = np.zeros((cap, n, n))
+    k = lib().oracle_chol_schedule(C.byref(built.req), cap, _p(out))
+    if k < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return out[:k].copy()
 
 
 def local_vol(built, asset, K, day):

@@ -2,7 +2,9 @@
 mixed models (Black-Scholes, Heston, LSV on a SABR surface), equicorrelated,
 with vanillas and barriers (every barrier type, discrete or continuous,
 absolute or relative strikes) on single equities and best/worst-of
-rainbows, short maturities so the oracle's LSV calibration stays fast."""
+rainbows, some with a distinct OIS discount curve or a two-pillar
+correlation term structure, short maturities so the oracle's LSV
+calibration stays fast."""
 import numpy as np
 
 from thoth_amd import abi
@@ -53,10 +55,20 @@ def random_book(seed, paths=1 << 10):
         contracts.append(Barrier(udl, days, strike, btype, level, is_call=call, discrete=discrete,
                                  monitoring_period_days=int(rng.integers(1, 6)), is_absolute_strike=absolute))
     lsv = any(e.vol_kind == abi.VOL_LSV for e in eqs)
-    return BookSpec(mcl=McConfig(paths=paths, max_day_step=int(rng.integers(1, 4)),
+    spec = BookSpec(mcl=McConfig(paths=paths, max_day_step=int(rng.integers(1, 4)),
                                  sobol_skip=int(rng.integers(0, 5000))),
                     equities=eqs, contracts=contracts, correlation=corr,
                     rate_days=[0, 30], rate_values=[float(rng.uniform(0, 4)), float(rng.uniform(0, 4))],
                     rainbows=rainbows, correlate_stochastic=bool(rng.random() < 0.3),
                     leverage_surface=(40, 8) if lsv and rng.random() < 0.5 else None,
                     greeks=int(rng.choice([0, abi.GREEK_DELTA | abi.GREEK_RHO])))
+    # drawn after everything above, so the earlier draws of each seed are unchanged:
+    # a distinct OIS discount curve, and a two-pillar correlation term structure
+    if rng.random() < 0.35:
+        spec.discount_days = [0, 45]
+        spec.discount_values = [float(rng.uniform(0, 4)), float(rng.uniform(0, 4))]
+    if n > 1 and rng.random() < 0.35:
+        rho2 = float(rng.uniform(-0.2, 0.7))
+        spec.correlation_maturities = [float(rng.uniform(0.01, 0.05)), 0.1]
+        spec.correlation_pillars = [corr, [[1.0 if i == j else rho2 for j in range(n)] for i in range(n)]]
+    return spec

@@ -21,7 +21,7 @@ REL = 1e-10
 
 def _cases():
     out = []
-    for f in ("simple_call", "lsv", "matrix"):
+    for f in ("simple_call", "lsv", "matrix", "multicurve"):
         objs = yf.load((SAMPLES / (f + ".yaml")).read_text())
         for n in yf.mcl_pricers(objs):
             try:

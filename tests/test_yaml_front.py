@@ -26,7 +26,7 @@ SUPPORTED = {
     "barrier_mono_continuous_in_call_mcl", "barrier_mono_continuous_down_out_call_mcl",
     "barrier_mono_continuous_down_in_call_mcl", "barrier_mono_continuous_up_out_put_mcl",
     "barrier_mono_discrete_in_call_mcl", "vanilla_mono_eu_put_mcl", "vanilla_mono_eu_call_mcl_gpu",
-    "vanilla_mono_eu_put_mcl_gpu", "vanilla_mono_eu_call_lsv_mcl",
+    "vanilla_mono_eu_put_mcl_gpu", "vanilla_mono_eu_call_lsv_mcl", "vanilla_mono_eu_call_ois_mcl",
 }
 
 REFUSED = {  # pricer -> a word of the reason
@@ -34,8 +34,8 @@ REFUSED = {  # pricer -> a word of the reason
     "vanilla_mono_eu_call_quanto_mcl": "currency", "vanilla_mono_eu_call_compo_mcl": "composite",
     "vanilla_multi_eu_call_basket_mcl": "basket", "vanilla_mono_eu_call_bates_mcl": "Bates",
     "variance_mono_mcl": "variance", "vanilla_mono_eu_call_sabr_mcl": "sabr_volatility",
-    "vanilla_mono_eu_call_mcl_pseudo": "use_sobol", "basket_term_correl_mcl": "term-structured",
-    "vanilla_mono_eu_call_ois_mcl": "discount_rate", "vanilla_mono_eu_call_hw_mcl": "rate_model",
+    "vanilla_mono_eu_call_mcl_pseudo": "use_sobol", "basket_term_correl_mcl": "basket",
+    "vanilla_mono_eu_call_hw_mcl": "rate_model",
     "autocallable_athena_mcl": "autocallable", "asian_mono_eu_call_mcl": "asian",
     "ratchet_mono_mcl": "ratchet", "digital_mono_cash_call_mcl": "digital",
 }
@@ -124,3 +124,18 @@ def test_theta_is_the_one_day_roll(oracle):
         return 100 * norm.cdf(d1) - 100 * math.exp(-0.08 * T) * norm.cdf(d1 - 0.3 * math.sqrt(T))
     ref = bs(364 / 365) - bs(1.0)
     assert ref < 0 and abs(theta - ref) < 2e-3, (theta, ref)
+
+
+def test_multicurve_sample_discounts_on_its_ois_curve(oracle):
+    # pricer/samples/multicurve.yaml: projection curve 8 %, OIS curve 5 %; the
+    # MCL task prices the 1y ATM call with the BS forward on the projection
+    # curve and the OIS discount factor (test_multicurve.cpp:54-67)
+    objs = _objs("multicurve")
+    assert yf.mcl_pricers(objs) == ["call_mcl"]
+    t = yf.translate(objs, "call_mcl", paths=1 << 15)
+    assert list(t.spec.rate_values) == [8.0, 8.0] and list(t.spec.discount_values) == [5.0, 5.0]
+    r = oracle.price(BuiltRequest(t.spec))
+    T = 365 / 365.0
+    d1 = (0.08 + 0.045) * T / (0.3 * math.sqrt(T))
+    ref = math.exp(-0.05 * T) * (100 * math.exp(0.08 * T) * norm.cdf(d1) - 100 * norm.cdf(d1 - 0.3 * math.sqrt(T)))
+    assert abs(r.premium - ref) <= 6 * r.premium_trust + 1e-2

@@ -69,7 +69,9 @@ class Request(C.Structure):
                 ("rainbows", C.POINTER(Rainbow)), ("n_contracts", _i32),
                 ("contracts", C.POINTER(Contract)), ("request_greeks", _i32),
                 ("precision", _i32), ("correlate_stochastic", _i32), ("uniform_steps", _i32),
-                ("leverage_spot_points", _i32), ("leverage_time_layers", _i32)]
+                ("leverage_spot_points", _i32), ("leverage_time_layers", _i32),
+                ("discount", Curve), ("n_corr_pillars", _i32), ("corr_maturities", _pd),
+                ("corr_matrices", _pd)]
 
 
 class Result(C.Structure):
@@ -92,7 +94,7 @@ class Book(C.Structure):
                 ("n_contracts", _i32), ("c_kind", _pi32), ("c_is_call", _pi32),
                 ("c_underlying", _pi32), ("c_flow_index", _pi32), ("c_strike", _pd),
                 ("c_is_up", _pi32), ("c_is_in", _pi32), ("c_level", _pd),
-                ("c_mon_offset", _pi32), ("c_mon_index", _pi32)]
+                ("c_mon_offset", _pi32), ("c_mon_index", _pi32), ("chol_dates", _pd)]
 
 
 class SweepCfg(C.Structure):

@@ -95,6 +95,14 @@ class BookSpec:
     # LSV leverage layout: (spot points, time layers) of one shared log-spot axis,
     # or None for one 201-point layer per diffusion date
     leverage_surface: Optional[Tuple[int, int]] = None
+    # `!currency discount_rate`: the OIS curve the flows are discounted on
+    # (pillar days, percent zero rates); empty: the rate curve discounts
+    discount_days: Sequence[int] = ()
+    discount_values: Sequence[float] = ()
+    # term-structured `!correlation_matrix`: pillar maturities (years) and one
+    # n x n matrix per pillar; empty: the constant `correlation`
+    correlation_maturities: Sequence[float] = ()
+    correlation_pillars: Sequence[Sequence[Sequence[float]]] = ()
 
 
 class BuiltRequest:
@@ -172,6 +180,18 @@ class BuiltRequest:
         r.uniform_steps = spec.uniform_steps
         if spec.leverage_surface is not None:
             r.leverage_spot_points, r.leverage_time_layers = spec.leverage_surface
+        r.discount.n = len(spec.discount_days)
+        for i, (d, v) in enumerate(zip(spec.discount_days, spec.discount_values)):
+            r.discount.days[i] = d
+            r.discount.values[i] = v
+        npl = len(spec.correlation_maturities)
+        if npl:
+            self.corr_mats = (C.c_double * npl)(*[float(x) for x in spec.correlation_maturities])
+            self.corr_pillars = (C.c_double * (npl * n * n))(
+                *[float(m[i][j]) for m in spec.correlation_pillars for i in range(n) for j in range(n)])
+            r.n_corr_pillars = npl
+            r.corr_maturities = self.corr_mats
+            r.corr_matrices = self.corr_pillars
         self.req = r
 
     def result(self):

@@ -45,7 +45,7 @@ struct Arena
 struct Offsets
 {
     size_t t, dt, sqrt_dt, bsq, binv, step_ops, op_b, op_left, op_w, op_wf, drift, sc, lev, levf, lnstep, offset,
-        offsetf, invstep, mon, flow, cv_start, cv_end, cv_scale, grid_mat_at, grid_kf, grid_kd;
+        offsetf, invstep, mon, flow, cv_start, cv_end, cv_scale, grid_mat_at, grid_kf, grid_kd, chol, cholf;
 };
 
 std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
@@ -110,14 +110,24 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
         return SetError( THOTH_ERR_UNSUPPORTED,
                          Why( bi, "steps x factors exceeds the 4097 Sobol dimensions (the reference draws the "
                                   "tail pseudo-randomly)" ) );
-    if ( na > 1 && !in.chol )
+    if ( na > 1 && !in.chol && !in.chol_dates )
         return SetError( THOTH_ERR_INVALID, Why( bi, "multi-equity book needs a Cholesky factor" ) );
-    for ( int a = 0; a < na; a++ )
-        for ( int v = 0; v <= a; v++ )
+    of.chol = of.cholf = 0;
+    if ( na > 1 )
+    {
+        // per-date lower factors [date][kMaxAssets][kMaxAssets], FP64 and FP32
+        std::vector<double> ct( (size_t)nd * kMaxAssets * kMaxAssets, 0.0 );
+        for ( int i = 0; i < nd; i++ )
         {
-            db.chol[a * kMaxAssets + v] = na > 1 ? in.chol[a * na + v] : 1.0;
-            db.cholf[a * kMaxAssets + v] = (float)db.chol[a * kMaxAssets + v];
+            const double* L = in.chol_dates ? in.chol_dates + (size_t)i * na * na : in.chol;
+            for ( int a = 0; a < na; a++ )
+                for ( int v = 0; v <= a; v++ )
+                    ct[( (size_t)i * kMaxAssets + a ) * kMaxAssets + v] = L[a * na + v];
         }
+        std::vector<float> cf( ct.begin(), ct.end() );
+        of.chol = ar.put( ct );
+        of.cholf = ar.put( cf );
+    }
     // grid derived arrays
     std::vector<double> sqrt_dt( nd, 0.0 ), bsq( nd, 0.0 );
     std::vector<float> binv( nd, 0.0f );
@@ -632,6 +642,11 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         d.op_wf = At<float>( dbase, o.op_wf );
         d.drift = At<double>( dbase, o.drift );
         d.sc = At<float>( dbase, o.sc );
+        if ( d.n_assets > 1 )
+        {
+            d.chol_t = At<double>( dbase, o.chol );
+            d.cholf_t = At<float>( dbase, o.cholf );
+        }
         if ( d.lev_points )
         {
             d.lev = At<double>( dbase, o.lev );

@@ -92,8 +92,11 @@ struct DevBook
     const float* op_wf;   // FP32 copy, left weight 0 where the op has no left anchor
     // equities
     DevAsset asset[kMaxAssets];
-    double chol[kMaxAssets * kMaxAssets];
-    float cholf[kMaxAssets * kMaxAssets];
+    // lower Cholesky factor that correlates the increments into date i,
+    // [n_dates][kMaxAssets][kMaxAssets] (the same factor at every date unless
+    // the correlation is term-structured); multi-equity books only
+    const double* chol_t;
+    const float* cholf_t;
     const double* drift;  // [n_assets * n_dates] drift-node value
     const float* sc;      // date-major [n_dates][n_assets][SC_COUNT]
     const double* lev;    // [n_assets * n_dates * lev_points]

@@ -211,19 +211,8 @@ int PrepareGridPlan( int n, const thoth_mcl_request* reqs, const thoth_vanilla_g
         const Market base = BuildMarket( q );
         grids_alive.push_back( BuildGrid( q, base, &days ) );
         const Grid& g = grids_alive.back();
-        std::vector<double> chol;
+        const std::vector<double> chol = SpotCholesky( q, base );
         const int na = q.n_assets;
-        if ( na > 1 )
-        {
-            if ( !q.correlation )
-                throw Invalid( THOTH_ERR_INVALID, "book pricing requires a correlation matrix" );
-            chol.assign( q.correlation, q.correlation + na * na );
-            if ( !CholeskyLower( chol, na ) )
-                throw Invalid( THOTH_ERR_INVALID, "correlation is not SDP" );
-            for ( int i = 0; i < na; i++ )
-                for ( int k = i + 1; k < na; k++ )
-                    chol[i * na + k] = 0;
-        }
         GridSpec gs;
         gs.nm = vg.n_maturities;
         gs.nk = vg.n_strikes;

@@ -135,8 +135,30 @@ void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, con
             A.drift.push_back( mk.Drift( a, g.days[i] ) );
     }
     A.chol = chol;
+    if ( n > 1 && mk.TermStructured() )
+    {
+        // one factor per date, of the matrix averaged over the step into it
+        // (date 0: the instantaneous matrix at t_0), correlation.cpp:860-903
+        A.chol_dates.assign( (size_t)nd * n * n, 0.0 );
+        std::vector<double> step( (size_t)n * n );
+        for ( int d = 0; d < nd; d++ )
+        {
+            const double t1 = g.t[d > 0 ? d - 1 : 0], t2 = g.t[d];
+            for ( int i = 0; i < n; i++ )
+                for ( int j = 0; j < n; j++ )
+                    step[i * n + j] = i == j ? 1.0 : mk.AvgCorrelation( i, j, t1, t2 );
+            if ( !CholeskyLower( step, n ) )
+                throw Invalid( THOTH_ERR_INVALID, "step-average correlation matrix at date index " +
+                                                      std::to_string( d ) + " is not positive-definite" );
+            for ( int i = 0; i < n; i++ )
+                for ( int j = 0; j <= i; j++ )
+                    A.chol_dates[( (size_t)d * n + i ) * n + j] = step[i * n + j];
+        }
+    }
+    // the flows are discounted on the OIS curve of a multi-curve currency
+    // (contract.cpp:74 over currency.cpp:129-142), else on the rate curve
     for ( int i = 0; i < nd; i++ )
-        A.disc.push_back( mk.rate.Value( g.days[i] ) );
+        A.disc.push_back( mk.multi_curve ? mk.discount.Value( g.days[i] ) : mk.rate.Value( g.days[i] ) );
     bool any_lsv = false;
     for ( int a = 0; a < n; a++ )
         any_lsv |= mk.eq[a].kind == THOTH_VOL_LSV;
@@ -258,6 +280,7 @@ void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, con
     b.rho = A.rho.data();
     b.drift = A.drift.data();
     b.chol = A.chol.empty() ? nullptr : A.chol.data();
+    b.chol_dates = A.chol_dates.empty() ? nullptr : A.chol_dates.data();
     b.correlate_stochastic = req.correlate_stochastic;
     b.lev_points = P;
     b.lev_ln_step = A.lev_ln_step.empty() ? nullptr : A.lev_ln_step.data();
@@ -333,19 +356,8 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     const Market base = BuildMarket( req );
     const Grid g = BuildGrid( req, base );
     // Cholesky of the spot/spot sub-matrix (pricer_mcl.cpp:462-491, correlation.cpp:794-859)
-    std::vector<double> chol;
+    const std::vector<double> chol = SpotCholesky( req, base );
     const int n = req.n_assets;
-    if ( n > 1 )
-    {
-        if ( !req.correlation )
-            throw Invalid( THOTH_ERR_INVALID, "book pricing requires a correlation matrix" );
-        chol.assign( req.correlation, req.correlation + n * n );
-        if ( !CholeskyLower( chol, n ) )
-            throw Invalid( THOTH_ERR_INVALID, "correlation is not SDP" );
-        for ( int i = 0; i < n; i++ )
-            for ( int j = i + 1; j < n; j++ )
-                chol[i * n + j] = 0;
-    }
     // scenarios of the bump-and-revalue Greeks (pricer.cpp:287-380)
     std::vector<Scenario> sc;
     sc.push_back( { base, 0, -1, 0 } );
@@ -377,7 +389,8 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     if ( req.request_greeks & THOTH_GREEK_RHO )
     {
         Scenario s{ base, 5, -1, kRateBump };
-        s.mk.rate.shift = kRateBump;
+        s.mk.rate.shift = kRateBump;  // the projection AND the discounting curve (pricer.cpp:344-359)
+        s.mk.discount.shift = kRateBump;
         sc.push_back( s );
     }
     // leverage calibration per scenario (each scenario rebuilds the leverage node, single.cpp:479-500)
@@ -498,19 +511,8 @@ int ResolveBaseBook( const thoth_mcl_request& req, BookArrays& A, Grid& g )
         throw Invalid( THOTH_ERR_UNSUPPORTED, "1..32 contracts per book" );
     const Market mk = BuildMarket( req );
     g = BuildGrid( req, mk );
-    std::vector<double> chol;
+    const std::vector<double> chol = SpotCholesky( req, mk );
     const int n = req.n_assets;
-    if ( n > 1 )
-    {
-        if ( !req.correlation )
-            throw Invalid( THOTH_ERR_INVALID, "book pricing requires a correlation matrix" );
-        chol.assign( req.correlation, req.correlation + n * n );
-        if ( !CholeskyLower( chol, n ) )
-            throw Invalid( THOTH_ERR_INVALID, "correlation is not SDP" );
-        for ( int i = 0; i < n; i++ )
-            for ( int j = i + 1; j < n; j++ )
-                chol[i * n + j] = 0;
-    }
     std::vector<LeverageGrid> lev( n );
     for ( int a = 0; a < n; a++ )
         if ( mk.eq[a].kind == THOTH_VOL_LSV )

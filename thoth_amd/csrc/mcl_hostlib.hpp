@@ -59,6 +59,16 @@ struct Market
 {
     std::vector<Equity> eq;
     Curve rate;
+    Curve discount;            // the OIS curve of a multi-curve currency (currency.cpp:19-27)
+    bool multi_curve = false;
+    // spot/spot correlation: today's matrix, and for a term structure the
+    // pillar maturities (years) with one matrix each (correlation.cpp:35-130)
+    std::vector<double> correlation;
+    std::vector<double> corr_pillars;
+    std::vector<std::vector<double>> corr_mats;
+    bool TermStructured() const { return corr_pillars.size() > 1; }
+    // entry (i, j) averaged over [a, b] years (correlation.cpp:528-620)
+    double AvgCorrelation( int i, int j, double a, double b ) const;
     int lev_spot_points = 0, lev_time_layers = 0;  // 0, 0: one 201-point layer per diffusion date
     StochParams Stoch( int a ) const;                       // StochasticParams()
     double DivRepo( int a, double day ) const;              // equity.cpp:149-154
@@ -98,6 +108,7 @@ struct BookArrays
 {
     std::vector<int32_t> model;
     std::vector<double> spot, vol, v0, kappa, theta, xi, rho, drift, chol, disc;
+    std::vector<double> chol_dates;  // [dates][n][n] per-date factors (term-structured correlation)
     std::vector<double> lev_ln_step, lev_values;
     std::vector<int64_t> lev_offset;
     std::vector<int32_t> rb_best, rb_n, rb_member;
@@ -119,6 +130,9 @@ Grid BuildGrid( const thoth_mcl_request& req, const Market& mk, const std::vecto
 void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, const std::vector<double>& chol,
               const std::vector<LeverageGrid>& lev, BookArrays& A );
 bool CholeskyLower( std::vector<double>& a, int n );  // maths.cpp:354-381
+// the lower factor of today's spot/spot matrix (empty for one equity); throws
+// Invalid when the matrix is missing or not positive-definite
+std::vector<double> SpotCholesky( const thoth_mcl_request& req, const Market& mk );
 int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& out );
 // the base scenario's device book (grid, Cholesky, LSV calibration, payoffs);
 // throws on invalid input like the other host entry points

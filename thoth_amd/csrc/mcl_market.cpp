@@ -297,6 +297,54 @@ Market BuildMarket( const thoth_mcl_request& req )
         mk.rate.days.push_back( 0 );
         mk.rate.values.push_back( 0 );
     }
+    // multi-curve: the flows are discounted on the OIS curve (currency.cpp:129-142)
+    if ( req.discount.n < 0 || req.discount.n > THOTH_MAX_PILLARS )
+        throw Invalid( THOTH_ERR_INVALID, "discount curve: 0..16 pillars" );
+    for ( int i = 0; i < req.discount.n; i++ )
+    {
+        mk.discount.days.push_back( req.discount.days[i] );
+        mk.discount.values.push_back( req.discount.values[i] * 0.01 );
+    }
+    mk.multi_curve = req.discount.n > 0;
+    // spot/spot correlation, constant or term-structured (correlation.cpp:35-130)
+    const int n = req.n_assets;
+    if ( req.n_corr_pillars < 0 || req.n_corr_pillars > THOTH_MAX_PILLARS )
+        throw Invalid( THOTH_ERR_INVALID, "correlation: 0..16 pillars" );
+    if ( req.n_corr_pillars > 0 )
+    {
+        if ( !req.corr_maturities || !req.corr_matrices )
+            throw Invalid( THOTH_ERR_INVALID, "correlation pillars need their maturities and matrices" );
+        for ( int p = 0; p < req.n_corr_pillars; p++ )
+        {
+            if ( !( req.corr_maturities[p] > 0 ) || ( p > 0 && !( req.corr_maturities[p] > req.corr_maturities[p - 1] ) ) )
+                throw Invalid( THOTH_ERR_INVALID, "correlation maturities must be positive and strictly increasing" );
+            const double* m = req.corr_matrices + (size_t)p * n * n;
+            for ( int i = 0; i < n; i++ )
+                for ( int j = 0; j < n; j++ )
+                    if ( !( std::fabs( m[i * n + j] - m[j * n + i] ) <= 1e-12 ) || !( std::fabs( m[i * n + j] ) <= 1.0 ) ||
+                         ( i == j && m[i * n + j] != 1.0 ) )
+                        throw Invalid( THOTH_ERR_INVALID, "correlation pillar " + std::to_string( p ) +
+                                                              " is not a correlation matrix" );
+            mk.corr_mats.emplace_back( m, m + n * n );
+            mk.corr_pillars.push_back( req.corr_maturities[p] );
+        }
+        if ( req.n_corr_pillars == 1 )
+        {
+            mk.correlation = mk.corr_mats[0];
+            mk.corr_mats.clear();
+            mk.corr_pillars.clear();
+        }
+        else
+        {
+            // today's matrix: the instantaneous one at t = 0 (the (0, 0) window)
+            mk.correlation.assign( (size_t)n * n, 0.0 );
+            for ( int i = 0; i < n; i++ )
+                for ( int j = 0; j < n; j++ )
+                    mk.correlation[i * n + j] = i == j ? 1.0 : mk.AvgCorrelation( i, j, 0.0, 0.0 );
+        }
+    }
+    else if ( req.correlation )
+        mk.correlation.assign( req.correlation, req.correlation + n * n );
     const int sp = req.leverage_spot_points, tl = req.leverage_time_layers;
     if ( sp != 0 || tl != 0 )
     {
@@ -306,6 +354,67 @@ Market BuildMarket( const thoth_mcl_request& req )
         mk.lev_time_layers = tl;
     }
     return mk;
+}
+
+// Entry (i, j) of the term structure is e(t): flat at the first pillar's value
+// before it, linear between pillars, flat after the last.  Its mean over
+// [a, b] is the exact integral of that piecewise-linear function (trapezes on
+// the pieces [a, b] cuts) over b - a; a window shorter than 1e-12 years gives
+// e(a) (correlation.cpp:528-620).
+double Market::AvgCorrelation( int i, int j, double a, double b ) const
+{
+    const size_t n = eq.size();
+    if ( !TermStructured() )
+        return correlation[i * n + j];
+    const std::vector<double>& m = corr_pillars;
+    const size_t np = m.size();
+    auto at = [&]( double t )
+    {
+        if ( t <= m[0] )
+            return corr_mats[0][i * n + j];
+        if ( t >= m[np - 1] )
+            return corr_mats[np - 1][i * n + j];
+        size_t p = 1;
+        while ( m[p] < t )
+            p++;
+        const double w = ( t - m[p - 1] ) / ( m[p] - m[p - 1] );
+        return corr_mats[p - 1][i * n + j] + w * ( corr_mats[p][i * n + j] - corr_mats[p - 1][i * n + j] );
+    };
+    a = std::max( a, 0.0 );
+    b = std::max( b, a );
+    if ( b - a < 1e-12 )
+        return at( a );
+    // breakpoints inside (a, b), then trapezes between consecutive ones
+    double area = 0.0, x0 = a, e0 = at( a );
+    for ( size_t p = 0; p <= np; p++ )
+    {
+        const double x1 = p < np ? std::min( std::max( m[p], a ), b ) : b;
+        if ( x1 <= x0 )
+            continue;
+        const double e1 = at( x1 );
+        area += 0.5 * ( e0 + e1 ) * ( x1 - x0 );
+        x0 = x1;
+        e0 = e1;
+    }
+    return area / ( b - a );
+}
+
+std::vector<double> SpotCholesky( const thoth_mcl_request& req, const Market& mk )
+{
+    // the spot/spot sub-matrix in name order (pricer_mcl.cpp:462-491, correlation.cpp:794-859)
+    const int n = req.n_assets;
+    std::vector<double> chol;
+    if ( n < 2 )
+        return chol;
+    if ( mk.correlation.size() != (size_t)n * n )
+        throw Invalid( THOTH_ERR_INVALID, "book pricing requires a correlation matrix" );
+    chol = mk.correlation;
+    if ( !CholeskyLower( chol, n ) )
+        throw Invalid( THOTH_ERR_INVALID, "correlation is not SDP" );
+    for ( int i = 0; i < n; i++ )
+        for ( int j = i + 1; j < n; j++ )
+            chol[i * n + j] = 0;
+    return chol;
 }
 
 bool CholeskyLower( std::vector<double>& A, int n )

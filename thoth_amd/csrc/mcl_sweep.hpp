@@ -1230,13 +1230,16 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             const int model = lsv_kind ? 2 : KIND == 2 ? 1 : B.asset[a].model;
             // correlated noise (correlated_noise_node.cpp:15-27, lower triangle, name order)
             R eps = nz[a];
+            // the factor of the step into date i (term-structured correlation:
+            // one per date, correlation.cpp:860-903), read by scalar loads
+            const size_t crow = ( (size_t)i * kMaxAssets + a ) * kMaxAssets;
             if constexpr ( corr_kind )
             {
-                // every spot correlated: the lower Cholesky row by scalar loads
+                // every spot correlated
                 eps = R( 0 );
 #pragma unroll
                 for ( int v = 0; v <= a; v++ )
-                    eps += nz[v] * as_const( B.cholf )[a * kMaxAssets + v];
+                    eps += nz[v] * as_const( B.cholf_t )[crow + v];
             }
             else if ( !uniform && na > 1 && ( model == 0 || B.corr_stoch ) )
             {
@@ -1246,11 +1249,12 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 {
                     if constexpr ( F64 )
                     {
-                        if ( B.chol[a * kMaxAssets + v] != 0.0 )
-                            eps += nz[v] * B.chol[a * kMaxAssets + v];
+                        const double c = as_const( B.chol_t )[crow + v];
+                        if ( c != 0.0 )
+                            eps += nz[v] * c;
                     }
                     else
-                        eps += nz[v] * B.cholf[a * kMaxAssets + v];
+                        eps += nz[v] * as_const( B.cholf_t )[crow + v];
                 }
             }
             cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;

@@ -115,12 +115,17 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
     today += datetime.timedelta(days=roll_days)
     ccy_name = p["currency"]
     ccy = _obj(objs, ccy_name, "currency")
-    for key in ("discount_rate", "rate_model"):
-        if key in ccy:
-            raise Unsupported("currency '%s' has a %s" % (ccy_name, key))
+    if "rate_model" in ccy:
+        raise Unsupported("currency '%s' has a rate_model" % ccy_name)
     curve = _obj(objs, ccy["rate"], "yield_curve")
     rate_days = [_days(today, d) for d in curve["dates"]]
     rate_values = [float(v) for v in curve["values"]]
+    # a distinct discount_rate (OIS) curve discounts the flows (marketdata/currency.cpp:19-27, 129-142)
+    disc_days, disc_values = [], []
+    if "discount_rate" in ccy and ccy["discount_rate"] != ccy["rate"]:
+        dc = _obj(objs, ccy["discount_rate"], "yield_curve")
+        disc_days = [_days(today, d) for d in dc["dates"]]
+        disc_values = [float(v) for v in dc["values"]]
 
     mc = _obj(objs, p["mcl_configuration"], "mcl_configuration")
     if not mc.get("use_sobol", False):
@@ -131,19 +136,28 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
                    allow_gpu=bool(mc.get("allow_gpu", False)))
 
     corr = _obj(objs, p["correlation"], "correlation_matrix") if "correlation" in p else None
-    if corr is not None and "maturities" in corr:
-        raise Unsupported("term-structured correlation '%s'" % p["correlation"])
     if corr is not None and "matrix" not in corr:
         raise Unsupported("correlation '%s' is not given as a full matrix" % p["correlation"])
-    # rows: the underlyings, then the forexs (vol_correl/correlation.cpp:120-150)
+    # rows: the underlyings, then the forexs (vol_correl/correlation.cpp:120-150);
+    # with `maturities` the matrix field concatenates one matrix per pillar
+    # (correlation.cpp:35-83)
     names = list(corr.get("underlyings", [])) + list(corr.get("forexs", [])) if corr else []
-    cmat = _flat_matrix(corr, len(names)) if corr else []
+    pillars = [float(x) for x in corr.get("maturities", [])] if corr else []
+    cmats = []
+    if corr:
+        flat = list(corr["matrix"])
+        flat = [x for row in flat for x in row] if flat and isinstance(flat[0], list) else flat
+        k = max(1, len(pillars))
+        nn = len(names) * len(names)
+        if len(flat) != k * nn:
+            raise ValueError("correlation matrix length is not %d x %d per pillar" % (len(names), len(names)))
+        cmats = [_flat_matrix({"matrix": flat[q * nn:(q + 1) * nn]}, len(names)) for q in range(k)]
 
-    def correl(a, b):
+    def correl(a, b, q=0):
         if a == b:
             return 1.0
         if a in names and b in names:
-            return float(cmat[names.index(a)][names.index(b)])
+            return float(cmats[q][names.index(a)][names.index(b)])
         return 0.0
 
     equities, eq_names, rainbows, rb_names = [], [], [], []
@@ -179,6 +193,9 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
             if corr is None:
                 raise Unsupported("stochastic-vol equity '%s' needs a correlation matrix" % name)
             out.rho_spot_var = correl(name, name + "_var")
+            if any(correl(name, name + "_var", q) != out.rho_spot_var for q in range(len(cmats))):
+                # correlation.cpp:270-285: the engines read one scalar spot/variance rho
+                raise ValueError("spot/variance entries of '%s' must be identical across maturities" % name)
             if out.vol_kind == abi.VOL_LSV:
                 s = _obj(objs, vol["surface"], "sabr_volatility")
                 out.surface = Sabr([float(x) for x in s["maturities"]], [float(x) for x in s["alpha"]],
@@ -234,10 +251,14 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
     greeks = [g for g in indicators if g in _GREEKS or g == "theta"]
     not_computed = [g for g in indicators if g != "premium" and g not in greeks]
     n = len(equities)
-    spec = BookSpec(mcl=cfg, equities=equities, contracts=contracts,
-                    correlation=[[correl(eq_names[i], eq_names[j]) for j in range(n)] for i in range(n)],
+    sub = lambda q: [[correl(eq_names[i], eq_names[j], q) for j in range(n)] for i in range(n)]
+    term = len(pillars) > 1
+    spec = BookSpec(mcl=cfg, equities=equities, contracts=contracts, correlation=sub(0),
                     rate_days=rate_days, rate_values=rate_values, rainbows=rainbows,
-                    greeks=sum(_GREEKS.get(g, 0) for g in greeks), precision=precision)
+                    greeks=sum(_GREEKS.get(g, 0) for g in greeks), precision=precision,
+                    discount_days=disc_days, discount_values=disc_values,
+                    correlation_maturities=pillars if term else [],
+                    correlation_pillars=[sub(q) for q in range(len(pillars))] if term else [])
     return Translation(spec, c_names, str(p.get("result", pricer)), greeks, not_computed)
 
 
