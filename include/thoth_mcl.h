@@ -235,6 +235,27 @@ int thoth_mcl_grid(const thoth_mcl_request* req, int32_t cap, double* days, doub
 int thoth_mcl_calibrate_leverage(const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_layers,
                                  double* t, double* ln_step, int64_t* offset, double* levels);
 
+/* The cluster master's path split (reference: modes/run_cluster.cpp:308-339):
+ * `total` paths cut into min(parts, total) disjoint Sobol blocks, the first
+ * total % n one path longer, block k skipping the paths of the blocks before
+ * it.  Fills paths[k] and skips[k] (room for `parts` entries) and returns the
+ * block count n, or a negative THOTH_* status.  Host only. */
+int thoth_mcl_split_paths(int64_t total, int32_t parts, int64_t* paths, int64_t* skips);
+
+/* Pools the results of `parts` disjoint blocks as the cluster master does
+ * (run_cluster.cpp:402-447): premiums, Greeks and per-contract values as
+ * path-weighted means (weight paths[k] / total), trusts in quadrature
+ * sqrt(sum_k w_k^2 trust_k^2); out's per-contract arrays (n_contracts entries,
+ * any may be NULL) are filled from the parts' arrays.  Host only. */
+int thoth_mcl_pool_results(int32_t parts, const int64_t* paths, const thoth_mcl_result* in, int32_t n_contracts,
+                           thoth_mcl_result* out);
+
+/* thoth_mcl_price of a book split into `parts` disjoint Sobol blocks, each
+ * priced as the cluster's slave would price it (its path count, seed k and
+ * sobol_skip), and pooled by thoth_mcl_pool_results: the single-device form
+ * of the reference's path-split cluster. */
+int thoth_mcl_price_split(const thoth_mcl_request* req, int32_t parts, thoth_mcl_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,7 @@ UNITS = [
     ("mcl_host.cpp", "g++", []),
     ("mcl_cv_host.cpp", "g++", []),
     ("mcl_grid.cpp", "g++", []),
+    ("mcl_split.cpp", "g++", []),
 ]
 
 

@@ -53,6 +53,9 @@ def lib():
                                                    P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_double)]
         L.thoth_mcl_price_pathwise.argtypes = [P(abi.Request), P(abi.PathwiseResult)]
         L.thoth_mcl_price_rqmc.argtypes = [P(abi.Request), C.c_int32, C.c_uint64, P(abi.RqmcResult)]
+        L.thoth_mcl_split_paths.argtypes = [C.c_int64, C.c_int32, P(C.c_int64), P(C.c_int64)]
+        L.thoth_mcl_pool_results.argtypes = [C.c_int32, P(C.c_int64), P(abi.Result), C.c_int32, P(abi.Result)]
+        L.thoth_mcl_price_split.argtypes = [P(abi.Request), C.c_int32, P(abi.Result)]
         L.thoth_grid_prepare.argtypes = [C.c_int32, P(abi.Request), P(abi.VanillaGrid), C.c_int32, P(C.c_void_p)]
         L.thoth_grid_run.argtypes = [C.c_void_p, P(abi.GridResult), P(C.c_double)]
         L.thoth_grid_get_tuning.argtypes = [C.c_void_p, P(abi.GridTuning)]
@@ -114,6 +117,36 @@ def price(built):
     """Price a `book.BuiltRequest`; returns the filled `abi.Result`."""
     res = built.result()
     _check(lib().thoth_mcl_price(C.byref(built.req), C.byref(res)))
+    return res
+
+
+def split_paths(total, parts):
+    """The cluster master's split: (paths, skips) of the disjoint Sobol blocks."""
+    paths = np.zeros(max(1, parts), np.int64)
+    skips = np.zeros(max(1, parts), np.int64)
+    n = lib().thoth_mcl_split_paths(total, parts, paths.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    skips.ctypes.data_as(C.POINTER(C.c_int64)))
+    if n < 0:
+        _check(-n)
+    return paths[:n].copy(), skips[:n].copy()
+
+
+def pool_results(built, paths, results):
+    """Pool per-block `abi.Result`s (each with its per-contract arrays) as the
+    cluster master does: path-weighted means, trusts in quadrature."""
+    n = len(results)
+    arr = (abi.Result * n)(*results)
+    p = np.ascontiguousarray(paths, np.int64)
+    out = built.result()
+    _check(lib().thoth_mcl_pool_results(n, p.ctypes.data_as(C.POINTER(C.c_int64)), arr,
+                                        len(built.spec.contracts), C.byref(out)))
+    return out
+
+
+def price_split(built, parts):
+    """thoth_mcl_price_split: the book priced as `parts` disjoint Sobol blocks, pooled."""
+    res = built.result()
+    _check(lib().thoth_mcl_price_split(C.byref(built.req), parts, C.byref(res)))
     return res
 
 
