@@ -164,6 +164,18 @@ typedef struct thoth_mcl_sweep_cfg {
      * out[b * R + r]. */
     int32_t replicates;
     uint64_t rqmc_seed;
+    /* Pseudo-random draws from one xoshiro256++ stream seeded by `seed`
+     * (splitmix64, rng.hpp:36-47; the reference seeds it once per PriceBook,
+     * pricer_mcl.cpp:227), path p from position p T (T draws per path):
+     * pseudo_random = 0 (use_sobol true): Sobol points with the Brownian
+     * bridge, and the dimensions past the 4097 of the Sobol table from the
+     * stream in dimension order (path_generator.cpp:140-157);
+     * pseudo_random = 1 (use_sobol false): every normal from the stream in
+     * (step, factor) order, no bridge (the reference's noise nodes draw in the
+     * order of its node schedule, noise_node.cpp:20-25: same law, another
+     * assignment of draws to steps).  Not with replicates > 0. */
+    int32_t pseudo_random;
+    uint64_t seed;
 } thoth_mcl_sweep_cfg;
 
 /* Kahan-compensated FP64 sums over paths of the discounted book value and of
@@ -180,6 +192,12 @@ typedef struct thoth_mcl_book_out {
  * out holds n_books entries (n_books * replicates in RQMC mode). */
 int thoth_mcl_sweep(int32_t n_books, const thoth_mcl_book* books,
                     const thoth_mcl_sweep_cfg* cfg, thoth_mcl_book_out* out);
+
+/* No reference counterpart: the xoshiro256++ state after `steps` draws of the
+ * stream seeded by `seed`, by the jump matrices of the device (steps a
+ * multiple of draws_per_path, the T of a book), so that the jump-ahead can be
+ * checked against sequential stepping on the host. */
+int thoth_prng_state_after(uint64_t seed, uint64_t steps, uint64_t draws_per_path, uint64_t* state);
 
 /* No reference counterpart: thoth_mcl_sweep for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses

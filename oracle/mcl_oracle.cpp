@@ -264,11 +264,12 @@ BridgeSchedule BuildBridgeSchedule( const std::vector<double>& times )
     return s;
 }
 
-PathGenerator::PathGenerator( const std::vector<double>& times, int factors, Rng* rng, uint64_t sobol_skip )
-    : _t( times ), _m( (int)times.size() - 1 ), _factors( factors ), _rng( rng )
+PathGenerator::PathGenerator( const std::vector<double>& times, int factors, Rng* rng, uint64_t sobol_skip,
+                              bool use_sobol )
+    : _t( times ), _m( (int)times.size() - 1 ), _factors( factors ), _use_sobol( use_sobol ), _rng( rng )
 {
     _b = BuildBridgeSchedule( _t );
-    if ( _m > 0 )
+    if ( _m > 0 && _use_sobol )
     {
         long total = (long)_m * _factors;
         _sobol_dim = (int)std::min<long>( total, SobolMaxDimension() );
@@ -287,6 +288,15 @@ void PathGenerator::NextPath()
 {
     if ( _m == 0 )
         return;
+    if ( !_use_sobol )
+    {
+        // independent draws, no bridge: the normalized increment of step i is
+        // the draw itself (noise_node.cpp:20-25)
+        for ( int i = 1; i <= _m; i++ )
+            for ( int f = 0; f < _factors; f++ )
+                _noise[f][i] = _rng->Gaussian();
+        return;
+    }
     if ( _sobol )
         _sobol->Next( _u );
     for ( int step = 0; step < _m; step++ )
@@ -1263,7 +1273,7 @@ inline void Kahan( double x, double& sum, double& c, double& sum2, double& c2 )
 }
 }  // namespace
 
-BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, int seed )
+BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, int seed, bool use_sobol )
 {
     const int n = b.n_assets;
     const int nd = (int)b.grid.t.size();
@@ -1274,7 +1284,7 @@ BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, i
     for ( int i = 1; i < nd; i++ )
         sqrt_dt[i] = std::sqrt( dt[i] );  // monte_carlo_node.cpp:121
     Rng rng( (uint64_t)seed );
-    PathGenerator pg( t, b.factors, &rng, sobol_skip );
+    PathGenerator pg( t, b.factors, &rng, sobol_skip, use_sobol );
     // factor of each equity's variance noise: stochastic names after every spot noise
     std::vector<int> var_factor( n, -1 );
     {
@@ -1463,7 +1473,11 @@ int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
     const Market base = BuildMarket( req );
     const Grid g = BuildGrid( req, base );
     const int nc = req.n_contracts;
-    auto run = [&]( const Market& mk ) { return SweepBook( ResolveBook( req, mk, g ), req.mcl.paths, req.mcl.sobol_skip, req.mcl.seed ); };
+    auto run = [&]( const Market& mk )
+    {
+        return SweepBook( ResolveBook( req, mk, g ), req.mcl.paths, req.mcl.sobol_skip, req.mcl.seed,
+                          req.mcl.use_sobol != 0 );
+    };
     BookSums b0 = run( base );
     res->premium = b0.Mean( -1 );
     res->premium_trust = b0.Trust( -1 );

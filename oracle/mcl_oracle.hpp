@@ -75,7 +75,11 @@ BridgeSchedule BuildBridgeSchedule( const std::vector<double>& times );
 class PathGenerator                                  // path_generator.cpp:11-198
 {
   public:
-    PathGenerator( const std::vector<double>& times, int factors, Rng* rng, uint64_t sobol_skip );
+    // use_sobol false: every normal from rng, step-major and factor-minor,
+    // with no bridge (the reference's noise nodes draw in node-schedule
+    // order instead, noise_node.cpp:20-25; same law, other assignment)
+    PathGenerator( const std::vector<double>& times, int factors, Rng* rng, uint64_t sobol_skip,
+                   bool use_sobol = true );
     void NextPath();
     const std::vector<double>& Buffer( int f ) const { return _noise[f]; }
     int SobolDim() const { return _sobol_dim; }
@@ -83,6 +87,7 @@ class PathGenerator                                  // path_generator.cpp:11-19
   private:
     std::vector<double> _t;
     int _m, _factors;
+    bool _use_sobol;
     Rng* _rng;
     Sobol* _sobol = nullptr;
     std::vector<Sobol> _sobol_store;
@@ -203,7 +208,7 @@ struct ResolvedBook
 Grid BuildGrid( const thoth_mcl_request& req, const Market& mk );
 Market BuildMarket( const thoth_mcl_request& req );
 ResolvedBook ResolveBook( const thoth_mcl_request& req, const Market& mk, const Grid& g );
-BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, int seed );
+BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, int seed, bool use_sobol = true );
 int Price( const thoth_mcl_request& req, thoth_mcl_result* res );
 
 }  // namespace oracle

@@ -122,12 +122,15 @@ int oracle_bridge_schedule( int n_times, const double* t, int* bidx, int* lidx, 
     return n_times - 1;
 }
 
-// normalized increments of `paths` paths: out[p][f][i], i = 0..n_times-1 (i=0 unused)
-int oracle_path_noise( int n_times, const double* t, int factors, int64_t skip, int64_t paths, double* out )
+// normalized increments of `paths` paths: out[p][f][i], i = 0..n_times-1 (i=0 unused),
+// the pseudo-random draws (dimensions past the Sobol table, or every draw when
+// use_sobol is 0) from the stream seeded by `seed`
+int oracle_path_noise_seeded( int n_times, const double* t, int factors, int64_t skip, int64_t paths, uint64_t seed,
+                              int use_sobol, double* out )
 {
     std::vector<double> tt( t, t + n_times );
-    Rng rng( 0 );
-    PathGenerator pg( tt, factors, &rng, (uint64_t)skip );
+    Rng rng( seed );
+    PathGenerator pg( tt, factors, &rng, (uint64_t)skip, use_sobol != 0 );
     for ( int64_t p = 0; p < paths; p++ )
     {
         pg.NextPath();
@@ -135,6 +138,11 @@ int oracle_path_noise( int n_times, const double* t, int factors, int64_t skip, 
             std::memcpy( out + ( p * factors + f ) * n_times, pg.Buffer( f ).data(), sizeof( double ) * n_times );
     }
     return 0;
+}
+
+int oracle_path_noise( int n_times, const double* t, int factors, int64_t skip, int64_t paths, double* out )
+{
+    return oracle_path_noise_seeded( n_times, t, factors, skip, paths, 0, 1, out );
 }
 
 static Sabr SabrOf( const thoth_sabr_surface* s )
@@ -268,7 +276,8 @@ int oracle_sweep( const thoth_mcl_request* req, double* book_sum, double* book_s
     {
         Market mk = BuildMarket( *req );
         Grid g = BuildGrid( *req, mk );
-        BookSums s = SweepBook( ResolveBook( *req, mk, g ), req->mcl.paths, req->mcl.sobol_skip, req->mcl.seed );
+        BookSums s = SweepBook( ResolveBook( *req, mk, g ), req->mcl.paths, req->mcl.sobol_skip, req->mcl.seed,
+                                req->mcl.use_sobol != 0 );
         *book_sum = s.book_sum;
         *book_sum2 = s.book_sum2;
         for ( size_t c = 0; c < s.sum.size(); c++ )
@@ -294,7 +303,8 @@ double oracle_sweep_seconds( const thoth_mcl_request* req, double* book_sum )
         Grid g = BuildGrid( *req, mk );
         ResolvedBook rb = ResolveBook( *req, mk, g );
         const auto t0 = std::chrono::steady_clock::now();
-        BookSums s = SweepBook( rb, req->mcl.paths, req->mcl.sobol_skip, req->mcl.seed );
+        BookSums s = SweepBook( rb, req->mcl.paths, req->mcl.sobol_skip, req->mcl.seed,
+                                req->mcl.use_sobol != 0 );
         const double sec = std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 ).count();
         if ( book_sum )
             *book_sum = s.book_sum;

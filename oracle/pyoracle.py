@@ -35,6 +35,9 @@ def lib():
     L.oracle_gaussians.argtypes = [C.c_uint64, C.c_int64, P(C.c_double)]
     L.oracle_bridge_schedule.argtypes = [C.c_int, P(C.c_double)] + [P(C.c_int)] * 3 + [P(C.c_double)] * 3
     L.oracle_path_noise.argtypes = [C.c_int, P(C.c_double), C.c_int, C.c_int64, C.c_int64, P(C.c_double)]
+    L.oracle_path_noise_seeded.argtypes = [C.c_int, P(C.c_double), C.c_int, C.c_int64, C.c_int64, C.c_uint64,
+                                           C.c_int, P(C.c_double)]
+    L.oracle_xoshiro_raw.argtypes = [P(C.c_uint64), C.c_int64, P(C.c_uint64)]
     L.oracle_sabr_vol.argtypes = [P(abi.SabrSurface), C.c_double, C.c_double, C.c_double]
     L.oracle_sabr_vol.restype = C.c_double
     L.oracle_sabr_hagan.argtypes = [P(abi.SabrSurface), C.c_double, C.c_double, C.c_double]
@@ -103,10 +106,17 @@ def bridge_schedule(t):
                     ints + dbls))
 
 
-def path_noise(t, factors, paths, skip=0):
+def path_noise(t, factors, paths, skip=0, seed=0, use_sobol=True):
     t = np.ascontiguousarray(t, np.float64)
     out = np.zeros((paths, factors, t.size), np.float64)
-    lib().oracle_path_noise(t.size, _p(t), factors, skip, paths, _p(out))
+    lib().oracle_path_noise_seeded(t.size, _p(t), factors, skip, paths, seed % (1 << 64), int(use_sobol), _p(out))
+    return out
+
+
+def xoshiro_raw(state, n):
+    st = np.ascontiguousarray(state, np.uint64)
+    out = np.zeros(n, np.uint64)
+    lib().oracle_xoshiro_raw(_p(st, C.c_uint64), n, _p(out, C.c_uint64))
     return out
 
 

@@ -132,9 +132,6 @@ def test_limits_and_invalid_requests_are_errors(gpu):
     assert _error(gpu, bs(paths=200_000_001)) == abi.ERR_INVALID
     assert _error(gpu, _with(bs(), 1000, (1 << 32) - 100)) == abi.ERR_UNSUPPORTED  # Sobol index past 2^32 - 2
     s = bs()
-    s.mcl.use_sobol = False
-    assert _error(gpu, s) == abi.ERR_UNSUPPORTED
-    s = bs()
     s.contracts = [Vanilla(0, 365, 100.0 + k) for k in range(33)]
     assert _error(gpu, s) == abi.ERR_UNSUPPORTED
     s = bs()

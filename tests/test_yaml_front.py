@@ -27,6 +27,7 @@ SUPPORTED = {
     "barrier_mono_continuous_down_in_call_mcl", "barrier_mono_continuous_up_out_put_mcl",
     "barrier_mono_discrete_in_call_mcl", "vanilla_mono_eu_put_mcl", "vanilla_mono_eu_call_mcl_gpu",
     "vanilla_mono_eu_put_mcl_gpu", "vanilla_mono_eu_call_lsv_mcl", "vanilla_mono_eu_call_ois_mcl",
+    "vanilla_mono_eu_call_mcl_pseudo",
 }
 
 REFUSED = {  # pricer -> a word of the reason
@@ -34,7 +35,7 @@ REFUSED = {  # pricer -> a word of the reason
     "vanilla_mono_eu_call_quanto_mcl": "currency", "vanilla_mono_eu_call_compo_mcl": "composite",
     "vanilla_multi_eu_call_basket_mcl": "basket", "vanilla_mono_eu_call_bates_mcl": "Bates",
     "variance_mono_mcl": "variance", "vanilla_mono_eu_call_sabr_mcl": "sabr_volatility",
-    "vanilla_mono_eu_call_mcl_pseudo": "use_sobol", "basket_term_correl_mcl": "basket",
+    "basket_term_correl_mcl": "basket",
     "vanilla_mono_eu_call_hw_mcl": "rate_model",
     "autocallable_athena_mcl": "autocallable", "asian_mono_eu_call_mcl": "asian",
     "ratchet_mono_mcl": "ratchet", "digital_mono_cash_call_mcl": "digital",

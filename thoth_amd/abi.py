@@ -99,7 +99,7 @@ class Book(C.Structure):
 
 class SweepCfg(C.Structure):
     _fields_ = [("paths", _i64), ("sobol_skip", _i64), ("precision", _i32), ("block", _i32),
-                ("replicates", _i32), ("rqmc_seed", C.c_uint64)]
+                ("replicates", _i32), ("rqmc_seed", C.c_uint64), ("pseudo_random", _i32), ("seed", C.c_uint64)]
 
 
 class BookOut(C.Structure):

@@ -44,6 +44,9 @@ UNITS = [
     ("mcl_pathwise.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_pathwise_f32.hip", "hipcc", []),
     ("mcl_noalloc.hip", "hipcc", []),
+    ("mcl_prng_f64.hip", "hipcc", ["-ffp-contract=off"]),
+    ("mcl_prng_f32.hip", "hipcc", []),
+    ("mcl_prng.cpp", "g++", []),
     ("mcl_gpu.cpp", "g++", []),
     ("mcl_books.cpp", "g++", []),
     ("mcl_market.cpp", "g++", []),

@@ -106,10 +106,6 @@ int PackBook( int bi, const thoth_mcl_book& in, DevBook& db, Offsets& of, Arena&
         any_lsv |= A.model == THOTH_MODEL_LSV;
     }
     db.factors = f;
-    if ( (long)( nd - 1 ) * f > kSobolDims )
-        return SetError( THOTH_ERR_UNSUPPORTED,
-                         Why( bi, "steps x factors exceeds the 4097 Sobol dimensions (the reference draws the "
-                                  "tail pseudo-randomly)" ) );
     if ( na > 1 && !in.chol && !in.chol_dates )
         return SetError( THOTH_ERR_INVALID, Why( bi, "multi-equity book needs a Cholesky factor" ) );
     of.chol = of.cholf = 0;
@@ -511,6 +507,33 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
         B.kind = std::max( kind, 0 );
     }
+    // pseudo-random draws: every normal (use_sobol false), or the dimensions
+    // of a book past the Sobol table (path_generator.cpp:140-157).  One stream
+    // per book, seeded alike: books with the same draws per path T share the
+    // jump tables and the tail normals.
+    B.pseudo = cfg->pseudo_random != 0;
+    B.prng = B.pseudo;
+    std::vector<int64_t> draws( n_books, 0 );
+    for ( int b = 0; b < n_books; b++ )
+    {
+        const int64_t dims = (int64_t)B.db[b].m * B.db[b].factors;
+        draws[b] = B.pseudo ? dims : std::max<int64_t>( 0, dims - kSobolDims );
+        B.prng |= draws[b] > 0;
+    }
+    if ( B.prng )
+    {
+        const char* why = rqmc ? "randomised QMC" : cv ? "the control variate" : pathwise ? "pathwise Greeks" : nullptr;
+        for ( int b = 0; b < n_books && !why; b++ )
+            if ( grids && grids[b].nm > 0 )
+                why = "vanilla grids";
+        if ( why )
+            return SetError( THOTH_ERR_UNSUPPORTED,
+                             std::string( why ) + " needs every normal from the Sobol table (use_sobol true, "
+                                                  "steps x factors <= 4097)" );
+        B.kind = 0;
+        if ( B.pseudo )
+            B.depth = 1;  // no bridge
+    }
     // dynamic LDS: the sweep's Sobol ring (the pathwise kernel: the Sobol block
     // table of the widest book), plus (FP32 LSV) a ring of 2G leverage layers
     // per equity of the kernel instantiation (G = 4, halved while two blocks
@@ -521,7 +544,8 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     bool any_grid = false;
     for ( int b = 0; b < n_books; b++ )
     {
-        dims = std::max( dims, (size_t)B.db[b].m * B.db[b].factors );
+        if ( !B.pseudo )
+            dims = std::max( dims, std::min( (size_t)B.db[b].m * B.db[b].factors, (size_t)kSobolDims ) );
         if ( cfg->precision == 1 && B.db[b].lev_points > 0 )
             levw = std::max( levw, (size_t)2 * na_inst * B.db[b].lev_points );
         any_grid |= B.db[b].grid_nm > 0;
@@ -602,10 +626,49 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         gtab.push_back( (long long)B.grid_cells[b] * 2 );
     }
     const size_t gtab_off = ar.put( gtab );
+    // jump tables per distinct T (arena) and, in Sobol mode, tail normals
+    // [T][paths] per distinct T (a region of their own after the sums)
+    B.fills.clear();
+    std::vector<size_t> jump_off( n_books, 0 ), tail_off( n_books, 0 );
+    size_t tail_total = 0;
+    const size_t elem = cfg->precision == 0 ? sizeof( double ) : sizeof( float );
+    if ( B.prng )
+    {
+        B.jump_levels = JumpLevels( cfg->paths );
+        SplitMixSeed( cfg->seed, B.seed_state );
+        for ( int b = 0; b < n_books; b++ )
+        {
+            int same = -1;
+            for ( int c = 0; c < b && same < 0; c++ )
+                if ( draws[c] == draws[b] )
+                    same = c;
+            if ( same >= 0 )
+            {
+                jump_off[b] = jump_off[same];
+                tail_off[b] = tail_off[same];
+                continue;
+            }
+            if ( draws[b] == 0 )
+                continue;
+            const std::vector<uint64_t>& tab = JumpTables( (uint64_t)draws[b], B.jump_levels );
+            jump_off[b] = ar.put( tab.data(), (size_t)B.jump_levels * kJumpWords );
+            if ( !B.pseudo )
+            {
+                tail_off[b] = tail_total;
+                tail_total += ( (size_t)draws[b] * (size_t)cfg->paths * elem + 255 ) & ~size_t( 255 );
+                B.fills.push_back( { draws[b], b, nullptr, nullptr } );
+            }
+        }
+        if ( tail_total > ( (size_t)128 << 30 ) )
+            return SetError( THOTH_ERR_UNSUPPORTED, "the normals past the Sobol table would take " +
+                                                        std::to_string( tail_total >> 30 ) +
+                                                        " GiB (fewer paths or steps)" );
+    }
     const size_t arena_bytes = ( ar.data.size() + 255 ) & ~size_t( 255 );
     const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
-    const size_t total =
-        arena_bytes + books_bytes + ( B.part_total + B.grid_total + B.grid_out_total ) * sizeof( double );
+    const size_t sums_bytes =
+        ( ( B.part_total + B.grid_total + B.grid_out_total ) * sizeof( double ) + 255 ) & ~size_t( 255 );
+    const size_t total = arena_bytes + books_bytes + sums_bytes + tail_total;
     if ( ext_base )
     {
         if ( total > ext_cap )
@@ -665,6 +728,24 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.cv_scale = At<float>( dbase, o.cv_scale );
         }
         d.partial = B.dpart + B.part_off[b];
+        if ( B.prng )
+        {
+            d.pseudo = B.pseudo ? 1 : 0;
+            d.jump_levels = B.jump_levels;
+            d.jump = draws[b] > 0 ? At<uint64_t>( dbase, jump_off[b] ) : nullptr;
+            d.tail_dims = B.pseudo ? 0 : (int)draws[b];
+            d.tail_stride = cfg->paths;
+            char* tail = dbase + arena_bytes + books_bytes + sums_bytes + tail_off[b];
+            if ( d.tail_dims > 0 )
+            {
+                if ( cfg->precision == 0 )
+                    d.tail = reinterpret_cast<const double*>( tail );
+                else
+                    d.tailf = reinterpret_cast<const float*>( tail );
+            }
+            for ( int q = 0; q < 4; q++ )
+                d.seed_state[q] = B.seed_state[q];
+        }
         if ( d.grid_nm > 0 )
         {
             d.grid_mat_at = At<int>( dbase, o.grid_mat_at );
@@ -672,6 +753,11 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.grid_kd = At<double>( dbase, o.grid_kd );
             d.grid_part = B.dgrid + B.grid_off[b];
         }
+    }
+    for ( DeviceBatch::TailFill& fl : B.fills )
+    {
+        fl.jump = B.db[fl.book].jump;
+        fl.dst = dbase + arena_bytes + books_bytes + sums_bytes + tail_off[fl.book];
     }
     B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
     B.dgrid_table = At<long long>( dbase, gtab_off );
@@ -698,7 +784,22 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
     const dim3 grid( B.nblocks, B.n_books, B.replicates );
     // kernel control-variate mode: in-loop inference of a gated network is mode 3
     const int kcv = B.cv_mode == 2 && B.cv_act == THOTH_CV_GATE ? 3 : B.cv_mode;
-    bool ok = B.precision == 0
+    // the normals past the Sobol table, once per distinct T, inside the timed region
+    for ( const DeviceBatch::TailFill& fl : B.fills )
+    {
+        const bool filled = B.precision == 0
+                                ? launch_tail_fill_f64( s, fl.jump, B.jump_levels, B.seed_state, (int)fl.T, B.paths,
+                                                        reinterpret_cast<double*>( fl.dst ) )
+                                : launch_tail_fill_f32( s, fl.jump, B.jump_levels, B.seed_state, (int)fl.T, B.paths,
+                                                        reinterpret_cast<float*>( fl.dst ) );
+        if ( !filled )
+            return SetError( THOTH_ERR_DEVICE, "tail normal fill launch failed" );
+    }
+    bool ok = B.prng ? ( B.precision == 0 ? launch_sweep_prng_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                                   DeviceDirections(), B.q0, B.first, B.last )
+                                          : launch_sweep_prng_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                                   DeviceDirections(), B.q0, B.first, B.last ) )
+              : B.precision == 0
                   ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
                                       B.first, B.last, B.dshifts )
                   : ( B.kind && launch_sweep_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,

@@ -21,7 +21,7 @@
 namespace thoth
 {
 
-static_assert( THOTH_CV_SILU == kActSilu && THOTH_CV_GATE == kActGate, "activation codes of the C-ABI" );
+static_assert( (int)THOTH_CV_SILU == (int)kActSilu && (int)THOTH_CV_GATE == (int)kActGate, "activation codes of the C-ABI" );
 
 namespace
 {

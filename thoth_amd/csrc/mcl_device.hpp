@@ -18,6 +18,8 @@ constexpr int kSobolDimsDev = 4097;       // implicit first dimension + 4096 Joe
 // the device direction table [kSobolDimsDev][32] is followed by per-dimension
 // lane nibble tables [kSobolDimsDev][32] (see the Sobol ring of the sweep)
 constexpr size_t kSobolNibbles = (size_t)kSobolDimsDev * 32;
+// one GF(2) jump matrix of the xoshiro256++ state: 256 columns of four words
+constexpr int kJumpWords = 256 * 4;
 
 // LDS of the in-loop hedge inference: BF16 A-operand image of the network
 // (40 tiles x 64 lanes x 8 bytes), FP32 biases (208), and per wave a
@@ -137,6 +139,19 @@ struct DevBook
     const float* grid_kf;       // [nk] absolute strikes
     const double* grid_kd;
     double* grid_part;          // [R][tiles][nm][2 nk][2] undiscounted payoff sums
+    // pseudo-random draws (mcl_prng.hpp): pseudo = 1 draws every normal of the
+    // path from the lane's xoshiro256++ stream in (step, factor) order, with no
+    // Sobol point and no bridge; otherwise the normals of the dimensions past
+    // the Sobol budget come precomputed from `tail` / `tailf`,
+    // [tail_dims][tail_stride] with the path (index - first) minor
+    int pseudo;
+    int tail_dims;
+    int jump_levels;            // tabulated jump matrices (paths <= 2^jump_levels)
+    int64_t tail_stride;
+    const double* tail;
+    const float* tailf;
+    const uint64_t* jump;       // [jump_levels][256][4]: (M^T)^(2^k), T the draws per path
+    uint64_t seed_state[4];     // splitmix64 expansion of the seed (rng.hpp:36-47)
 };
 
 }  // namespace thoth

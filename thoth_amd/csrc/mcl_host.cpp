@@ -344,9 +344,6 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     // mcl_configuration.cpp:30-46 guards
     if ( req.mcl.paths <= 1 || req.mcl.paths > 200000000 )
         throw Invalid( THOTH_ERR_INVALID, "paths must be in (1, 2e8]" );
-    if ( !req.mcl.use_sobol )
-        throw Invalid( THOTH_ERR_UNSUPPORTED,
-                       "the device engine runs the Sobol + Brownian-bridge mode only (use_sobol: true)" );
     if ( req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS || !req.assets )
         throw Invalid( THOTH_ERR_INVALID, "1..8 equities" );
     if ( req.n_contracts < 1 || req.n_contracts > THOTH_MCL_MAX_CONTRACTS || !req.contracts )
@@ -416,6 +413,10 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     cfg.paths = req.mcl.paths;
     cfg.sobol_skip = req.mcl.sobol_skip;
     cfg.precision = req.precision;
+    // the pseudo-random stream: seeded by the configuration's int seed as
+    // Rng::Seed((uint64_t)seed) (pricer_mcl.cpp:227)
+    cfg.pseudo_random = req.mcl.use_sobol ? 0 : 1;
+    cfg.seed = (uint64_t)(int64_t)req.mcl.seed;
     int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, plan.batch );
     if ( rc != THOTH_OK )
         return rc;
@@ -505,6 +506,9 @@ int RunPlan( const Plan& plan, thoth_mcl_result* res )
 // base scenario only (no Greek bumps): grid, Cholesky, calibration, resolution
 int ResolveBaseBook( const thoth_mcl_request& req, BookArrays& A, Grid& g )
 {
+    // its callers (randomised QMC, the control variate) shift Sobol points
+    if ( !req.mcl.use_sobol )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "randomised QMC and the control variate need use_sobol: true" );
     if ( req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS || !req.assets )
         throw Invalid( THOTH_ERR_INVALID, "1..8 equities" );
     if ( req.n_contracts < 1 || req.n_contracts > THOTH_MCL_MAX_CONTRACTS || !req.contracts )

@@ -36,6 +36,12 @@ const uint32_t* DeviceDirections();
 // address of its static scratch array on the current device
 int NoAllocSetup();
 int SetError( int code, const std::string& msg );
+// pseudo-random stream (mcl_prng.cpp): the splitmix64 expansion of a seed,
+// the jump matrices (M^T)^(2^k), k < levels (cached by T; callers hold the
+// library lock), and the levels that cover path indices 0 .. paths - 1
+void SplitMixSeed( uint64_t seed, uint64_t st[4] );
+const std::vector<uint64_t>& JumpTables( uint64_t T, int levels );
+int JumpLevels( int64_t paths );
 double StudentT975( int df );  // two-sided 95 % quantile of Student's t
 struct DevBook;
 // A packed, uploaded batch of books ready to sweep (device-resident inputs).
@@ -73,6 +79,20 @@ struct DeviceBatch
     std::vector<std::vector<int>> step_ops;
     int nf_inst = 2;
     int64_t paths = 0;
+    // pseudo-random draws: prng when any book draws from the stream (kind 4
+    // of the sweep), pseudo for use_sobol false; one fill of tail normals per
+    // distinct T in Sobol mode (jump: device tables, dst: [T][paths])
+    bool prng = false, pseudo = false;
+    int jump_levels = 0;
+    uint64_t seed_state[4] = { 0, 0, 0, 0 };
+    struct TailFill
+    {
+        int64_t T;
+        int book;
+        const uint64_t* jump;
+        char* dst;
+    };
+    std::vector<TailFill> fills;
     DeviceBatch() = default;
     DeviceBatch( const DeviceBatch& ) = delete;
     DeviceBatch& operator=( const DeviceBatch& ) = delete;

@@ -85,6 +85,17 @@ bool launch_persistent_f32( int na, int depth, dim3 grid, hipStream_t s, size_t 
                            const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
                            int n_items, uint32_t ntiles, int* counter );
 void launch_counter_reset( hipStream_t s, int* counter );
+// books whose normals are not all Sobol points (pseudo-random tail dimensions,
+// or use_sobol false): sweep_block kind 4, and the fill of the tail normals
+// [T][paths] from the jump tables (levels: paths <= 2^levels)
+bool launch_sweep_prng_f64( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                            const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
+bool launch_sweep_prng_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                            const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
+bool launch_tail_fill_f64( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
+                           int64_t paths, double* out );
+bool launch_tail_fill_f32( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
+                           int64_t paths, float* out );
 bool launch_pathwise( int precision, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 bool launch_pathwise_f32( int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,

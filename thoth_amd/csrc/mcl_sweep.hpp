@@ -16,6 +16,7 @@
 
 #include "mcl_device.hpp"
 #include "mcl_mlp.hpp"
+#include "mcl_prng.hpp"
 
 namespace thoth
 {
@@ -141,33 +142,7 @@ __device__ __forceinline__ uint32_t sobol_x( const uint32_t* __restrict__ V, int
 __device__ __forceinline__ double normal_f64( uint32_t x )
 {
     const double u0 = x * ( 1.0 / 4294967296.0 );
-    const double P = fmin( 1.0 - 1e-12, fmax( 1e-12, u0 ) );
-    const double a0 = -3.969683028665376e+01, a1 = 2.209460984245205e+02, a2 = -2.759285104469687e+02,
-                 a3 = 1.383577518672690e+02, a4 = -3.066479806614716e+01, a5 = 2.506628277459239e+00;
-    const double b0 = -5.447609879822406e+01, b1 = 1.615858368580409e+02, b2 = -1.556989798598866e+02,
-                 b3 = 6.680131188771972e+01, b4 = -1.328068155288572e+01;
-    const double c0 = -7.784894002430293e-03, c1 = -3.223964580411365e-01, c2 = -2.400758277161838e+00,
-                 c3 = -2.549732539343734e+00, c4 = 4.374664141464968e+00, c5 = 2.938163982698783e+00;
-    const double d0 = 7.784695709041462e-03, d1 = 3.224671290700398e-01, d2 = 2.445134137142996e+00,
-                 d3 = 3.754408661907416e+00;
-    const double p_low = 0.02425;
-    const double p_high = 1.0 - p_low;
-    if ( P < p_low )
-    {
-        double q = sqrt( -2.0 * log( P ) );
-        return ( ( ( ( ( c0 * q + c1 ) * q + c2 ) * q + c3 ) * q + c4 ) * q + c5 ) /
-               ( ( ( ( d0 * q + d1 ) * q + d2 ) * q + d3 ) * q + 1.0 );
-    }
-    if ( P <= p_high )
-    {
-        double q = P - 0.5;
-        double r = q * q;
-        return ( ( ( ( ( a0 * r + a1 ) * r + a2 ) * r + a3 ) * r + a4 ) * r + a5 ) * q /
-               ( ( ( ( ( b0 * r + b1 ) * r + b2 ) * r + b3 ) * r + b4 ) * r + 1.0 );
-    }
-    double q = sqrt( -2.0 * log( 1.0 - P ) );
-    return -( ( ( ( ( c0 * q + c1 ) * q + c2 ) * q + c3 ) * q + c4 ) * q + c5 ) /
-           ( ( ( ( d0 * q + d1 ) * q + d2 ) * q + d3 ) * q + 1.0 );
+    return acklam_f64( fmin( 1.0 - 1e-12, fmax( 1e-12, u0 ) ) );
 }
 
 // FP32 inverse normal from the 32-bit integer coordinate, in the erfinv form
@@ -255,6 +230,17 @@ __device__ __forceinline__ R normal_of( uint32_t x )
         return normal_f64( x );
     else
         return normal_f32( x );
+}
+
+// one pseudo-random normal: Rng::Gaussian in FP64; the FP32 sweeps read the
+// draw's 32 high bits as a grid coordinate, as they read a Sobol point
+template <class R>
+__device__ __forceinline__ R xoshiro_normal( Xoshiro256& x )
+{
+    if constexpr ( std::is_same_v<R, double> )
+        return xoshiro_gauss_f64( x );
+    else
+        return normal_f32( (uint32_t)( x.next() >> 32 ) );
 }
 
 // Andersen QE variance step, FP64, in the reference's operation order
@@ -498,7 +484,11 @@ static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 
 // vol-of-vol > 1e-12 and uncorrelated stochastic spots, and the book has
 // exactly NA equities -- the host checks this -- so the per-equity model
 // branches fold away and the equities' steps interleave in one basic block;
-// 3 as 1 with the spot noises Cholesky-correlated across the equities.
+// 3 as 1 with the spot noises Cholesky-correlated across the equities;
+// 4 any book (the arithmetic of 0) whose normals are not all Sobol points:
+// the dimensions past the Sobol budget read precomputed pseudo-random
+// normals, and pseudo books (use_sobol false) draw every normal from the
+// lane's xoshiro256++ stream.
 template <class R, int NA, int DEPTH, int CV, int KIND = 0>
 __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* __restrict__ V, uint32_t bq,
                                              uint32_t first_idx, uint32_t last_idx,
@@ -506,12 +496,14 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                                              uint32_t ntiles )
 {
     constexpr bool F64 = std::is_same_v<R, double>;
-    constexpr bool uniform = KIND != 0;
+    constexpr bool prng = KIND == 4;  // pseudo-random draws
+    constexpr bool uniform = KIND != 0 && !prng;
     constexpr bool lsv_kind = KIND == 1 || KIND == 3;  // every equity LSV
     constexpr bool corr_kind = KIND == 3;              // correlated spot noises
     constexpr bool qe_phase = uniform;
     const int nas = uniform ? NA : B.n_assets;  // row length of the date-major per-step arrays
-    static_assert( KIND == 0 || !F64, "uniform-book kernels are FP32" );
+    static_assert( !uniform || !F64, "uniform-book kernels are FP32" );
+    static_assert( !prng || CV == 0, "pseudo-random draws run without the control variate" );
     constexpr int NF = 2 * NA;
     constexpr int NFV = NF;
     constexpr int NAV = NA;
@@ -524,6 +516,20 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     const uint32_t idx = bq * 256u + (uint32_t)threadIdx.x;
     const bool valid = idx >= first_idx && idx <= last_idx;
     const uint32_t* sh = shifts ? shifts + (size_t)z * kSobolDimsDev : nullptr;
+
+    // pseudo-random draws: the lane's path (index - first) is its position in
+    // the book's one stream of draws; invalid lanes read path 0
+    const bool pseudo = prng && B.pseudo != 0;
+    [[maybe_unused]] const size_t prow = valid ? (size_t)( idx - first_idx ) : 0;
+    [[maybe_unused]] Xoshiro256 prs;
+    if constexpr ( prng )
+        if ( pseudo )
+        {
+#pragma unroll
+            for ( int q = 0; q < 4; q++ )
+                prs.s[q] = B.seed_state[q];
+            xoshiro_jump( prs, B.jump, B.jump_levels, prow );
+        }
 
     const int F = B.factors;
     const int m = B.m;
@@ -538,7 +544,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     constexpr bool block_table = CV < 2;
     extern __shared__ uint32_t lds[];
     const int nslots = B.sob_slots;
-    const int tab_words = block_table ? ( m * F + 3 ) & ~3 : 0;
+    const int tab_words = block_table && !pseudo ? ( min( m * F, kSobolDimsDev ) + 3 ) & ~3 : 0;
     const int ring_words = tab_words + nslots * NF * 32;  // through the end of the ring
     uint32_t* tab = lds;
     uint32_t* ring = lds + tab_words;
@@ -597,6 +603,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         // four adjacent lanes per (op, factor), lane q of the four owning
         // words 8q .. 8q+7 of the direction numbers and of the nibble tables
         // (n and kBlock are multiples of four: the four lanes run together)
+        if ( pseudo )
+            return;
         const int n = ( o1 - o0 ) * NF * 4;
         for ( int k = threadIdx.x; k < n; k += kBlock )
         {
@@ -607,6 +615,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             if ( f >= F )
                 continue;
             const int d = B.op_b[o] * F + f;
+            if constexpr ( prng )
+                if ( d >= kSobolDimsDev )
+                    continue;  // past the Sobol budget: read from the tail normals
             uint32_t blk;
             if constexpr ( block_table )
                 blk = tab[d];
@@ -710,10 +721,11 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
     };
     if constexpr ( block_table )
-    {
-        sobol_block_table( tab, V, m * F, bq, sh );
-        __syncthreads();
-    }
+        if ( !pseudo )
+        {
+            sobol_block_table( tab, V, min( m * F, kSobolDimsDev ), bq, sh );
+            __syncthreads();
+        }
     // the ops of bridge_step(0) and of the first group's steps
     stage_ops( 0, so( G + 1 ) );
     if ( lev_lds && m >= 1 )
@@ -761,7 +773,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     {
         prev[f] = R( 0 );
         top[f] = R( 0 );
-        if ( f < F )
+        if ( f < F && !pseudo )
         {
             const R z = normal_of<R>( sob_point( f ) );
             if constexpr ( F64 )
@@ -1065,14 +1077,42 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         }
         else
         {
+            [[maybe_unused]] const int d0 = B.op_b[o] * F;
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
                 if ( f < F )
+                {
+                    if constexpr ( prng )
+                        if ( d0 + f >= kSobolDimsDev )
+                        {
+                            // dimension past the Sobol budget (path_generator.cpp:150-154)
+                            const size_t at = (size_t)( d0 + f - kSobolDimsDev ) * B.tail_stride + prow;
+                            if constexpr ( F64 )
+                                zf[f] = B.tail[at];
+                            else
+                                zf[f] = B.tailf[at];
+                            continue;
+                        }
                     zf[f] = normal_of<R>( sob_op( o, f ) );
+                }
         }
     };
     auto bridge_step = [&]( int p, facv& nz ) __attribute__( ( always_inline ) )
     {
+        if constexpr ( prng )
+            if ( pseudo )
+            {
+                // no bridge: the step's normals in factor order, straight
+                // from the stream (noise_node.cpp:20-25)
+#pragma unroll
+                for ( int f = 0; f < NFV; f++ )
+                    nz[f] = R( 0 );
+#pragma unroll
+                for ( int f = 0; f < NF; f++ )
+                    if ( f < F )
+                        nz[f] = xoshiro_normal<R>( prs );
+                return;
+            }
         const int i = p + 1;
         const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
         if ( o1 - o0 == 1 && sp >= 1 )

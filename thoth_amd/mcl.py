@@ -43,6 +43,7 @@ def lib():
         L.thoth_gpu_special_f32.argtypes = [C.c_int32, C.c_int64, P(C.c_uint32), P(C.c_float)]
         L.thoth_mcl_sweep.argtypes = [C.c_int32, P(abi.Book), P(abi.SweepCfg), P(abi.BookOut)]
         L.thoth_mcl_sweep_noalloc.argtypes = [C.c_int32, P(abi.Book), P(abi.SweepCfg), P(abi.BookOut)]
+        L.thoth_prng_state_after.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint64)]
         L.thoth_mcl_price.argtypes = [P(abi.Request), P(abi.Result)]
         L.thoth_mcl_prepare.argtypes = [P(abi.Request), P(C.c_void_p)]
         L.thoth_mcl_run.argtypes = [C.c_void_p, P(abi.Result)]
@@ -295,18 +296,32 @@ class BookArrays:
 NOALLOC_SCRATCH_BYTES = 64 << 20  # THOTH_NOALLOC_SCRATCH_BYTES
 
 
-def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_seed=0, noalloc=False):
+def sweep(books, paths, precision=abi.FP64, sobol_skip=0, replicates=0, rqmc_seed=0, noalloc=False,
+          pseudo_random=False, seed=0):
     """Run thoth_mcl_sweep (or, noalloc=True, thoth_mcl_sweep_noalloc, which
     allocates no device memory) over a list of BookArrays; returns the BookOut
-    list (len(books) * replicates entries in RQMC mode, book-major)."""
+    list (len(books) * replicates entries in RQMC mode, book-major).
+    pseudo_random=True is use_sobol false; `seed` seeds the xoshiro256++
+    stream of those draws and of the dimensions past the Sobol table (an int
+    seed is taken modulo 2^64, as the reference's (uint64_t) cast does)."""
     n = len(books)
     arr = (abi.Book * n)(*[b.book for b in books])
     cfg = abi.SweepCfg(paths=paths, sobol_skip=sobol_skip, precision=precision, block=0,
-                       replicates=replicates, rqmc_seed=rqmc_seed)
+                       replicates=replicates, rqmc_seed=rqmc_seed, pseudo_random=int(bool(pseudo_random)),
+                       seed=int(seed) % (1 << 64))
     out = (abi.BookOut * (n * max(1, replicates)))()
     fn = lib().thoth_mcl_sweep_noalloc if noalloc else lib().thoth_mcl_sweep
     _check(fn(n, arr, C.byref(cfg), out))
     return list(out)
+
+
+def prng_state_after(seed, steps, draws_per_path):
+    """xoshiro256++ state after `steps` draws of the stream seeded by `seed`,
+    by the device's jump matrices (steps a multiple of draws_per_path)."""
+    out = (C.c_uint64 * 4)()
+    _check(lib().thoth_prng_state_after(C.c_uint64(int(seed) % (1 << 64)), C.c_uint64(steps),
+                                        C.c_uint64(draws_per_path), out))
+    return [int(x) for x in out]
 
 
 # ---- learned martingale control variate (include/thoth_mcl_cv.h) ----------

@@ -7,7 +7,8 @@ core/object_registry.cpp:89-126) and writes one result block per pricer
 reads the subset of that graph the device engine prices:
 
 * `!mcl_pricer` (tasks/pricer.cpp:27-41, tasks/pricer_mcl.cpp:67) with
-  `!mcl_configuration` (tasks/mcl_configuration.cpp:15-30), `use_sobol: true`;
+  `!mcl_configuration` (tasks/mcl_configuration.cpp:15-30), Sobol or
+  pseudo-random (`use_sobol: false`);
 * `!book` of `!vanilla` (european, contracts/vanilla.cpp:18-27) and `!barrier`
   (contracts/barrier.cpp:18-33) contracts paid in the pricer's currency;
 * underlyings: `!equity` (marketdata/equity.cpp:20-39) with `!bs_volatility`,
@@ -128,11 +129,11 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
         disc_values = [float(v) for v in dc["values"]]
 
     mc = _obj(objs, p["mcl_configuration"], "mcl_configuration")
-    if not mc.get("use_sobol", False):
-        raise Unsupported("pseudo-random paths (use_sobol: false)")
+    # use_sobol defaults to true (tasks/mcl_configuration.hpp:5, mcl_configuration.cpp:23)
     cfg = McConfig(paths=int(paths or mc["paths"]), max_day_step=int(mc["max_day_step"]),
                    min_day_step=int(mc.get("min_day_step", 1)), vol_year_step=float(mc.get("vol_year_step", 0.0)),
-                   use_sobol=True, seed=int(mc.get("seed", 0)), sobol_skip=int(mc.get("sobol_skip", 0)),
+                   use_sobol=bool(mc.get("use_sobol", True)), seed=int(mc.get("seed", 0)),
+                   sobol_skip=int(mc.get("sobol_skip", 0)),
                    allow_gpu=bool(mc.get("allow_gpu", False)))
 
     corr = _obj(objs, p["correlation"], "correlation_matrix") if "correlation" in p else None
