@@ -58,11 +58,22 @@ def basket(paths, precision=abi.FP32, correlated=False, greeks=0):
                                                     correlate_stochastic=correlated, greeks=greeks))
 
 
-def timed_plan(built, steps, warmup):
-    """Sweeps of a prepared plan: (setup s, host-clock s per sweep list, device ms list, last result)."""
+def shape_of(sh, tune_s):
+    return {"lev_group": sh.lev_group, "workgroups_per_cu": sh.workgroups_per_cu, "tuned_ms": sh.device_ms,
+            "default_shape_ms": sh.default_ms, "candidates": sh.candidates, "tune_s": tune_s}
+
+
+def timed_plan(built, steps, warmup, tune=True):
+    """Sweeps of a prepared plan, its launch shape picked by the tuner first:
+    (setup s, host-clock s per sweep list, device ms list, last result, shape)."""
     t0 = time.perf_counter()
     plan = mcl.Plan(built)
     setup = time.perf_counter() - t0
+    shape = None
+    if tune:
+        t0 = time.perf_counter()
+        sh = plan.tune()
+        shape = shape_of(sh, time.perf_counter() - t0)
     for _ in range(warmup):
         plan.run()
     wall, dev = [], []
@@ -72,7 +83,7 @@ def timed_plan(built, steps, warmup):
         res = plan.run()
         wall.append(time.perf_counter() - t)
         dev.append(res.sweep_ms)
-    return setup, wall, dev, res
+    return setup, wall, dev, res, shape
 
 
 def roofline(flops, kernel_ms, peak, kernel):
@@ -84,7 +95,7 @@ def roofline(flops, kernel_ms, peak, kernel):
 def basket_leg(paths, steps, warmup, precision=abi.FP32, correlated=False):
     """The basket sweep (uniform LSV kernel; correlated: its Cholesky form; FP64: the oracle-matched kernel)."""
     b = basket(paths, precision, correlated)
-    setup, wall, dev, res = timed_plan(b, steps, warmup)
+    setup, wall, dev, res, shape = timed_plan(b, steps, warmup)
     _, t, _ = mcl.grid(b)
     m = len(t) - 1
     na = len(b.spec.equities)
@@ -97,6 +108,7 @@ def basket_leg(paths, steps, warmup, precision=abi.FP32, correlated=False):
             "precision": "fp32 stepping, fp64 accumulation" if fp32 else "fp64",
             "paths_per_s": paths * len(wall) / sum(wall), "ms_per_sweep": 1e3 * sum(wall) / len(wall),
             "kernel_ms": kernel_ms, "setup_s": setup, "premium": res.premium, "premium_trust": res.premium_trust,
+            "launch_shape": shape,
             "roofline": roofline(flops, kernel_ms, FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS, kernel)}
 
 
@@ -322,12 +334,15 @@ def heston_barrier(paths=1 << 20, steps=252, repeats=3):
         spec = configs.heston_down_out_call(paths=paths, precision=prec, uniform_steps=steps, greeks=greeks)
         b = BuiltRequest(spec)
         plan = mcl.Plan(b)
+        t0 = time.perf_counter()
+        shape = shape_of(plan.tune(), time.perf_counter() - t0)
         plan.run()
         ms = [plan.run().sweep_ms for _ in range(repeats)]
         r = plan.run()
         pw = [mcl.price_pathwise(b) for _ in range(repeats)]
         res[name] = {"bump": {"premium": r.premium, "premium_trust": r.premium_trust, "delta": r.delta,
-                              "gamma": r.gamma, "vega": r.vega, "ms": min(ms), "books": r.n_scenarios},
+                              "gamma": r.gamma, "vega": r.vega, "ms": min(ms), "books": r.n_scenarios,
+                              "launch_shape": shape},
                      "pathwise": {"premium": pw[-1].premium, "delta": pw[-1].delta, "delta_trust": pw[-1].delta_trust,
                                   "gamma": pw[-1].gamma, "vega": pw[-1].vega, "vega_trust": pw[-1].vega_trust,
                                   "ms": min(p.sweep_ms for p in pw)}}

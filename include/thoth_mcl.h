@@ -181,6 +181,28 @@ int thoth_mcl_prepare(const thoth_mcl_request* req, thoth_mcl_plan* plan);
 int thoth_mcl_run(thoth_mcl_plan plan, thoth_mcl_result* res);
 void thoth_mcl_release(thoth_mcl_plan plan);
 
+/* No reference counterpart (the reference's GPU hook has no launch shape to
+ * choose): the launch shape of a plan's sweep.  lev_group: time steps per LDS
+ * barrier (leverage layers and Sobol ops staged per group); workgroups_per_cu:
+ * 0 launches one workgroup per Sobol block of 256 paths, W > 0 a persistent
+ * grid of W workgroups per CU pulling (book, block) items.  Every shape gives
+ * the same per-block partial sums, hence the same result bit for bit.
+ * thoth_mcl_tune times every candidate (lev_group 1, 2, 4, 8 x the launch
+ * forms that exist for the plan's books) twice on the plan's own books, keeps
+ * the fastest and caches it by book shape for the process (candidates = 0:
+ * the cached choice); thoth_mcl_set_shape forces one; thoth_mcl_get_shape
+ * reports the current one (device_ms: the tuner's best time, 0 if untuned). */
+typedef struct thoth_plan_shape {
+    int32_t lev_group;
+    int32_t workgroups_per_cu;
+    double device_ms;
+    int32_t candidates;
+    double default_ms;   /* thoth_mcl_tune: time of the shape the plan had before (0 when cached) */
+} thoth_plan_shape;
+int thoth_mcl_tune(thoth_mcl_plan plan, thoth_plan_shape* shape);
+int thoth_mcl_set_shape(thoth_mcl_plan plan, int32_t lev_group, int32_t workgroups_per_cu);
+int thoth_mcl_get_shape(thoth_mcl_plan plan, thoth_plan_shape* shape);
+
 /* Randomised QMC price of the base book of req (no Greeks): R = replicates
  * random digital shifts of Sobol points 0 .. req->mcl.paths - 1, each an
  * independent estimate; the 95 % interval is Student-t over the R replicate

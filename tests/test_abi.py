@@ -45,7 +45,8 @@ def test_structs_match_header_sizes(tmp_path):
                ("thoth_cv_config", abi.CvConfig), ("thoth_cv_train_report", abi.CvTrainReport),
                ("thoth_cv_result", abi.CvResult), ("thoth_rqmc_result", abi.RqmcResult),
                ("thoth_pathwise_result", abi.PathwiseResult), ("thoth_vanilla_grid", abi.VanillaGrid),
-               ("thoth_grid_result", abi.GridResult), ("thoth_grid_tuning", abi.GridTuning)]
+               ("thoth_grid_result", abi.GridResult), ("thoth_grid_tuning", abi.GridTuning),
+               ("thoth_plan_shape", abi.PlanShape)]
     src = tmp_path / "abi_probe.c"
     src.write_text('#include <stdio.h>\n#include "thoth_mcl_gpu.h"\n#include "thoth_mcl_cv.h"\n#include "thoth_mcl_grid.h"\nint main(){'
                    + "".join('printf("%%zu\\n", sizeof(%s));' % n for n, _ in structs) + "}\n")

@@ -97,6 +97,11 @@ class Book(C.Structure):
                 ("c_mon_offset", _pi32), ("c_mon_index", _pi32), ("chol_dates", _pd)]
 
 
+class PlanShape(C.Structure):
+    _fields_ = [("lev_group", _i32), ("workgroups_per_cu", _i32), ("device_ms", _d), ("candidates", _i32),
+                ("default_ms", _d)]
+
+
 class SweepCfg(C.Structure):
     _fields_ = [("paths", _i64), ("sobol_skip", _i64), ("precision", _i32), ("block", _i32),
                 ("replicates", _i32), ("rqmc_seed", C.c_uint64), ("pseudo_random", _i32), ("seed", C.c_uint64)]

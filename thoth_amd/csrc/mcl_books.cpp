@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -418,6 +419,10 @@ DeviceBatch::~DeviceBatch()
 {
     if ( dbase && !external )
         (void)hipFree( dbase );
+    if ( ditems )
+        (void)hipFree( ditems );
+    if ( dcounter )
+        (void)hipFree( dcounter );
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
@@ -776,15 +781,12 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     return THOTH_OK;
 }
 
-int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
+// the sweep's device work on stream s: the tail normals (once per distinct
+// T), then the sweep kernel in the batch's launch shape
+static int EnqueueSweep( const DeviceBatch& B, hipStream_t s )
 {
-    hipStream_t s = Stream();
-    hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
-    HIP_OK( hipEventRecord( e0, s ) );
-    const dim3 grid( B.nblocks, B.n_books, B.replicates );
     // kernel control-variate mode: in-loop inference of a gated network is mode 3
     const int kcv = B.cv_mode == 2 && B.cv_act == THOTH_CV_GATE ? 3 : B.cv_mode;
-    // the normals past the Sobol table, once per distinct T, inside the timed region
     for ( const DeviceBatch::TailFill& fl : B.fills )
     {
         const bool filled = B.precision == 0
@@ -795,21 +797,52 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
         if ( !filled )
             return SetError( THOTH_ERR_DEVICE, "tail normal fill launch failed" );
     }
-    bool ok = B.prng ? ( B.precision == 0 ? launch_sweep_prng_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
-                                                                   DeviceDirections(), B.q0, B.first, B.last )
-                                          : launch_sweep_prng_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
-                                                                   DeviceDirections(), B.q0, B.first, B.last ) )
-              : B.precision == 0
-                  ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                      B.first, B.last, B.dshifts )
-                  : ( B.kind && launch_sweep_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
-                                                          DeviceDirections(), B.q0, B.first, B.last, B.dshifts,
-                                                          kcv ) ) ||
-                        launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                          B.first, B.last, B.dshifts, kcv );
+    bool ok;
+    if ( B.persist_wpc > 0 )
+    {
+        // work queue of (book, Sobol block) items; the counter is cleared by a kernel
+        launch_counter_reset( s, B.dcounter );
+        const dim3 pg( (uint32_t)( B.cus * B.persist_wpc ) );
+        const DevBook* db = B.dbooks;
+        const uint32_t* V = DeviceDirections();
+        ok = B.precision == 0 ? launch_persistent_f64( B.na_max, B.depth, pg, s, B.lds_bytes, db, V, B.q0, B.first,
+                                                       B.last, B.ditems, B.n_items, B.nblocks, B.dcounter )
+             : B.kind         ? launch_persistent_f32_uniform( B.kind, B.na_max, B.depth, pg, s, B.lds_bytes, db, V,
+                                                               B.q0, B.first, B.last, B.ditems, B.n_items, B.nblocks,
+                                                               B.dcounter )
+                              : launch_persistent_f32( B.na_max, B.depth, pg, s, B.lds_bytes, db, V, B.q0, B.first,
+                                                       B.last, B.ditems, B.n_items, B.nblocks, B.dcounter );
+    }
+    else
+    {
+        const dim3 grid( B.nblocks, B.n_books, B.replicates );
+        ok = B.prng ? ( B.precision == 0 ? launch_sweep_prng_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                                  DeviceDirections(), B.q0, B.first, B.last )
+                                         : launch_sweep_prng_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                                  DeviceDirections(), B.q0, B.first, B.last ) )
+             : B.precision == 0
+                 ? launch_sweep_f64( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
+                                     B.first, B.last, B.dshifts )
+                 : ( B.kind && launch_sweep_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                         DeviceDirections(), B.q0, B.first, B.last, B.dshifts,
+                                                         kcv ) ) ||
+                       launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
+                                         B.first, B.last, B.dshifts, kcv );
+    }
     if ( !ok )
         return SetError( THOTH_ERR_UNSUPPORTED, "no kernel instantiation for this book shape" );
     HIP_OK( hipGetLastError() );
+    return THOTH_OK;
+}
+
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
+{
+    hipStream_t s = Stream();
+    hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
+    HIP_OK( hipEventRecord( e0, s ) );
+    int rc = EnqueueSweep( B, s );
+    if ( rc != THOTH_OK )
+        return rc;
     HIP_OK( hipEventRecord( e1, s ) );
     std::vector<double> part( B.part_total );
     HIP_OK( hipMemcpyAsync( part.data(), B.dpart, B.part_total * sizeof( double ), hipMemcpyDeviceToHost, s ) );
@@ -868,6 +901,136 @@ int SetLevGroup( DeviceBatch& B, int G )
     HIP_OK( hipMemcpyAsync( const_cast<DevBook*>( B.dbooks ), B.db.data(), B.db.size() * sizeof( DevBook ),
                             hipMemcpyHostToDevice, s ) );
     HIP_OK( hipStreamSynchronize( s ) );
+    return THOTH_OK;
+}
+
+// the persistent form exists for the generic kernels and the uniform-book
+// instantiations of the workloads (one-equity LSV and Heston, five-equity LSV)
+static bool PersistentAvailable( const DeviceBatch& B )
+{
+    if ( B.prng || B.cv_mode != 0 || B.replicates != 1 || B.pathwise || B.grid_total > 0 )
+        return false;
+    if ( B.precision == 0 || B.kind == 0 )
+        return true;
+    if ( B.depth > 10 )
+        return false;
+    return ( ( B.kind == 1 || B.kind == 2 ) && B.na_max == 1 ) || ( ( B.kind == 1 || B.kind == 3 ) && B.na_max == 5 );
+}
+
+int SetShape( DeviceBatch& B, int lev_group, int wpc )
+{
+    if ( wpc < 0 || wpc > 16 )
+        return SetError( THOTH_ERR_INVALID, "workgroups per CU must be in [0, 16]" );
+    if ( wpc > 0 && !PersistentAvailable( B ) )
+        return SetError( THOTH_ERR_UNSUPPORTED, "no persistent form of the sweep for these books" );
+    int rc = SetLevGroup( B, lev_group );
+    if ( rc != THOTH_OK )
+        return rc;
+    if ( wpc > 0 && !B.ditems )
+    {
+        // items book-major, Sobol blocks in order
+        std::vector<int2> items;
+        items.reserve( (size_t)B.n_books * B.nblocks );
+        for ( int b = 0; b < B.n_books; b++ )
+            for ( uint32_t t = 0; t < B.nblocks; t++ )
+                items.push_back( make_int2( b, (int)t ) );
+        hipDeviceProp_t prop;
+        HIP_OK( hipGetDeviceProperties( &prop, 0 ) );
+        B.cus = prop.multiProcessorCount;
+        HIP_OK( hipMalloc( &B.ditems, items.size() * sizeof( int2 ) ) );
+        HIP_OK( hipMalloc( &B.dcounter, sizeof( int ) ) );
+        hipStream_t s = Stream();
+        HIP_OK( hipMemcpyAsync( B.ditems, items.data(), items.size() * sizeof( int2 ), hipMemcpyHostToDevice, s ) );
+        HIP_OK( hipStreamSynchronize( s ) );
+        B.n_items = (int)items.size();
+    }
+    B.persist_wpc = wpc;
+    return THOTH_OK;
+}
+
+namespace
+{
+std::string ShapeKey( const DeviceBatch& B )
+{
+    std::string k = std::to_string( B.precision ) + "/" + std::to_string( B.kind ) + "/" + std::to_string( B.na_max ) +
+                    "/" + std::to_string( B.depth ) + "/" + std::to_string( B.n_books ) + "/" +
+                    std::to_string( B.nblocks ) + "/" + std::to_string( B.prng ) + "/" + std::to_string( B.lds_tail );
+    for ( const DevBook& d : B.db )
+        k += ":" + std::to_string( d.m ) + "," + std::to_string( d.factors ) + "," + std::to_string( d.lev_points ) +
+             "," + std::to_string( d.n_contracts ) + "," + std::to_string( d.n_ops );
+    return k;
+}
+
+int TimeSweep( const DeviceBatch& B, double& ms )
+{
+    hipStream_t s = Stream();
+    hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
+    HIP_OK( hipEventRecord( e0, s ) );
+    int rc = EnqueueSweep( B, s );
+    if ( rc != THOTH_OK )
+        return rc;
+    HIP_OK( hipEventRecord( e1, s ) );
+    HIP_OK( hipEventSynchronize( e1 ) );
+    float f = 0;
+    HIP_OK( hipEventElapsedTime( &f, e0, e1 ) );
+    ms = f;
+    return THOTH_OK;
+}
+}  // namespace
+
+int TuneShape( DeviceBatch& B, ShapeChoice& out )
+{
+    // callers hold the library lock
+    static std::map<std::string, ShapeChoice> cache;
+    const std::string key = ShapeKey( B );
+    auto it = cache.find( key );
+    if ( it != cache.end() )
+    {
+        int rc = SetShape( B, it->second.lev_group, it->second.wpc );
+        if ( rc != THOTH_OK )
+            return rc;
+        out = it->second;
+        out.candidates = 0;
+        out.default_ms = 0;
+        return THOTH_OK;
+    }
+    const int groups[] = { 1, 2, 4, 8 };
+    const int forms[] = { 0, 1, 2, 3, 4 };  // 0: one workgroup per Sobol block; W > 0: persistent, W per CU
+    ShapeChoice best;
+    best.ms = 1e300;
+    const int g0 = B.db.empty() ? 4 : B.db[0].lev_group, w0 = B.persist_wpc;
+    int candidates = 0;
+    for ( int g : groups )
+        for ( int w : forms )
+        {
+            int rc = SetShape( B, g, w );
+            if ( rc == THOTH_ERR_UNSUPPORTED )
+                continue;  // LDS does not fit, or no persistent form
+            if ( rc != THOTH_OK )
+                return rc;
+            double ms = 0;
+            if ( ( rc = TimeSweep( B, ms ) ) != THOTH_OK || ( rc = TimeSweep( B, ms ) ) != THOTH_OK )
+                return rc;
+            candidates++;
+            if ( g == g0 && w == w0 )
+                best.default_ms = ms;
+            if ( ms < best.ms )
+            {
+                const double d = best.default_ms;
+                best.ms = ms;
+                best.lev_group = g;
+                best.wpc = w;
+                best.default_ms = d;
+            }
+        }
+    if ( candidates == 0 )
+        return SetError( THOTH_ERR_UNSUPPORTED, "no launch shape fits these books" );
+    int rc = SetShape( B, best.lev_group, best.wpc );
+    if ( rc != THOTH_OK )
+        return rc;
+    best.candidates = candidates;
+    cache[key] = best;
+    out = best;
     return THOTH_OK;
 }
 

@@ -314,6 +314,7 @@ struct Plan
     std::vector<double> h;
     int n_contracts = 0, n_dates = 0;
     double calib_ms = 0;
+    double tuned_ms = 0;  // best sweep time of the launch-shape tuner (0: untuned)
 };
 
 namespace
@@ -810,6 +811,51 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_run( thoth
         return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
     std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
     return thoth::host::RunPlan( *reinterpret_cast<thoth::host::Plan*>( plan ), res );
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_tune( thoth_mcl_plan plan,
+                                                                        thoth_plan_shape* shape )
+{
+    if ( !plan || !shape )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    auto* p = reinterpret_cast<thoth::host::Plan*>( plan );
+    thoth::ShapeChoice c;
+    int rc = thoth::TuneShape( p->batch, c );
+    if ( rc != THOTH_OK )
+        return rc;
+    p->tuned_ms = c.ms;
+    shape->lev_group = c.lev_group;
+    shape->workgroups_per_cu = c.wpc;
+    shape->device_ms = c.ms;
+    shape->candidates = c.candidates;
+    shape->default_ms = c.default_ms;
+    return THOTH_OK;
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_set_shape( thoth_mcl_plan plan,
+                                                                             int32_t lev_group,
+                                                                             int32_t workgroups_per_cu )
+{
+    if ( !plan )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    return thoth::SetShape( reinterpret_cast<thoth::host::Plan*>( plan )->batch, lev_group, workgroups_per_cu );
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_get_shape( thoth_mcl_plan plan,
+                                                                             thoth_plan_shape* shape )
+{
+    if ( !plan || !shape )
+        return thoth::SetError( THOTH_ERR_INVALID, "null argument" );
+    std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
+    const auto* p = reinterpret_cast<const thoth::host::Plan*>( plan );
+    shape->lev_group = p->batch.db.empty() ? 0 : p->batch.db[0].lev_group;
+    shape->workgroups_per_cu = p->batch.persist_wpc;
+    shape->device_ms = p->tuned_ms;
+    shape->candidates = 0;
+    shape->default_ms = 0;
+    return THOTH_OK;
 }
 
 extern "C" __attribute__( ( visibility( "default" ) ) ) void thoth_mcl_release( thoth_mcl_plan plan )

@@ -93,6 +93,13 @@ struct DeviceBatch
         char* dst;
     };
     std::vector<TailFill> fills;
+    // launch shape: one workgroup per Sobol block of every book (persist_wpc
+    // = 0), or a persistent grid of persist_wpc workgroups per CU pulling
+    // (book, block) items (the same per-block partial sums, so the same
+    // result bit for bit); the item list and counter are device-resident
+    int persist_wpc = 0, cus = 0, n_items = 0;
+    int2* ditems = nullptr;
+    int* dcounter = nullptr;
     DeviceBatch() = default;
     DeviceBatch( const DeviceBatch& ) = delete;
     DeviceBatch& operator=( const DeviceBatch& ) = delete;
@@ -137,6 +144,21 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
                   const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, bool pathwise = false,
                   char* ext_base = nullptr, size_t ext_cap = 0 );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
+// Launch shape of a plan's batch (reference-mode sweeps without the control
+// variate): steps per LDS barrier G and workgroups per CU of the persistent
+// form (0: one workgroup per Sobol block).  SetShape applies one (refusing
+// shapes whose LDS does not fit or that have no kernel instantiation);
+// TuneShape times every candidate (G in {1, 2, 4, 8} x launch forms) twice
+// on the batch itself, keeps the fastest, and caches it by book shape.
+struct ShapeChoice
+{
+    int lev_group = 4, wpc = 0;
+    double ms = 0;
+    int candidates = 0;   // timed (0: taken from the cache)
+    double default_ms = 0;  // the shape in effect before tuning (0 when cached)
+};
+int SetShape( DeviceBatch& B, int lev_group, int wpc );
+int TuneShape( DeviceBatch& B, ShapeChoice& out );
 // grid sums of every book, [nm][2 nk][2] each, from the last sweep (device reduction + copy)
 int ReduceGrids( const DeviceBatch& B, hipStream_t s );
 // re-stage the leverage layers G per barrier (ring of 2G): updates the device

@@ -57,6 +57,9 @@ bool launch_sweep_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream
 bool launch_sweep_f32_uniform5( int kind, int cv_mode, int depth, dim3 grid, hipStream_t s, size_t lds,
                                 const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last,
                                 const uint32_t* shifts );
+bool launch_persistent_f32_uniform5( int kind, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                                     const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const int2* items,
+                                     int n_items, uint32_t ntiles, int* counter );
 bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
                                     const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first,
                                     uint32_t last, const int2* items, int n_items, uint32_t ntiles, int* counter );

@@ -47,6 +47,9 @@ def lib():
         L.thoth_mcl_price.argtypes = [P(abi.Request), P(abi.Result)]
         L.thoth_mcl_prepare.argtypes = [P(abi.Request), P(C.c_void_p)]
         L.thoth_mcl_run.argtypes = [C.c_void_p, P(abi.Result)]
+        L.thoth_mcl_tune.argtypes = [C.c_void_p, P(abi.PlanShape)]
+        L.thoth_mcl_set_shape.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.thoth_mcl_get_shape.argtypes = [C.c_void_p, P(abi.PlanShape)]
         L.thoth_mcl_release.argtypes = [C.c_void_p]
         L.thoth_mcl_release.restype = None
         L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
@@ -185,6 +188,21 @@ class Plan:
         res = self.built.result()
         _check(lib().thoth_mcl_run(self.handle, C.byref(res)))
         return res
+
+    def tune(self):
+        """Pick the fastest launch shape for this plan's books (cached by book
+        shape); returns abi.PlanShape."""
+        sh = abi.PlanShape()
+        _check(lib().thoth_mcl_tune(self.handle, C.byref(sh)))
+        return sh
+
+    def set_shape(self, lev_group, workgroups_per_cu=0):
+        _check(lib().thoth_mcl_set_shape(self.handle, lev_group, workgroups_per_cu))
+
+    def shape(self):
+        sh = abi.PlanShape()
+        _check(lib().thoth_mcl_get_shape(self.handle, C.byref(sh)))
+        return sh
 
     def close(self):
         if self.handle:
