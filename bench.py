@@ -444,6 +444,7 @@ def main():
             "premium": c3["premium"],
             "premium_trust": c3["premium_trust"],
             "roofline": dict(c3["roofline"], traffic=None),
+            "launch_shape": c3["launch_shape"],
             "device": mcl.device_info(),
         })
     if "c3corr" in legs:

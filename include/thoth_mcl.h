@@ -109,6 +109,9 @@ typedef struct thoth_contract {
     int32_t monitoring_period_days;
     double barrier_up_level;
     double barrier_down_level;
+    /* exercise: 0 european, 1 american (vanilla on an equity: Longstaff-Schwartz
+     * on the recorded spot paths, pricer_mcl_american.cpp:137-447) */
+    int32_t american;
 } thoth_contract;
 
 typedef struct thoth_mcl_request {

@@ -1201,6 +1201,9 @@ ResolvedBook ResolveBook( const thoth_mcl_request& req, const Market& mk, const 
         rc.is_call = k.is_call;
         rc.underlying = k.underlying;
         rc.flow_index = index_of( k.maturity_days );
+        rc.american = k.american != 0;
+        if ( rc.american && ( k.kind != THOTH_VANILLA || k.underlying < 0 ) )
+            throw std::runtime_error( "american exercise: vanilla contracts on an equity" );
         // vanilla.cpp:29-34 / barrier.cpp:36-41: relative strike on the BASE spot
         double base_spot = k.underlying >= 0 ? req.assets[k.underlying].spot : 100.0;
         rc.strike = k.is_absolute_strike ? k.strike : k.strike / 100 * base_spot;
@@ -1322,6 +1325,11 @@ BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, i
         return x;
     };
     std::vector<double> cv( nc );
+    // path recording of the American contracts' equities (pricer_mcl_american.cpp:59-94)
+    out.rec.assign( n, {} );
+    for ( const ResolvedBook::Contract& k : b.contracts )
+        if ( k.american && out.rec[k.underlying].empty() )
+            out.rec[k.underlying].assign( (size_t)paths * nd, 0.0 );
     for ( int64_t p = 0; p < paths; p++ )
     {
         pg.NextPath();
@@ -1463,36 +1471,250 @@ BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, i
             Kahan( cv[c], out.sum[c], out.c[c], out.sum2[c], out.c2[c] );
         Kahan( book, out.book_sum, out.book_c, out.book_sum2, out.book_c2 );
         out.count++;
+        for ( int a = 0; a < n; a++ )
+            if ( !out.rec[a].empty() )
+                std::copy( S[a].begin(), S[a].end(), out.rec[a].begin() + (size_t)p * nd );
     }
     return out;
 }
 
+// ---- American Longstaff-Schwartz ---------------------------------------------
+std::vector<double> LeastSquares( const std::vector<double>& X, size_t n, size_t p, const std::vector<double>& y )
+{
+    // normal equations A = X^T X, g = X^T y (maths.cpp:419-441)
+    std::vector<double> A( p * p, 0.0 ), g( p, 0.0 );
+    for ( size_t a = 0; a < p; a++ )
+    {
+        for ( size_t b = 0; b <= a; b++ )
+        {
+            double s = 0.0;
+            for ( size_t k = 0; k < n; k++ )
+                s += X[k * p + a] * X[k * p + b];
+            A[a * p + b] = s;
+            A[b * p + a] = s;
+        }
+        double sg = 0.0;
+        for ( size_t k = 0; k < n; k++ )
+            sg += X[k * p + a] * y[k];
+        g[a] = sg;
+    }
+    // A = L L^T (maths.cpp:354-381), then L z = g and L^T beta = z (maths.cpp:447-469)
+    if ( !CholeskyDecomposeLower( A, (int)p ) )
+        return {};
+    std::vector<double> z( p ), beta( p );
+    for ( size_t i = 0; i < p; i++ )
+    {
+        double s = g[i];
+        for ( size_t k = 0; k < i; k++ )
+            s -= A[i * p + k] * z[k];
+        z[i] = s / A[i * p + i];
+    }
+    for ( size_t i = p; i-- > 0; )
+    {
+        double s = z[i];
+        for ( size_t k = i + 1; k < p; k++ )
+            s -= A[k * p + i] * beta[k];
+        beta[i] = s / A[i * p + i];
+    }
+    return beta;
+}
+
+namespace
+{
+double Intrinsic( double s, double strike, bool is_call )  // vanilla.cpp:79-82, finance.cpp:9-32
+{
+    return std::max( is_call ? s - strike : strike - s, 0.0 );
+}
+}  // namespace
+
+AmericanPolicy FitAmericanPolicy( double strike, bool is_call, const std::vector<double>& paths, size_t N, size_t M,
+                                  const std::vector<double>& tau, const std::vector<double>& rates )
+{
+    AmericanPolicy pol;
+    if ( M < 2 )
+        return pol;
+    const size_t kMinItm = 50;  // pricer_mcl_american.cpp:274
+    pol.basis_norm = strike > 0 ? strike : paths[0];  // vanilla.hpp:38-41
+    pol.b0.assign( M, 0.0 );
+    pol.b1.assign( M, 0.0 );
+    pol.b2.assign( M, 0.0 );
+    pol.has_fit.assign( M, 0 );
+    std::vector<double> cf( N );
+    for ( size_t p = 0; p < N; p++ )
+        cf[p] = Intrinsic( paths[p * M + M - 1], strike, is_call );
+    std::vector<size_t> itm;
+    std::vector<double> X, y;
+    for ( int t = (int)M - 2; t >= 1; t-- )
+    {
+        const double df = std::exp( rates[t] * tau[t] - rates[t + 1] * tau[t + 1] );
+        for ( size_t p = 0; p < N; p++ )
+            cf[p] *= df;
+        itm.clear();
+        for ( size_t p = 0; p < N; p++ )
+            if ( Intrinsic( paths[p * M + t], strike, is_call ) > 0 )
+                itm.push_back( p );
+        if ( itm.size() < kMinItm )
+            continue;
+        const size_t ni = itm.size();
+        X.assign( ni * 3, 0.0 );
+        y.assign( ni, 0.0 );
+        for ( size_t k = 0; k < ni; k++ )
+        {
+            const double m = paths[itm[k] * M + t] / pol.basis_norm;
+            X[k * 3] = 1.0;
+            X[k * 3 + 1] = m;
+            X[k * 3 + 2] = m * m;
+            y[k] = cf[itm[k]];
+        }
+        const std::vector<double> beta = LeastSquares( X, ni, 3, y );
+        if ( beta.empty() )
+            continue;
+        pol.b0[t] = beta[0];
+        pol.b1[t] = beta[1];
+        pol.b2[t] = beta[2];
+        pol.has_fit[t] = 1;
+        for ( size_t p : itm )
+        {
+            const double s = paths[p * M + t];
+            const double m = s / pol.basis_norm;
+            const double continuation = pol.b0[t] + pol.b1[t] * m + pol.b2[t] * m * m;
+            const double intrinsic = Intrinsic( s, strike, is_call );
+            if ( intrinsic >= continuation )
+                cf[p] = intrinsic;
+        }
+    }
+    return pol;
+}
+
+double ApplyAmericanPolicy( double strike, bool is_call, const std::vector<double>& paths, size_t N, size_t M,
+                            const std::vector<double>& tau, const std::vector<double>& rates,
+                            const AmericanPolicy& pol, double& trust )
+{
+    trust = 0;
+    if ( M < 2 )
+        return 0;
+    double sum = 0, sum2 = 0;
+    for ( size_t p = 0; p < N; p++ )
+    {
+        double value = 0;
+        for ( size_t t = 1; t < M; t++ )
+        {
+            const double s = paths[p * M + t];
+            const double intrinsic = Intrinsic( s, strike, is_call );
+            if ( t == M - 1 )
+            {
+                value = intrinsic * std::exp( -rates[t] * ( tau[t] - tau[0] ) );
+                break;
+            }
+            if ( intrinsic > 0 && pol.has_fit[t] )
+            {
+                const double m = s / pol.basis_norm;
+                const double continuation = pol.b0[t] + pol.b1[t] * m + pol.b2[t] * m * m;
+                if ( intrinsic >= continuation )
+                {
+                    value = intrinsic * std::exp( -rates[t] * ( tau[t] - tau[0] ) );
+                    break;
+                }
+            }
+        }
+        sum += value;
+        sum2 += value * value;
+    }
+    const double mean = sum / (double)N;
+    trust = std::sqrt( ( sum2 / (double)N - mean * mean ) / (double)( N - 1 ) );
+    return std::max( mean, Intrinsic( paths[0], strike, is_call ) );
+}
+
 // ---- pricing + bump-and-revalue Greeks (pricer.cpp:273-391, pricer_mcl_greeks.cpp:114-218)
+// With American contracts (pricer_mcl_american.cpp:137-248): the exercise policy
+// of each is fitted once on the base scenario's recorded paths and applied frozen
+// to every scenario's; the contract's value replaces its European value in the
+// scenario's book premium, and the base book premium is re-aggregated from the
+// contracts (the book trust stays the sweep's).
 int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
 {
     const Market base = BuildMarket( req );
     const Grid g = BuildGrid( req, base );
     const int nc = req.n_contracts;
-    auto run = [&]( const Market& mk )
+    struct Scen
     {
-        return SweepBook( ResolveBook( req, mk, g ), req.mcl.paths, req.mcl.sobol_skip, req.mcl.seed,
-                          req.mcl.use_sobol != 0 );
+        BookSums sums;
+        double book;                  // book premium for the finite differences
+        std::vector<double> contract;  // contract premiums (American: the policy's value)
+        std::vector<double> trust;
     };
-    BookSums b0 = run( base );
-    res->premium = b0.Mean( -1 );
-    res->premium_trust = b0.Trust( -1 );
+    std::vector<int> am;
+    std::vector<AmericanPolicy> pol;
+    const ResolvedBook rb0 = ResolveBook( req, base, g );
+    for ( int c = 0; c < nc; c++ )
+        if ( rb0.contracts[c].american )
+            am.push_back( c );
+    // one contract's exercise data of a scenario: paths [N][M], tau, zero rates
+    auto exercise_data = [&]( const ResolvedBook& rb, const BookSums& s, int c, std::vector<double>& P,
+                              std::vector<double>& tau, std::vector<double>& rates, size_t& N, size_t& M )
+    {
+        const ResolvedBook::Contract& k = rb.contracts[c];
+        const size_t nd = g.t.size();
+        M = (size_t)k.flow_index + 1;
+        N = (size_t)s.count;
+        P.assign( N * M, 0.0 );
+        const std::vector<double>& rec = s.rec[k.underlying];
+        for ( size_t p = 0; p < N; p++ )
+            std::copy( rec.begin() + p * nd, rec.begin() + p * nd + M, P.begin() + p * M );
+        tau.assign( g.t.begin(), g.t.begin() + M );
+        rates.assign( rb.disc_rate.begin(), rb.disc_rate.begin() + M );
+    };
+    auto run = [&]( const Market& mk, bool fit )
+    {
+        const ResolvedBook rb = ResolveBook( req, mk, g );
+        Scen sc;
+        sc.sums = SweepBook( rb, req.mcl.paths, req.mcl.sobol_skip, req.mcl.seed, req.mcl.use_sobol != 0 );
+        sc.book = sc.sums.Mean( -1 );
+        sc.contract.resize( nc );
+        sc.trust.resize( nc );
+        for ( int c = 0; c < nc; c++ )
+        {
+            sc.contract[c] = sc.sums.Mean( c );
+            sc.trust[c] = sc.sums.Trust( c );
+        }
+        std::vector<double> P, tau, rates;
+        size_t N = 0, M = 0;
+        for ( size_t j = 0; j < am.size(); j++ )
+        {
+            const int c = am[j];
+            const ResolvedBook::Contract& k = rb.contracts[c];
+            exercise_data( rb, sc.sums, c, P, tau, rates, N, M );
+            if ( fit )
+                pol.push_back( FitAmericanPolicy( k.strike, k.is_call != 0, P, N, M, tau, rates ) );
+            double tr = 0;
+            const double v = ApplyAmericanPolicy( k.strike, k.is_call != 0, P, N, M, tau, rates, pol[j], tr );
+            sc.book += v - sc.contract[c];
+            sc.contract[c] = v;
+            sc.trust[c] = tr;
+        }
+        return sc;
+    };
+    const Scen b0 = run( base, true );
+    double p0 = b0.book;
+    if ( !am.empty() )
+    {
+        p0 = 0;
+        for ( int c = 0; c < nc; c++ )
+            p0 += b0.contract[c];
+    }
+    res->premium = p0;
+    res->premium_trust = b0.sums.Trust( -1 );
     res->n_dates = (int)g.t.size();
     res->n_scenarios = 1;
     for ( int c = 0; c < nc; c++ )
     {
         if ( res->contract_premium )
-            res->contract_premium[c] = b0.Mean( c );
+            res->contract_premium[c] = b0.contract[c];
         if ( res->contract_trust )
-            res->contract_trust[c] = b0.Trust( c );
+            res->contract_trust[c] = b0.trust[c];
     }
     res->delta = res->gamma = res->vega = res->rho = 0;
     std::vector<double> cd( nc, 0 ), cg( nc, 0 ), cvv( nc, 0 ), cr( nc, 0 );
-    const double p0 = res->premium;
     for ( int a = 0; a < req.n_assets; a++ )
     {
         const double spot = base.eq[a].spot;
@@ -1501,11 +1723,11 @@ int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
             const double h = GREEK_SPOT_BUMP * spot;
             Market mk = base;
             mk.eq[a].spot = spot + h;
-            BookSums s = run( mk );
+            const Scen s = run( mk, false );
             res->n_scenarios++;
-            res->delta += ( s.Mean( -1 ) - p0 ) / h;
+            res->delta += ( s.book - p0 ) / h;
             for ( int c = 0; c < nc; c++ )
-                cd[c] += ( s.Mean( c ) - b0.Mean( c ) ) / h;
+                cd[c] += ( s.contract[c] - b0.contract[c] ) / h;
         }
         if ( req.request_greeks & THOTH_GREEK_GAMMA )
         {
@@ -1513,11 +1735,11 @@ int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
             Market mu = base, md = base;
             mu.eq[a].spot = spot + h;
             md.eq[a].spot = spot - h;
-            BookSums su = run( mu ), sd = run( md );
+            const Scen su = run( mu, false ), sd = run( md, false );
             res->n_scenarios += 2;
-            res->gamma += ( su.Mean( -1 ) - 2 * p0 + sd.Mean( -1 ) ) / ( h * h );
+            res->gamma += ( su.book - 2 * p0 + sd.book ) / ( h * h );
             for ( int c = 0; c < nc; c++ )
-                cg[c] += ( su.Mean( c ) - 2 * b0.Mean( c ) + sd.Mean( c ) ) / ( h * h );
+                cg[c] += ( su.contract[c] - 2 * b0.contract[c] + sd.contract[c] ) / ( h * h );
         }
     }
     if ( req.request_greeks & THOTH_GREEK_VEGA )
@@ -1525,22 +1747,22 @@ int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
         Market mk = base;
         for ( Equity& e : mk.eq )
             e.vol_shift = GREEK_VOL_BUMP;
-        BookSums s = run( mk );
+        const Scen s = run( mk, false );
         res->n_scenarios++;
-        res->vega = ( s.Mean( -1 ) - p0 ) / GREEK_VOL_BUMP * 0.01;
+        res->vega = ( s.book - p0 ) / GREEK_VOL_BUMP * 0.01;
         for ( int c = 0; c < nc; c++ )
-            cvv[c] = ( s.Mean( c ) - b0.Mean( c ) ) / GREEK_VOL_BUMP * 0.01;
+            cvv[c] = ( s.contract[c] - b0.contract[c] ) / GREEK_VOL_BUMP * 0.01;
     }
     if ( req.request_greeks & THOTH_GREEK_RHO )
     {
         Market mk = base;
         mk.rate.shift = GREEK_RATE_BUMP;  // projection AND discounting curves (pricer.cpp:344-359)
         mk.discount.shift = GREEK_RATE_BUMP;
-        BookSums s = run( mk );
+        const Scen s = run( mk, false );
         res->n_scenarios++;
-        res->rho = ( s.Mean( -1 ) - p0 ) / GREEK_RATE_BUMP * 0.01;
+        res->rho = ( s.book - p0 ) / GREEK_RATE_BUMP * 0.01;
         for ( int c = 0; c < nc; c++ )
-            cr[c] = ( s.Mean( c ) - b0.Mean( c ) ) / GREEK_RATE_BUMP * 0.01;
+            cr[c] = ( s.contract[c] - b0.contract[c] ) / GREEK_RATE_BUMP * 0.01;
     }
     for ( int c = 0; c < nc; c++ )
     {

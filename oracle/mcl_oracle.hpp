@@ -176,6 +176,10 @@ struct BookSums
     double book_sum = 0, book_sum2 = 0, book_c = 0, book_c2 = 0;
     std::vector<double> sum, sum2, c, c2;
     long count = 0;
+    // recorded spot paths of the equities American contracts exercise on
+    // (path_recorder: every diffusion date), rec[a] = [paths][n_dates], empty
+    // for the other equities
+    std::vector<std::vector<double>> rec;
     double Mean( int i ) const;                      // i < 0: book
     double Trust( int i ) const;
 };
@@ -199,7 +203,7 @@ struct ResolvedBook
     {
         int kind, is_call, underlying, flow_index;
         double strike, level;
-        bool is_up, is_in;
+        bool is_up, is_in, american;
         std::vector<int> monitor;
     };
     std::vector<Contract> contracts;
@@ -210,5 +214,22 @@ Market BuildMarket( const thoth_mcl_request& req );
 ResolvedBook ResolveBook( const thoth_mcl_request& req, const Market& mk, const Grid& g );
 BookSums SweepBook( const ResolvedBook& b, int64_t paths, uint64_t sobol_skip, int seed, bool use_sobol = true );
 int Price( const thoth_mcl_request& req, thoth_mcl_result* res );
+
+// ---- American Longstaff-Schwartz (pricer_mcl_american.cpp:250-447) -----------
+// ordinary least squares by the normal equations and Cholesky (maths.cpp:419-470);
+// X is [n][p] row-major; empty when X^T X is not positive definite
+std::vector<double> LeastSquares( const std::vector<double>& X, size_t n, size_t p, const std::vector<double>& y );
+struct AmericanPolicy
+{
+    double basis_norm = 1;
+    std::vector<double> b0, b1, b2;  // continuation = b0 + b1 m + b2 m^2, m = S / basis_norm
+    std::vector<char> has_fit;
+};
+// paths [N][M] (column 0 today, M - 1 maturity), tau and zero rates per column
+AmericanPolicy FitAmericanPolicy( double strike, bool is_call, const std::vector<double>& paths, size_t N, size_t M,
+                                  const std::vector<double>& tau, const std::vector<double>& rates );
+double ApplyAmericanPolicy( double strike, bool is_call, const std::vector<double>& paths, size_t N, size_t M,
+                            const std::vector<double>& tau, const std::vector<double>& rates,
+                            const AmericanPolicy& pol, double& trust );
 
 }  // namespace oracle

@@ -145,6 +145,34 @@ int oracle_path_noise( int n_times, const double* t, int factors, int64_t skip, 
     return oracle_path_noise_seeded( n_times, t, factors, skip, paths, 0, 1, out );
 }
 
+// LeastSquares (maths.cpp:419-470): X [n][p] row-major; returns 1 and beta, 0 if singular
+int oracle_least_squares( const double* X, int64_t n, int p, const double* y, double* beta )
+{
+    std::vector<double> x( X, X + n * p ), yy( y, y + n );
+    const std::vector<double> b = LeastSquares( x, (size_t)n, (size_t)p, yy );
+    if ( b.empty() )
+        return 0;
+    std::copy( b.begin(), b.end(), beta );
+    return 1;
+}
+
+// Longstaff-Schwartz on given paths [N][M]: the fitted policy (b[3 t + k], fit[t])
+// and the value of applying it to the same paths, with its trust
+double oracle_lsm( const double* paths, int64_t N, int M, const double* tau, const double* rates, double strike,
+                   int is_call, double* b, int* fit, double* trust )
+{
+    std::vector<double> P( paths, paths + N * M ), t( tau, tau + M ), r( rates, rates + M );
+    const AmericanPolicy pol = FitAmericanPolicy( strike, is_call != 0, P, (size_t)N, (size_t)M, t, r );
+    for ( int k = 0; k < M; k++ )
+    {
+        b[3 * k] = pol.b0.empty() ? 0 : pol.b0[k];
+        b[3 * k + 1] = pol.b1.empty() ? 0 : pol.b1[k];
+        b[3 * k + 2] = pol.b2.empty() ? 0 : pol.b2[k];
+        fit[k] = pol.has_fit.empty() ? 0 : pol.has_fit[k];
+    }
+    return ApplyAmericanPolicy( strike, is_call != 0, P, (size_t)N, (size_t)M, t, r, pol, *trust );
+}
+
 static Sabr SabrOf( const thoth_sabr_surface* s )
 {
     Sabr sb;

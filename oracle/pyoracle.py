@@ -38,6 +38,10 @@ def lib():
     L.oracle_path_noise_seeded.argtypes = [C.c_int, P(C.c_double), C.c_int, C.c_int64, C.c_int64, C.c_uint64,
                                            C.c_int, P(C.c_double)]
     L.oracle_xoshiro_raw.argtypes = [P(C.c_uint64), C.c_int64, P(C.c_uint64)]
+    L.oracle_least_squares.argtypes = [P(C.c_double), C.c_int64, C.c_int, P(C.c_double), P(C.c_double)]
+    L.oracle_lsm.argtypes = [P(C.c_double), C.c_int64, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_int,
+                             P(C.c_double), P(C.c_int), P(C.c_double)]
+    L.oracle_lsm.restype = C.c_double
     L.oracle_sabr_vol.argtypes = [P(abi.SabrSurface), C.c_double, C.c_double, C.c_double]
     L.oracle_sabr_vol.restype = C.c_double
     L.oracle_sabr_hagan.argtypes = [P(abi.SabrSurface), C.c_double, C.c_double, C.c_double]
@@ -111,6 +115,28 @@ def path_noise(t, factors, paths, skip=0, seed=0, use_sobol=True):
     out = np.zeros((paths, factors, t.size), np.float64)
     lib().oracle_path_noise_seeded(t.size, _p(t), factors, skip, paths, seed % (1 << 64), int(use_sobol), _p(out))
     return out
+
+
+def least_squares(X, y):
+    X = np.ascontiguousarray(X, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    beta = np.zeros(X.shape[1])
+    ok = lib().oracle_least_squares(_p(X), X.shape[0], X.shape[1], _p(y), _p(beta))
+    return beta if ok else None
+
+
+def lsm(paths, tau, rates, strike, is_call):
+    """Longstaff-Schwartz on paths [N][M]: (value, trust, betas [M][3], fitted [M])."""
+    P = np.ascontiguousarray(paths, np.float64)
+    N, M = P.shape
+    tau = np.ascontiguousarray(tau, np.float64)
+    rates = np.ascontiguousarray(rates, np.float64)
+    b = np.zeros((M, 3))
+    fit = np.zeros(M, np.int32)
+    tr = C.c_double()
+    v = lib().oracle_lsm(_p(P), N, M, _p(tau), _p(rates), strike, int(is_call), _p(b), _p(fit, C.c_int),
+                         C.byref(tr))
+    return v, tr.value, b, fit.astype(bool)
 
 
 def xoshiro_raw(state, n):

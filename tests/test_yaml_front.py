@@ -27,11 +27,12 @@ SUPPORTED = {
     "barrier_mono_continuous_down_in_call_mcl", "barrier_mono_continuous_up_out_put_mcl",
     "barrier_mono_discrete_in_call_mcl", "vanilla_mono_eu_put_mcl", "vanilla_mono_eu_call_mcl_gpu",
     "vanilla_mono_eu_put_mcl_gpu", "vanilla_mono_eu_call_lsv_mcl", "vanilla_mono_eu_call_ois_mcl",
-    "vanilla_mono_eu_call_mcl_pseudo",
+    "vanilla_mono_eu_call_mcl_pseudo", "vanilla_mono_am_put_mcl", "vanilla_mono_am_put_heston_mcl",
+    "vanilla_mono_am_call_mcl",
 }
 
 REFUSED = {  # pricer -> a word of the reason
-    "vanilla_mono_am_put_mcl": "american", "vanilla_mono_eu_call_div_mcl": "discrete_dividends",
+    "vanilla_mono_eu_call_div_mcl": "discrete_dividends",
     "vanilla_mono_eu_call_quanto_mcl": "currency", "vanilla_mono_eu_call_compo_mcl": "composite",
     "vanilla_multi_eu_call_basket_mcl": "basket", "vanilla_mono_eu_call_bates_mcl": "Bates",
     "variance_mono_mcl": "variance", "vanilla_mono_eu_call_sabr_mcl": "sabr_volatility",
@@ -103,7 +104,12 @@ def test_simple_call_prices_black_scholes(oracle):
 
 def test_unsupported_is_a_value_error():
     with pytest.raises(ValueError):
-        yf.translate(_objs("matrix"), "vanilla_mono_am_put_mcl")
+        yf.translate(_objs("matrix"), "vanilla_mono_eu_call_bates_mcl")
+
+
+def test_american_exercise_translates():
+    c = yf.translate(_objs("matrix"), "vanilla_mono_am_put_mcl").spec.contracts[0]
+    assert isinstance(c, Vanilla) and c.american and not c.is_call
 
 
 def test_theta_is_the_one_day_roll(oracle):

@@ -60,7 +60,7 @@ class Contract(C.Structure):
     _fields_ = [("kind", _i32), ("is_call", _i32), ("underlying", _i32), ("maturity_days", _i32),
                 ("strike", _d), ("is_absolute_strike", _i32), ("barrier_type", _i32),
                 ("discrete_monitoring", _i32), ("monitoring_period_days", _i32),
-                ("barrier_up_level", _d), ("barrier_down_level", _d)]
+                ("barrier_up_level", _d), ("barrier_down_level", _d), ("american", _i32)]
 
 
 class Request(C.Structure):

@@ -58,6 +58,7 @@ class Vanilla:
     strike: float
     is_call: bool = True
     is_absolute_strike: bool = True
+    american: bool = False            # exercise: american (Longstaff-Schwartz) or european
 
 
 @dataclass
@@ -158,6 +159,7 @@ class BuiltRequest:
                     x.barrier_down_level = k.level
             else:
                 x.kind = abi.VANILLA
+                x.american = int(k.american)
         r = abi.Request()
         m = spec.mcl
         r.mcl.paths, r.mcl.max_day_step, r.mcl.min_day_step = m.paths, m.max_day_step, m.min_day_step

@@ -47,6 +47,7 @@ UNITS = [
     ("mcl_prng_f64.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_prng_f32.hip", "hipcc", []),
     ("mcl_prng.cpp", "g++", []),
+    ("mcl_lsm.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_gpu.cpp", "g++", []),
     ("mcl_books.cpp", "g++", []),
     ("mcl_market.cpp", "g++", []),

@@ -426,7 +426,8 @@ DeviceBatch::~DeviceBatch()
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv, const GridSpec* grids, bool pathwise, char* ext_base, size_t ext_cap )
+                  const CvSpec* cv, const GridSpec* grids, bool pathwise, char* ext_base, size_t ext_cap,
+                  const RecordSpec* rec )
 {
     B.pathwise = pathwise;
     int rc = EnsureDevice();
@@ -538,6 +539,24 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         B.kind = 0;
         if ( B.pseudo )
             B.depth = 1;  // no bridge
+    }
+    // path recording (American exercise): the generic kernels, plain sweeps
+    B.recording = false;
+    for ( int b = 0; rec && b < n_books; b++ )
+        if ( rec[b].n > 0 )
+        {
+            if ( rec[b].udl < 0 || rec[b].udl >= B.db[b].n_assets || rec[b].n > B.db[b].n_dates )
+                return SetError( THOTH_ERR_INVALID, Why( b, "recorded equity or date count out of range" ) );
+            B.recording = true;
+        }
+    if ( B.recording )
+    {
+        if ( rqmc || cv || pathwise )
+            return SetError( THOTH_ERR_UNSUPPORTED, "American exercise runs in the reference-mode sweep only" );
+        for ( int b = 0; b < n_books; b++ )
+            if ( grids && grids[b].nm > 0 )
+                return SetError( THOTH_ERR_UNSUPPORTED, "American exercise and vanilla grids are separate sweeps" );
+        B.kind = 0;
     }
     // dynamic LDS: the sweep's Sobol ring (the pathwise kernel: the Sobol block
     // table of the widest book), plus (FP32 LSV) a ring of 2G leverage layers
@@ -673,7 +692,16 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     const size_t books_bytes = ( n_books * sizeof( DevBook ) + 255 ) & ~size_t( 255 );
     const size_t sums_bytes =
         ( ( B.part_total + B.grid_total + B.grid_out_total ) * sizeof( double ) + 255 ) & ~size_t( 255 );
-    const size_t total = arena_bytes + books_bytes + sums_bytes + tail_total;
+    // recorded paths [n][paths] per recording book, after the tail normals
+    std::vector<size_t> rec_off( n_books, 0 );
+    size_t rec_total = 0;
+    for ( int b = 0; B.recording && b < n_books; b++ )
+        if ( rec[b].n > 0 )
+        {
+            rec_off[b] = rec_total;
+            rec_total += ( (size_t)rec[b].n * (size_t)cfg->paths * elem + 255 ) & ~size_t( 255 );
+        }
+    const size_t total = arena_bytes + books_bytes + sums_bytes + tail_total + rec_total;
     if ( ext_base )
     {
         if ( total > ext_cap )
@@ -733,6 +761,18 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.cv_scale = At<float>( dbase, o.cv_scale );
         }
         d.partial = B.dpart + B.part_off[b];
+        d.rec_udl = -1;
+        if ( B.recording && rec[b].n > 0 )
+        {
+            char* r = dbase + arena_bytes + books_bytes + sums_bytes + tail_total + rec_off[b];
+            d.rec_udl = rec[b].udl;
+            d.rec_n = rec[b].n;
+            d.rec_stride = cfg->paths;
+            if ( cfg->precision == 0 )
+                d.rec = reinterpret_cast<double*>( r );
+            else
+                d.recf = reinterpret_cast<float*>( r );
+        }
         if ( B.prng )
         {
             d.pseudo = B.pseudo ? 1 : 0;

@@ -152,6 +152,12 @@ struct DevBook
     const float* tailf;
     const uint64_t* jump;       // [jump_levels][256][4]: (M^T)^(2^k), T the draws per path
     uint64_t seed_state[4];     // splitmix64 expansion of the seed (rng.hpp:36-47)
+    // American exercise: the spot path of equity rec_udl at dates 0 .. rec_n - 1
+    // (rec_n = 0: none), [rec_n][rec_stride] with the path (index - first) minor
+    int rec_udl, rec_n;
+    int64_t rec_stride;
+    double* rec;
+    float* recf;
 };
 
 }  // namespace thoth

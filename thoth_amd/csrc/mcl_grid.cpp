@@ -170,6 +170,10 @@ int PrepareGridPlan( int n, const thoth_mcl_request* reqs, const thoth_vanilla_g
     if ( n < 1 || !reqs || !grids )
         throw Invalid( THOTH_ERR_INVALID, "null argument" );
     const thoth_mcl_request& r0 = reqs[0];
+    for ( int r = 0; r < n; r++ )
+        for ( int c = 0; reqs[r].contracts && c < reqs[r].n_contracts; c++ )
+            if ( reqs[r].contracts[c].american )
+                throw Invalid( THOTH_ERR_UNSUPPORTED, "vanilla grids price european exercise" );
     P.n_req = n;
     P.paths = r0.mcl.paths;
     std::vector<BookArrays> arrays;

@@ -24,6 +24,14 @@
 #include "mcl_internal.hpp"
 #include "thoth_mcl.h"
 
+#define HIP_OK( expr )                                                                                      \
+    do                                                                                                      \
+    {                                                                                                       \
+        hipError_t e_ = ( expr );                                                                           \
+        if ( e_ != hipSuccess )                                                                             \
+            return SetError( THOTH_ERR_DEVICE, std::string( #expr " failed: " ) + hipGetErrorString( e_ ) ); \
+    } while ( 0 )
+
 namespace thoth::host
 {
 
@@ -315,6 +323,32 @@ struct Plan
     int n_contracts = 0, n_dates = 0;
     double calib_ms = 0;
     double tuned_ms = 0;  // best sweep time of the launch-shape tuner (0: untuned)
+    // American contracts (pricer_mcl_american.cpp:137-248): Longstaff-Schwartz
+    // on the equity path every scenario book records; per contract its column
+    // count M (dates 0 .. flow index), strike and type
+    struct AmericanLeg
+    {
+        int c, M, is_call;
+        double strike;
+    };
+    std::vector<AmericanLeg> am;
+    int rec_udl = -1;
+    std::vector<double> t;                   // year fraction of every date
+    std::vector<std::vector<double>> rates;  // per scenario: discount zero rate of every date
+    std::vector<double> spot0;               // per scenario: the recorded equity's spot today
+    // device scratch of the fit and the apply: beta [4 nd], df [nd], disc
+    // [scenarios][nd], cf [paths], part, recorded-path pointers [scenarios]
+    char* lsm_mem = nullptr;
+    double *dbeta = nullptr, *ddf = nullptr, *ddisc = nullptr, *dcf = nullptr, *dpart = nullptr;
+    const void** dpaths = nullptr;
+    Plan() = default;
+    Plan( const Plan& ) = delete;
+    Plan& operator=( const Plan& ) = delete;
+    ~Plan()
+    {
+        if ( lsm_mem )
+            (void)hipFree( lsm_mem );
+    }
 };
 
 namespace
@@ -410,6 +444,38 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
         Resolve( req, sc[k].mk, g, chol, levs[k], arrays[k] );
         books.push_back( arrays[k].book );
     }
+    // American contracts: vanilla on one equity per book, which every scenario
+    // book records at dates 0 .. the latest American flow (pricer_mcl_american.cpp:59-94)
+    plan.am.clear();
+    plan.rec_udl = -1;
+    int rec_n = 0;
+    for ( int c = 0; c < req.n_contracts; c++ )
+    {
+        const thoth_contract& k = req.contracts[c];
+        if ( !k.american )
+            continue;
+        if ( k.kind != THOTH_VANILLA || k.underlying < 0 )
+            throw Invalid( THOTH_ERR_UNSUPPORTED, "american exercise: vanilla contracts on an equity" );
+        if ( plan.rec_udl >= 0 && k.underlying != plan.rec_udl )
+            throw Invalid( THOTH_ERR_UNSUPPORTED, "american contracts of one book exercise on one equity" );
+        plan.rec_udl = k.underlying;
+        const int M = arrays[0].c_flow[c] + 1;
+        rec_n = std::max( rec_n, M );
+        plan.am.push_back( { c, M, k.is_call ? 1 : 0, arrays[0].c_strike[c] } );
+    }
+    std::vector<RecordSpec> rec( sc.size() );
+    if ( !plan.am.empty() )
+    {
+        plan.t = g.t;
+        plan.rates.assign( sc.size(), {} );
+        plan.spot0.assign( sc.size(), 0.0 );
+        for ( size_t k = 0; k < sc.size(); k++ )
+        {
+            rec[k] = { plan.rec_udl, rec_n };
+            plan.rates[k] = arrays[k].disc;
+            plan.spot0[k] = arrays[k].spot[plan.rec_udl];
+        }
+    }
     thoth_mcl_sweep_cfg cfg{};
     cfg.paths = req.mcl.paths;
     cfg.sobol_skip = req.mcl.sobol_skip;
@@ -418,9 +484,36 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     // Rng::Seed((uint64_t)seed) (pricer_mcl.cpp:227)
     cfg.pseudo_random = req.mcl.use_sobol ? 0 : 1;
     cfg.seed = (uint64_t)(int64_t)req.mcl.seed;
-    int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, plan.batch );
+    int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, plan.batch, nullptr, nullptr, false, nullptr, 0,
+                           plan.am.empty() ? nullptr : rec.data() );
     if ( rc != THOTH_OK )
         return rc;
+    if ( !plan.am.empty() )
+    {
+        const size_t K = sc.size(), nd = g.t.size(), N = (size_t)req.mcl.paths;
+        const size_t nblk = (size_t)lsm_blocks( req.mcl.paths );
+        const size_t part = std::max( nblk * 8, K * nblk * 2 );
+        const size_t doubles = 4 * nd + nd + K * nd + N + part;
+        const size_t bytes = doubles * sizeof( double ) + K * sizeof( void* );
+        hipError_t e = hipMalloc( &plan.lsm_mem, bytes );
+        if ( e != hipSuccess )
+            return SetError( THOTH_ERR_DEVICE, std::string( "hipMalloc: " ) + hipGetErrorString( e ) );
+        double* d = reinterpret_cast<double*>( plan.lsm_mem );
+        plan.dbeta = d;
+        plan.ddf = d + 4 * nd;
+        plan.ddisc = plan.ddf + nd;
+        plan.dcf = plan.ddisc + K * nd;
+        plan.dpart = plan.dcf + N;
+        plan.dpaths = reinterpret_cast<const void**>( plan.dpart + part );
+        std::vector<const void*> ptrs( K );
+        for ( size_t k = 0; k < K; k++ )
+            ptrs[k] = req.precision == THOTH_FP64 ? static_cast<const void*>( plan.batch.db[k].rec )
+                                                  : static_cast<const void*>( plan.batch.db[k].recf );
+        hipStream_t s = Stream();
+        if ( hipMemcpyAsync( plan.dpaths, ptrs.data(), K * sizeof( void* ), hipMemcpyHostToDevice, s ) != hipSuccess ||
+             hipStreamSynchronize( s ) != hipSuccess )
+            return SetError( THOTH_ERR_DEVICE, "upload of the recorded-path table" );
+    }
     plan.kind.clear();
     plan.h.clear();
     for ( const Scenario& s : sc )
@@ -434,6 +527,75 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     return THOTH_OK;
 }
 
+// Longstaff-Schwartz of every American contract (pricer_mcl_american.cpp:137-248):
+// the policy fitted on the base scenario's recorded paths, applied frozen to
+// every scenario's; its value replaces the contract's European value in the
+// scenario's contract and book premiums.  Returns the device milliseconds.
+int PriceAmerican( const Plan& plan, std::vector<double>& book, std::vector<std::vector<double>>& cprem,
+                   std::vector<double>& ctrust, float& ms )
+{
+    ms = 0;
+    hipStream_t s = Stream();
+    const DeviceBatch& B = plan.batch;
+    const int K = (int)plan.kind.size();
+    const int64_t N = B.paths;
+    const size_t nd = plan.t.size();
+    const int nblk = lsm_blocks( N );
+    const void* base_paths = B.precision == 0 ? static_cast<const void*>( B.db[0].rec )
+                                              : static_cast<const void*>( B.db[0].recf );
+    hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
+    HIP_OK( hipEventRecord( e0, s ) );
+    std::vector<double> part( (size_t)K * nblk * 2 );
+    for ( const Plan::AmericanLeg& leg : plan.am )
+    {
+        const int M = leg.M;
+        const std::vector<double>& t = plan.t;
+        const std::vector<double>& r0 = plan.rates[0];
+        // discount over [t, t+1] as the ratio of zero-coupon prices
+        // (pricer_mcl_american.cpp:307-312), and to today per scenario (:426-439)
+        std::vector<double> df( nd, 0.0 ), disc( (size_t)K * nd, 0.0 );
+        for ( int i = 0; i + 1 < M; i++ )
+            df[i] = std::exp( r0[i] * t[i] - r0[i + 1] * t[i + 1] );
+        for ( int k = 0; k < K; k++ )
+            for ( int i = 0; i < M; i++ )
+                disc[(size_t)k * M + i] = std::exp( -plan.rates[k][i] * ( t[i] - t[0] ) );
+        // basis normaliser: the strike (vanilla.hpp:38-41)
+        const double norm = leg.strike > 0 ? leg.strike : plan.spot0[0];
+        HIP_OK( hipMemcpyAsync( plan.ddf, df.data(), nd * sizeof( double ), hipMemcpyHostToDevice, s ) );
+        HIP_OK( hipMemcpyAsync( plan.ddisc, disc.data(), disc.size() * sizeof( double ), hipMemcpyHostToDevice, s ) );
+        HIP_OK( hipMemsetAsync( plan.dbeta, 0, 4 * nd * sizeof( double ), s ) );
+        if ( !launch_lsm_fit( s, B.precision, base_paths, N, M, leg.strike, leg.is_call, norm, plan.ddf, plan.dbeta,
+                              plan.dcf, plan.dpart ) ||
+             !launch_lsm_apply( s, B.precision, plan.dpaths, K, N, M, leg.strike, leg.is_call, norm, plan.ddisc,
+                                plan.dbeta, plan.dpart ) )
+            return SetError( THOTH_ERR_DEVICE, "Longstaff-Schwartz launch failed" );
+        HIP_OK( hipMemcpyAsync( part.data(), plan.dpart, part.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+        HIP_OK( hipStreamSynchronize( s ) );
+        for ( int k = 0; k < K; k++ )
+        {
+            double sum = 0, sum2 = 0;
+            for ( int b = 0; b < nblk; b++ )
+            {
+                sum += part[( (size_t)k * nblk + b ) * 2];
+                sum2 += part[( (size_t)k * nblk + b ) * 2 + 1];
+            }
+            const double mean = sum / (double)N;
+            const double trust = std::sqrt( ( sum2 / (double)N - mean * mean ) / (double)( N - 1 ) );
+            const double s0 = plan.spot0[k];
+            const double now = std::max( leg.is_call ? s0 - leg.strike : leg.strike - s0, 0.0 );
+            const double v = std::max( mean, now );
+            book[k] += v - cprem[k][leg.c];
+            cprem[k][leg.c] = v;
+            if ( k == 0 )
+                ctrust[leg.c] = trust;
+        }
+    }
+    HIP_OK( hipEventRecord( e1, s ) );
+    HIP_OK( hipEventSynchronize( e1 ) );
+    HIP_OK( hipEventElapsedTime( &ms, e0, e1 ) );
+    return THOTH_OK;
+}
+
 int RunPlan( const Plan& plan, thoth_mcl_result* res )
 {
     std::vector<thoth_mcl_book_out> out( plan.kind.size() );
@@ -442,40 +604,57 @@ int RunPlan( const Plan& plan, thoth_mcl_result* res )
         return rc;
     // Tree_Read (pricer_mcl.cpp:677-709) + ComputeGreeks finite differences
     const int nc = plan.n_contracts;
-    const thoth_mcl_book_out& o0 = out[0];
-    const double p0 = Mean( o0, -1 );
+    const size_t K = plan.kind.size();
+    std::vector<double> book( K );
+    std::vector<std::vector<double>> cprem( K, std::vector<double>( nc ) );
+    std::vector<double> ctrust( nc );
+    for ( size_t k = 0; k < K; k++ )
+    {
+        book[k] = Mean( out[k], -1 );
+        for ( int c = 0; c < nc; c++ )
+            cprem[k][c] = Mean( out[k], c );
+    }
+    for ( int c = 0; c < nc; c++ )
+        ctrust[c] = Trust( out[0], c );
+    float lsm_ms = 0;
+    double p0 = book[0];
+    if ( !plan.am.empty() )
+    {
+        if ( ( rc = PriceAmerican( plan, book, cprem, ctrust, lsm_ms ) ) != THOTH_OK )
+            return rc;
+        // the book premium re-aggregated from its contracts (pricer_mcl_american.cpp:251-257)
+        p0 = 0;
+        for ( int c = 0; c < nc; c++ )
+            p0 += cprem[0][c];
+    }
     res->premium = p0;
-    res->premium_trust = Trust( o0, -1 );
+    res->premium_trust = Trust( out[0], -1 );
     res->delta = res->gamma = res->vega = res->rho = 0;
     std::vector<double> cd( nc, 0 ), cg( nc, 0 ), cv( nc, 0 ), cr( nc, 0 );
-    for ( size_t k = 1; k < plan.kind.size(); k++ )
+    for ( size_t k = 1; k < K; k++ )
     {
-        const thoth_mcl_book_out& o = out[k];
         const double h = plan.h[k];
         switch ( plan.kind[k] )
         {
         case 1:
-            res->delta += ( Mean( o, -1 ) - p0 ) / h;
+            res->delta += ( book[k] - p0 ) / h;
             for ( int c = 0; c < nc; c++ )
-                cd[c] += ( Mean( o, c ) - Mean( o0, c ) ) / h;
+                cd[c] += ( cprem[k][c] - cprem[0][c] ) / h;
             break;
         case 2:
-        {
-            const thoth_mcl_book_out& od = out[k + 1];
-            res->gamma += ( Mean( o, -1 ) - 2 * p0 + Mean( od, -1 ) ) / ( h * h );
+            res->gamma += ( book[k] - 2 * p0 + book[k + 1] ) / ( h * h );
             for ( int c = 0; c < nc; c++ )
-                cg[c] += ( Mean( o, c ) - 2 * Mean( o0, c ) + Mean( od, c ) ) / ( h * h );
+                cg[c] += ( cprem[k][c] - 2 * cprem[0][c] + cprem[k + 1][c] ) / ( h * h );
             break;
-        }
         case 4:
-            res->vega = ( Mean( o, -1 ) - p0 ) / kVolBump * 0.01;
+            res->vega = ( book[k] - p0 ) / kVolBump * 0.01;
             for ( int c = 0; c < nc; c++ )
-                cv[c] = ( Mean( o, c ) - Mean( o0, c ) ) / kVolBump * 0.01;
+                cv[c] = ( cprem[k][c] - cprem[0][c] ) / kVolBump * 0.01;
             break;
         case 5:
-            res->rho = ( Mean( o, -1 ) - p0 ) / kRateBump * 0.01;
+            res->rho = ( book[k] - p0 ) / kRateBump * 0.01;
             for ( int c = 0; c < nc; c++ )
-                cr[c] = ( Mean( o, c ) - Mean( o0, c ) ) / kRateBump * 0.01;
+                cr[c] = ( cprem[k][c] - cprem[0][c] ) / kRateBump * 0.01;
             break;
         default:
             break;
@@ -484,9 +663,9 @@ int RunPlan( const Plan& plan, thoth_mcl_result* res )
     for ( int c = 0; c < nc; c++ )
     {
         if ( res->contract_premium )
-            res->contract_premium[c] = Mean( o0, c );
+            res->contract_premium[c] = cprem[0][c];
         if ( res->contract_trust )
-            res->contract_trust[c] = Trust( o0, c );
+            res->contract_trust[c] = ctrust[c];
         if ( res->contract_delta )
             res->contract_delta[c] = cd[c];
         if ( res->contract_gamma )
@@ -498,7 +677,7 @@ int RunPlan( const Plan& plan, thoth_mcl_result* res )
     }
     res->n_dates = plan.n_dates;
     res->n_scenarios = (int32_t)plan.kind.size();
-    res->sweep_ms = out[0].device_ms;
+    res->sweep_ms = out[0].device_ms + lsm_ms;
     res->calib_ms = plan.calib_ms;
     return THOTH_OK;
 }
@@ -507,9 +686,13 @@ int RunPlan( const Plan& plan, thoth_mcl_result* res )
 // base scenario only (no Greek bumps): grid, Cholesky, calibration, resolution
 int ResolveBaseBook( const thoth_mcl_request& req, BookArrays& A, Grid& g )
 {
-    // its callers (randomised QMC, the control variate) shift Sobol points
+    // its callers (randomised QMC, the control variate) shift Sobol points and
+    // price European payoffs
     if ( !req.mcl.use_sobol )
         throw Invalid( THOTH_ERR_UNSUPPORTED, "randomised QMC and the control variate need use_sobol: true" );
+    for ( int c = 0; req.contracts && c < req.n_contracts; c++ )
+        if ( req.contracts[c].american )
+            throw Invalid( THOTH_ERR_UNSUPPORTED, "randomised QMC and the control variate price european exercise" );
     if ( req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS || !req.assets )
         throw Invalid( THOTH_ERR_INVALID, "1..8 equities" );
     if ( req.n_contracts < 1 || req.n_contracts > THOTH_MCL_MAX_CONTRACTS || !req.contracts )
@@ -658,6 +841,8 @@ int PricePathwise( const thoth_mcl_request& req, thoth_pathwise_result* res )
 {
     if ( req.n_assets != 1 || !req.assets || req.n_contracts != 1 || !req.contracts )
         throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: one equity and one contract" );
+    if ( req.contracts[0].american )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: european exercise" );
     const thoth_asset& a0 = req.assets[0];
     if ( a0.vol_kind == THOTH_VOL_LSV )
         throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: Black-Scholes or Heston equities" );

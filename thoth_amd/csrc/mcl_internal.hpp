@@ -83,6 +83,7 @@ struct DeviceBatch
     // of the sweep), pseudo for use_sobol false; one fill of tail normals per
     // distinct T in Sobol mode (jump: device tables, dst: [T][paths])
     bool prng = false, pseudo = false;
+    bool recording = false;  // some book records a path for American exercise
     int jump_levels = 0;
     uint64_t seed_state[4] = { 0, 0, 0, 0 };
     struct TailFill
@@ -137,12 +138,19 @@ struct CvSums
 {
     double sum = 0, sum2 = 0;
 };
+// Path recording of one book for American exercise: the spot of equity udl at
+// dates 0 .. n - 1 (n = 0: none) into [n][paths] device memory of the batch
+// (DevBook::rec / recf)
+struct RecordSpec
+{
+    int udl = -1, n = 0;
+};
 // ext_base / ext_cap: device memory of the caller to hold the whole batch
 // (the no-allocation entry point's scratch) instead of a hipMalloc; a batch
 // that needs more than ext_cap bytes is refused before any device work
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
                   const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, bool pathwise = false,
-                  char* ext_base = nullptr, size_t ext_cap = 0 );
+                  char* ext_base = nullptr, size_t ext_cap = 0, const RecordSpec* rec = nullptr );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
 // Launch shape of a plan's batch (reference-mode sweeps without the control
 // variate): steps per LDS barrier G and workgroups per CU of the persistent

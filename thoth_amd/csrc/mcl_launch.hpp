@@ -99,6 +99,16 @@ bool launch_tail_fill_f64( hipStream_t s, const uint64_t* jump, int levels, cons
                            int64_t paths, double* out );
 bool launch_tail_fill_f32( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
                            int64_t paths, float* out );
+// Longstaff-Schwartz on recorded paths S [M][N] (precision 0: double, 1: float):
+// the fit writes beta[4 t + 0..3] (coefficients, 1 where fitted) for t = M-2 .. 1
+// (beta zeroed by the caller; cf [N], part [lsm_blocks(N) * 8] are scratch);
+// the apply writes part[(scenario * lsm_blocks(N) + block) * 2 + {0, 1}] sums of
+// the discounted value and its square, disc [scenarios][M]
+int lsm_blocks( int64_t N );
+bool launch_lsm_fit( hipStream_t s, int precision, const void* S, int64_t N, int M, double strike, int is_call,
+                     double norm, const double* df, double* beta, double* cf, double* part );
+bool launch_lsm_apply( hipStream_t s, int precision, const void* const* paths, int n_scen, int64_t N, int M,
+                       double strike, int is_call, double norm, const double* disc, const double* beta, double* part );
 bool launch_pathwise( int precision, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 bool launch_pathwise_f32( int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,

@@ -1044,6 +1044,26 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         __syncthreads();
     };
 
+    // path recording of an American contract's equity (path_recorder; the
+    // generic kernels only): its spot at dates 0 .. rec_n - 1, [date][path]
+    [[maybe_unused]] const size_t rrow = valid ? (size_t)( idx - first_idx ) : 0;
+    auto record = [&]( int i ) __attribute__( ( always_inline ) )
+    {
+        if constexpr ( !uniform )
+            if ( i < B.rec_n && valid )
+            {
+                R x = R( 0 );
+#pragma unroll
+                for ( int a = 0; a < NA; a++ )
+                    if ( a == B.rec_udl )
+                        x = S[a];
+                if constexpr ( F64 )
+                    B.rec[(size_t)i * B.rec_stride + rrow] = x;
+                else
+                    B.recf[(size_t)i * B.rec_stride + rrow] = x;
+            }
+    };
+    record( 0 );
     observe( 0 );
     if constexpr ( cv_on )
         cv_date( 0 );
@@ -1467,6 +1487,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 }
             }
         }
+        record( i );
         observe( i );
         if constexpr ( CV == 0 )
             if ( B.grid_nm > 0 && as_const( B.grid_mat_at )[i] >= 0 )

@@ -232,9 +232,18 @@ def translate(objs: Dict[str, dict], pricer: str, precision: int = abi.FP64, pat
         call = {"call": True, "put": False}[c["type"]]
         absolute = bool(c.get("is_absolute_strike", True))
         if c["__tag__"] == "vanilla":
-            if c.get("exercise", "european") != "european":
-                raise Unsupported("contract '%s' is %s" % (cname, c["exercise"]))
-            contracts.append(Vanilla(udl, days, float(c["strike"]), is_call=call, is_absolute_strike=absolute))
+            # helpers/enums.hpp:64-71: european or american (Longstaff-Schwartz on
+            # the recorded paths of the equity, tasks/pricer_mcl_american.cpp)
+            exercise = c.get("exercise", "european")
+            if exercise not in ("european", "american"):
+                raise ValueError("unknown exercise mode '%s'" % exercise)
+            american = exercise == "american"
+            if american and udl < 0:
+                raise Unsupported("american contract '%s' on a rainbow" % cname)
+            if american and any(k.american and k.underlying != udl for k in contracts if isinstance(k, Vanilla)):
+                raise Unsupported("american contracts on two equities in one book")
+            contracts.append(Vanilla(udl, days, float(c["strike"]), is_call=call, is_absolute_strike=absolute,
+                                     american=american))
         else:
             bt = _BARRIERS[c["barrier_type"]]
             level = float(c.get("barrier_up_level", 0) if bt in (abi.UP_OUT, abi.UP_IN)
