@@ -388,6 +388,11 @@ def vanilla_grids(paths, replays=5):
 LEGS = ("c3", "c3corr", "c3fp64", "c2", "c4", "c5", "t1bp", "cpu", "traffic")
 
 
+def progress(msg):
+    """One line per leg on stderr, so that a long run shows it is alive."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +427,7 @@ def main():
     legs = set(args.leg or LEGS)
     line = {}
     if "c3" in legs:
+        progress("c3 basket")
         c3 = basket_leg(args.paths, args.steps, args.warmup)
         line.update({
             "metric": "paths/s & time-to-1bp CI: LSV basket, learned-CV FP32 vs plain MC, 1 MI355X",
@@ -448,14 +454,19 @@ def main():
             "device": mcl.device_info(),
         })
     if "c3corr" in legs:
+        progress("c3 correlated")
         line["basket_correlated"] = basket_leg(args.paths, max(3, args.steps // 2), 1, correlated=True)
     if "c3fp64" in legs:
+        progress("c3 fp64")
         line["basket_fp64"] = basket_leg(args.paths, 3, 1, precision=abi.FP64)
     if "c4" in legs:
+        progress("c4 vanilla grids")
         line["vanilla_grids"] = vanilla_grids(args.grid_paths)
     if "c2" in legs:
+        progress("c2 heston barrier")
         line["heston_barrier"] = heston_barrier()
     if "c5" in legs or "t1bp" in legs:
+        progress("c5 learned control variate")
         cfg = mcl.cv_config(adam_steps=args.adam_steps)
         w, rep, c5 = learned_cv(cfg)
         line["learned_cv"] = c5
@@ -464,6 +475,7 @@ def main():
         if "t1bp" in legs:
             line["time_to_1bp"] = time_to_1bp(cfg, w, rep.train_ms + rep.data_ms)
     if "traffic" in legs and "roofline" in line:
+        progress("hbm traffic")
         traffic, detail = hbm_traffic(args.paths)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_detail"] = detail
@@ -471,6 +483,7 @@ def main():
             line["roofline"]["hbm_gbs"] = traffic / (line["kernel_ms"] * 1e-3) / 1e9
             line["roofline"]["hbm_frac"] = line["roofline"]["hbm_gbs"] / HBM_PEAK_GBS
     if "cpu" in legs:
+        progress("cpu baselines")
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample)
         if args.cpu_all_cores_paths:
             line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(args.cpu_all_cores_paths)
