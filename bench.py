@@ -63,17 +63,29 @@ def shape_of(sh, tune_s):
             "default_shape_ms": sh.default_ms, "candidates": sh.candidates, "tune_s": tune_s}
 
 
-def timed_plan(built, steps, warmup, tune=True):
+# --shape G:W forces the plans' launch shape instead of tuning it (profiling
+# runs, whose kernel statistics would otherwise include the tuner's candidates)
+FORCED_SHAPE = None
+
+
+def apply_shape(plan):
+    """Tune the plan's launch shape (or force FORCED_SHAPE); returns its description."""
+    t0 = time.perf_counter()
+    if FORCED_SHAPE:
+        plan.set_shape(*FORCED_SHAPE)
+        sh = plan.shape()
+    else:
+        sh = plan.tune()
+    return shape_of(sh, time.perf_counter() - t0)
+
+
+def timed_plan(built, steps, warmup):
     """Sweeps of a prepared plan, its launch shape picked by the tuner first:
     (setup s, host-clock s per sweep list, device ms list, last result, shape)."""
     t0 = time.perf_counter()
     plan = mcl.Plan(built)
     setup = time.perf_counter() - t0
-    shape = None
-    if tune:
-        t0 = time.perf_counter()
-        sh = plan.tune()
-        shape = shape_of(sh, time.perf_counter() - t0)
+    shape = apply_shape(plan)
     for _ in range(warmup):
         plan.run()
     wall, dev = [], []
@@ -334,8 +346,7 @@ def heston_barrier(paths=1 << 20, steps=252, repeats=3):
         spec = configs.heston_down_out_call(paths=paths, precision=prec, uniform_steps=steps, greeks=greeks)
         b = BuiltRequest(spec)
         plan = mcl.Plan(b)
-        t0 = time.perf_counter()
-        shape = shape_of(plan.tune(), time.perf_counter() - t0)
+        shape = apply_shape(plan)
         plan.run()
         ms = [plan.run().sweep_ms for _ in range(repeats)]
         r = plan.run()
@@ -356,6 +367,54 @@ def heston_barrier(paths=1 << 20, steps=252, repeats=3):
         kind: {k: res["fp32"][kind][k] - res["fp64"][kind][k] for k in ("premium", "delta", "gamma", "vega")}
         for kind in ("bump", "pathwise")}
     out["fp64_ms"] = {kind: res["fp64"][kind]["ms"] for kind in ("bump", "pathwise")}
+    return out
+
+
+def american_leg(repeats=3):
+    """The American put of the reference's tests (S = K = 100, r 8 %, vol 30 %,
+    1y in 30-day steps, pseudo-random draws, 300,000 paths) with delta, gamma,
+    vega and rho: the sweep records the equity path of each of the 6 scenario
+    books, then the Longstaff-Schwartz fit and the frozen-policy apply run on
+    the device.  sweep_ms is the device time of both."""
+    from thoth_amd.book import BookSpec, Equity, McConfig, Vanilla
+    greeks = abi.GREEK_DELTA | abi.GREEK_GAMMA | abi.GREEK_VEGA | abi.GREEK_RHO
+    out = {"workload": "american_put_lsm", "paths": 300000, "exercise_dates": None}
+    for prec, name in ((abi.FP32, "fp32"), (abi.FP64, "fp64")):
+        spec = BookSpec(mcl=McConfig(paths=300000, max_day_step=30, min_day_step=-1, vol_year_step=0.01,
+                                     use_sobol=False),
+                        equities=[Equity(spot=100.0, vol_kind=abi.VOL_BS, volatility=30.0)],
+                        contracts=[Vanilla(0, 365, 100.0, is_call=False, american=True)],
+                        rate_days=[0], rate_values=[8.0], greeks=greeks, precision=prec)
+        plan = mcl.Plan(BuiltRequest(spec))
+        plan.run()
+        walls, rs = [], []
+        for _ in range(repeats):
+            t = time.perf_counter()
+            rs.append(plan.run())
+            walls.append(time.perf_counter() - t)
+        r = rs[-1]
+        out["exercise_dates"] = r.n_dates
+        out[name] = {"premium": r.premium, "trust": r.contract_trust[0], "delta": r.delta, "gamma": r.gamma,
+                     "vega": r.vega, "rho": r.rho, "device_ms": min(x.sweep_ms for x in rs),
+                     "wall_ms": 1e3 * min(walls), "books": r.n_scenarios}
+    return out
+
+
+def prng_leg(paths=1 << 20):
+    """Pseudo-random draws: the 8-equity LSV worst-of on 365 daily steps (16
+    factors, 5,840 dimensions: 1,743 per path past the Sobol table, drawn
+    from the device xoshiro stream), and the 5-asset basket with use_sobol
+    false (every normal from the stream, no bridge), FP32, device ms per sweep."""
+    out = {}
+    spec = configs.lsv_worst_of_basket(paths=paths, n_assets=8, maturity_days=365, max_day_step=1)
+    _, _, dev, res, shape = timed_plan(BuiltRequest(spec), 3, 1)
+    out["lsv8_365_sobol_tail"] = {"paths": paths, "dimensions": 5840, "tail_dimensions": 1743,
+                                  "device_ms": min(dev), "premium": res.premium, "trust": res.premium_trust}
+    spec = configs.lsv_worst_of_basket(paths=paths)
+    spec.mcl.use_sobol = False
+    _, _, dev, res, shape = timed_plan(BuiltRequest(spec), 3, 1)
+    out["basket_use_sobol_false"] = {"paths": paths, "device_ms": min(dev), "premium": res.premium,
+                                     "trust": res.premium_trust}
     return out
 
 
@@ -385,7 +444,7 @@ def vanilla_grids(paths, replays=5):
             "setup_s": setup}
 
 
-LEGS = ("c3", "c3corr", "c3fp64", "c2", "c4", "c5", "t1bp", "cpu", "traffic")
+LEGS = ("c3", "c3corr", "c3fp64", "c2", "c4", "c5", "t1bp", "american", "prng", "cpu", "traffic")
 
 
 def progress(msg):
@@ -408,9 +467,15 @@ def main():
     ap.add_argument("--no-gated", action="store_true", help="skip the smoothstep-gated network variant")
     ap.add_argument("--adam-steps", type=int, default=2000,
                     help="Adam steps of the hedge-network training (2000 as specified; fewer only for profiling)")
+    ap.add_argument("--shape", default=None, metavar="G:W",
+                    help="force the plans' launch shape (steps per LDS barrier G, persistent workgroups per CU W, "
+                         "0 = one workgroup per Sobol block) instead of tuning it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.shape:
+        global FORCED_SHAPE
+        FORCED_SHAPE = tuple(int(x) for x in args.shape.split(":"))
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -474,6 +539,12 @@ def main():
             line["learned_cv_gated"] = learned_cv(mcl.cv_config_gated(adam_steps=args.adam_steps))[2]
         if "t1bp" in legs:
             line["time_to_1bp"] = time_to_1bp(cfg, w, rep.train_ms + rep.data_ms)
+    if "american" in legs:
+        progress("american put")
+        line["american"] = american_leg()
+    if "prng" in legs:
+        progress("pseudo-random draws")
+        line["pseudo_random"] = prng_leg(args.paths)
     if "traffic" in legs and "roofline" in line:
         progress("hbm traffic")
         traffic, detail = hbm_traffic(args.paths)

@@ -653,7 +653,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     // jump tables per distinct T (arena) and, in Sobol mode, tail normals
     // [T][paths] per distinct T (a region of their own after the sums)
     B.fills.clear();
-    std::vector<size_t> jump_off( n_books, 0 ), tail_off( n_books, 0 );
+    std::vector<size_t> jump_off( n_books, 0 ), tail_off( n_books, 0 ), state_off( n_books, 0 );
     size_t tail_total = 0;
     const size_t elem = cfg->precision == 0 ? sizeof( double ) : sizeof( float );
     if ( B.prng )
@@ -670,18 +670,20 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             {
                 jump_off[b] = jump_off[same];
                 tail_off[b] = tail_off[same];
+                state_off[b] = state_off[same];
                 continue;
             }
             if ( draws[b] == 0 )
                 continue;
             const std::vector<uint64_t>& tab = JumpTables( (uint64_t)draws[b], B.jump_levels );
             jump_off[b] = ar.put( tab.data(), (size_t)B.jump_levels * kJumpWords );
+            // start states [4][paths], then (Sobol mode) the tail normals
+            state_off[b] = tail_total;
+            tail_total += ( 4 * sizeof( uint64_t ) * (size_t)cfg->paths + 255 ) & ~size_t( 255 );
+            tail_off[b] = tail_total;
             if ( !B.pseudo )
-            {
-                tail_off[b] = tail_total;
                 tail_total += ( (size_t)draws[b] * (size_t)cfg->paths * elem + 255 ) & ~size_t( 255 );
-                B.fills.push_back( { draws[b], b, nullptr, nullptr } );
-            }
+            B.fills.push_back( { draws[b], b, nullptr, nullptr, nullptr } );
         }
         if ( tail_total > ( (size_t)128 << 30 ) )
             return SetError( THOTH_ERR_UNSUPPORTED, "the normals past the Sobol table would take " +
@@ -778,6 +780,9 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.pseudo = B.pseudo ? 1 : 0;
             d.jump_levels = B.jump_levels;
             d.jump = draws[b] > 0 ? At<uint64_t>( dbase, jump_off[b] ) : nullptr;
+            d.states = draws[b] > 0 ? reinterpret_cast<const uint64_t*>( dbase + arena_bytes + books_bytes +
+                                                                         sums_bytes + state_off[b] )
+                                    : nullptr;
             d.tail_dims = B.pseudo ? 0 : (int)draws[b];
             d.tail_stride = cfg->paths;
             char* tail = dbase + arena_bytes + books_bytes + sums_bytes + tail_off[b];
@@ -802,7 +807,8 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     for ( DeviceBatch::TailFill& fl : B.fills )
     {
         fl.jump = B.db[fl.book].jump;
-        fl.dst = dbase + arena_bytes + books_bytes + sums_bytes + tail_off[fl.book];
+        fl.states = const_cast<uint64_t*>( B.db[fl.book].states );
+        fl.dst = B.pseudo ? nullptr : dbase + arena_bytes + books_bytes + sums_bytes + tail_off[fl.book];
     }
     B.dbooks = reinterpret_cast<const DevBook*>( dbase + arena_bytes );
     B.dgrid_table = At<long long>( dbase, gtab_off );
@@ -829,13 +835,13 @@ static int EnqueueSweep( const DeviceBatch& B, hipStream_t s )
     const int kcv = B.cv_mode == 2 && B.cv_act == THOTH_CV_GATE ? 3 : B.cv_mode;
     for ( const DeviceBatch::TailFill& fl : B.fills )
     {
-        const bool filled = B.precision == 0
-                                ? launch_tail_fill_f64( s, fl.jump, B.jump_levels, B.seed_state, (int)fl.T, B.paths,
-                                                        reinterpret_cast<double*>( fl.dst ) )
-                                : launch_tail_fill_f32( s, fl.jump, B.jump_levels, B.seed_state, (int)fl.T, B.paths,
-                                                        reinterpret_cast<float*>( fl.dst ) );
-        if ( !filled )
-            return SetError( THOTH_ERR_DEVICE, "tail normal fill launch failed" );
+        bool ok = launch_prng_start_states( s, fl.jump, B.jump_levels, B.seed_state, (int)fl.T, B.paths, fl.states );
+        if ( ok && fl.dst )
+            ok = B.precision == 0
+                     ? launch_tail_fill_f64( s, fl.states, (int)fl.T, B.paths, reinterpret_cast<double*>( fl.dst ) )
+                     : launch_tail_fill_f32( s, fl.states, (int)fl.T, B.paths, reinterpret_cast<float*>( fl.dst ) );
+        if ( !ok )
+            return SetError( THOTH_ERR_DEVICE, "pseudo-random stream set-up launch failed" );
     }
     bool ok;
     if ( B.persist_wpc > 0 )

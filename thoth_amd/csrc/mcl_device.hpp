@@ -151,6 +151,7 @@ struct DevBook
     const double* tail;
     const float* tailf;
     const uint64_t* jump;       // [jump_levels][256][4]: (M^T)^(2^k), T the draws per path
+    const uint64_t* states;     // [4][tail_stride]: the stream state at the start of each path
     uint64_t seed_state[4];     // splitmix64 expansion of the seed (rng.hpp:36-47)
     // American exercise: the spot path of equity rec_udl at dates 0 .. rec_n - 1
     // (rec_n = 0: none), [rec_n][rec_stride] with the path (index - first) minor

@@ -80,8 +80,9 @@ struct DeviceBatch
     int nf_inst = 2;
     int64_t paths = 0;
     // pseudo-random draws: prng when any book draws from the stream (kind 4
-    // of the sweep), pseudo for use_sobol false; one fill of tail normals per
-    // distinct T in Sobol mode (jump: device tables, dst: [T][paths])
+    // of the sweep), pseudo for use_sobol false; per distinct T the start
+    // states of the paths (jump: device tables, states: [4][paths]) and, in
+    // Sobol mode, the tail normals drawn from them (dst: [T][paths])
     bool prng = false, pseudo = false;
     bool recording = false;  // some book records a path for American exercise
     int jump_levels = 0;
@@ -91,7 +92,8 @@ struct DeviceBatch
         int64_t T;
         int book;
         const uint64_t* jump;
-        char* dst;
+        uint64_t* states;
+        char* dst;  // nullptr for use_sobol false (no tail)
     };
     std::vector<TailFill> fills;
     // launch shape: one workgroup per Sobol block of every book (persist_wpc

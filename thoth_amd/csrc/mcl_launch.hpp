@@ -95,10 +95,12 @@ bool launch_sweep_prng_f64( int na, int depth, dim3 grid, hipStream_t s, size_t 
                             const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 bool launch_sweep_prng_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                             const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
-bool launch_tail_fill_f64( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
-                           int64_t paths, double* out );
-bool launch_tail_fill_f32( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
-                           int64_t paths, float* out );
+// the stream state at the start of every path, [4][paths] (jump tables of T,
+// paths <= 2^levels), then the tail normals [T][paths] drawn from those states
+bool launch_prng_start_states( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
+                               int64_t paths, uint64_t* states );
+bool launch_tail_fill_f64( hipStream_t s, const uint64_t* states, int T, int64_t paths, double* out );
+bool launch_tail_fill_f32( hipStream_t s, const uint64_t* states, int T, int64_t paths, float* out );
 // Longstaff-Schwartz on recorded paths S [M][N] (precision 0: double, 1: float):
 // the fit writes beta[4 t + 0..3] (coefficients, 1 where fitted) for t = M-2 .. 1
 // (beta zeroed by the caller; cf [N], part [lsm_blocks(N) * 8] are scratch);

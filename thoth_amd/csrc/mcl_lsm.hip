@@ -5,7 +5,7 @@
 // per date does, per path, the exercise decision of date t+1 (fitted in the
 // previous launch), the discounting of the path's cashflow to t and the eight
 // sums of the {1, m, m^2} regression over the in-the-money paths of t (FP64,
-// per-block partials); a one-block kernel adds the partials in block order,
+// per-block partials); a one-block kernel adds the partials in a fixed order,
 // factors the 3 x 3 normal matrix by Cholesky and stores the date's
 // coefficients.  Apply: the frozen policy walks every path of every scenario
 // forward to its first exercise; per-block sums of the discounted value and
@@ -103,17 +103,28 @@ lsm_step_kernel( const R* __restrict__ S, int64_t N, int M, int t, double strike
     }
 }
 
-// the date's regression: partials in block order, the normal equations by
+// the date's regression: partials in a fixed order, the normal equations by
 // Cholesky (maths.cpp:354-381, 441-469), at least 50 in-the-money paths
 __global__ void __launch_bounds__( 64 ) lsm_solve_kernel( const double* __restrict__ part, int nblk, int t,
                                                            double* __restrict__ beta )
 {
+    // thread j adds the blocks b = j / 8 (mod 8) of moment j % 8 (independent
+    // loads in flight), then the eight sub-sums of each moment in order
+    __shared__ double sub[8][kLsmSums];
     __shared__ double sum[kLsmSums];
+    {
+        const int k = threadIdx.x & 7, r = threadIdx.x >> 3;
+        double v = 0.0;
+        for ( int b = r; b < nblk; b += 8 )
+            v += part[(size_t)b * kLsmSums + k];
+        sub[r][k] = v;
+    }
+    __syncthreads();
     if ( threadIdx.x < kLsmSums )
     {
         double v = 0.0;
-        for ( int b = 0; b < nblk; b++ )
-            v += part[(size_t)b * kLsmSums + threadIdx.x];
+        for ( int r = 0; r < 8; r++ )
+            v += sub[r][threadIdx.x];
         sum[threadIdx.x] = v;
     }
     __syncthreads();

@@ -20,9 +20,8 @@ bool launch_sweep_prng_f32( int na, int depth, dim3 grid, hipStream_t s, size_t 
     THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, lds, books, V, bq0, first, last );
 }
 
-bool launch_tail_fill_f32( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
-                           int64_t paths, float* out )
+bool launch_tail_fill_f32( hipStream_t s, const uint64_t* states, int T, int64_t paths, float* out )
 {
-    return launch_tail_fill<float>( s, jump, levels, seed_state, T, paths, out );
+    return launch_tail_fill<float>( s, states, T, paths, out );
 }
 }  // namespace thoth

@@ -21,9 +21,14 @@ bool launch_sweep_prng_f64( int na, int depth, dim3 grid, hipStream_t s, size_t 
     THOTH_DISPATCH_SWEEP( launch_one, na, depth, grid, s, lds, books, V, bq0, first, last );
 }
 
-bool launch_tail_fill_f64( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
-                           int64_t paths, double* out )
+bool launch_tail_fill_f64( hipStream_t s, const uint64_t* states, int T, int64_t paths, double* out )
 {
-    return launch_tail_fill<double>( s, jump, levels, seed_state, T, paths, out );
+    return launch_tail_fill<double>( s, states, T, paths, out );
+}
+
+bool launch_prng_start_states( hipStream_t s, const uint64_t* jump, int levels, const uint64_t* seed_state, int T,
+                               int64_t paths, uint64_t* states )
+{
+    return launch_prng_states( s, jump, levels, seed_state, T, paths, states );
 }
 }  // namespace thoth

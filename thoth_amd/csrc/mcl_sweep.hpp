@@ -518,7 +518,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     const uint32_t* sh = shifts ? shifts + (size_t)z * kSobolDimsDev : nullptr;
 
     // pseudo-random draws: the lane's path (index - first) is its position in
-    // the book's one stream of draws; invalid lanes read path 0
+    // the book's one stream of draws, whose start state the batch computed
+    // once for every book of the same T; invalid lanes read path 0
     const bool pseudo = prng && B.pseudo != 0;
     [[maybe_unused]] const size_t prow = valid ? (size_t)( idx - first_idx ) : 0;
     [[maybe_unused]] Xoshiro256 prs;
@@ -527,8 +528,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
         {
 #pragma unroll
             for ( int q = 0; q < 4; q++ )
-                prs.s[q] = B.seed_state[q];
-            xoshiro_jump( prs, B.jump, B.jump_levels, prow );
+                prs.s[q] = B.states[(size_t)q * B.tail_stride + prow];
         }
 
     const int F = B.factors;
