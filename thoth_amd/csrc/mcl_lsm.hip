@@ -105,17 +105,18 @@ lsm_step_kernel( const R* __restrict__ S, int64_t N, int M, int t, double strike
 
 // the date's regression: partials in a fixed order, the normal equations by
 // Cholesky (maths.cpp:354-381, 441-469), at least 50 in-the-money paths
-__global__ void __launch_bounds__( 64 ) lsm_solve_kernel( const double* __restrict__ part, int nblk, int t,
-                                                           double* __restrict__ beta )
+__global__ void __launch_bounds__( kLsmThreads ) lsm_solve_kernel( const double* __restrict__ part, int nblk, int t,
+                                                                    double* __restrict__ beta )
 {
-    // thread j adds the blocks b = j / 8 (mod 8) of moment j % 8 (independent
-    // loads in flight), then the eight sub-sums of each moment in order
-    __shared__ double sub[8][kLsmSums];
+    // thread j adds the blocks b = j / 8 (mod 32) of moment j % 8 (independent
+    // loads in flight), then the 32 sub-sums of each moment in order
+    constexpr int kRows = kLsmThreads / kLsmSums;
+    __shared__ double sub[kRows][kLsmSums];
     __shared__ double sum[kLsmSums];
     {
-        const int k = threadIdx.x & 7, r = threadIdx.x >> 3;
+        const int k = threadIdx.x % kLsmSums, r = threadIdx.x / kLsmSums;
         double v = 0.0;
-        for ( int b = r; b < nblk; b += 8 )
+        for ( int b = r; b < nblk; b += kRows )
             v += part[(size_t)b * kLsmSums + k];
         sub[r][k] = v;
     }
@@ -123,7 +124,7 @@ __global__ void __launch_bounds__( 64 ) lsm_solve_kernel( const double* __restri
     if ( threadIdx.x < kLsmSums )
     {
         double v = 0.0;
-        for ( int r = 0; r < 8; r++ )
+        for ( int r = 0; r < kRows; r++ )
             v += sub[r][threadIdx.x];
         sum[threadIdx.x] = v;
     }
@@ -233,7 +234,7 @@ lsm_apply_kernel( const R* const* __restrict__ paths, int64_t N, int M, double s
 int lsm_blocks( int64_t N )
 {
     const int64_t b = ( N + kLsmThreads - 1 ) / kLsmThreads;
-    return (int)( b < 1024 ? b : 1024 );
+    return (int)( b < 256 ? b : 256 );  // each thread then takes a few paths; few partials to add
 }
 
 bool launch_lsm_fit( hipStream_t s, int precision, const void* S, int64_t N, int M, double strike, int is_call,
@@ -250,7 +251,7 @@ bool launch_lsm_fit( hipStream_t s, int precision, const void* S, int64_t N, int
         else
             hipLaunchKernelGGL( lsm_step_kernel<float>, dim3( nblk ), dim3( kLsmThreads ), 0, s,
                                 static_cast<const float*>( S ), N, M, t, strike, is_call, norm, df, beta, cf, part );
-        hipLaunchKernelGGL( lsm_solve_kernel, dim3( 1 ), dim3( 64 ), 0, s, part, nblk, t, beta );
+        hipLaunchKernelGGL( lsm_solve_kernel, dim3( 1 ), dim3( kLsmThreads ), 0, s, part, nblk, t, beta );
     }
     return hipGetLastError() == hipSuccess;
 }
