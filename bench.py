@@ -434,12 +434,16 @@ def vanilla_grids(paths, replays=5):
     n_books = 3 * len(books)
     # per book-path: 365 LSV steps (one equity: no Cholesky) + 12 maturities x 82 cells
     flops = n_books * paths * (365 * LSV_STEP + 12 * GRID_MATURITY)
+    # K books to a lane: the K equities of the grouped kernel (kind 5)
+    k = t.books_per_lane
+    kernel = "sweep_persistent<float, %d, 8, 5>" % k if k > 1 else "sweep_persistent<float, 1, 8, 1>"
     return {"workload": "lsv_vanilla_grids", "underlyings": len(books), "books": n_books,
             "cells_per_underlying": 2 * len(configs.GRID_STRIKES) * len(configs.GRID_MATURITIES),
             "paths_per_book": paths, "replays": replays, "device_ms": dev,
             "book_paths_per_s": n_books * paths / (dev * 1e-3),
-            "roofline": roofline(flops, dev, FP32_PEAK_TFLOPS, "sweep_persistent<float, 1, 8, 1>"),
-            "autotuned": {"workgroups_per_cu": t.blocks_per_cu, "leverage_layers_per_barrier": t.lev_group,
+            "roofline": roofline(flops, dev, FP32_PEAK_TFLOPS, kernel),
+            "autotuned": {"books_per_lane": k, "workgroups_per_cu": t.blocks_per_cu,
+                          "leverage_layers_per_barrier": t.lev_group,
                           "candidates": t.candidates, "best_ms": t.device_ms},
             "setup_s": setup}
 

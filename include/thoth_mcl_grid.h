@@ -55,6 +55,7 @@ typedef struct thoth_grid_tuning {
     int32_t lev_group;
     double device_ms;     /* best time measured while tuning */
     int32_t candidates;   /* configurations timed */
+    int32_t books_per_lane; /* books stepped together by one lane on shared normals (1: none) */
 } thoth_grid_tuning;
 
 typedef struct thoth_grid_plan_s* thoth_grid_plan;
@@ -64,6 +65,16 @@ typedef struct thoth_grid_plan_s* thoth_grid_plan;
  * device packing, autotuning (autotune != 0) and the hipGraph. */
 int thoth_grid_prepare(int32_t n, const thoth_mcl_request* reqs, const thoth_vanilla_grid* grids, int32_t autotune,
                        thoth_grid_plan* plan);
+/* No reference counterpart: thoth_grid_prepare with flags.  When every book
+ * (request x bump scenario) is a single-equity LSV book on the same grid
+ * (FP32), the plan steps 5 (else 2) of them per lane as the equities of one
+ * device book: they read the same Sobol points, so one set of normals and one
+ * Brownian bridge serve all of them and each book's results are those of its
+ * own sweep.  THOTH_GRID_SEPARATE_BOOKS keeps one device book per book. */
+#define THOTH_GRID_AUTOTUNE 1
+#define THOTH_GRID_SEPARATE_BOOKS 2
+int thoth_grid_prepare_ex(int32_t n, const thoth_mcl_request* reqs, const thoth_vanilla_grid* grids, int32_t flags,
+                          thoth_grid_plan* plan);
 /* Replay the graph; results[n]; device_ms: the replay's device time. */
 int thoth_grid_run(thoth_grid_plan plan, thoth_grid_result* results, double* device_ms);
 int thoth_grid_get_tuning(thoth_grid_plan plan, thoth_grid_tuning* tuning);

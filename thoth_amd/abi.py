@@ -163,8 +163,12 @@ class GridResult(C.Structure):
                 ("put_delta", _pd), ("call_vega", _pd), ("put_vega", _pd)]
 
 
+GRID_AUTOTUNE, GRID_SEPARATE_BOOKS = 1, 2
+
+
 class GridTuning(C.Structure):
-    _fields_ = [("blocks_per_cu", _i32), ("lev_group", _i32), ("device_ms", _d), ("candidates", _i32)]
+    _fields_ = [("blocks_per_cu", _i32), ("lev_group", _i32), ("device_ms", _d), ("candidates", _i32),
+                ("books_per_lane", _i32)]
 
 
 class PathwiseResult(C.Structure):

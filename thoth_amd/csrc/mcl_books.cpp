@@ -356,10 +356,27 @@ int PackGrid( int bi, const thoth_mcl_book& in, const GridSpec& gs, DevBook& db,
         at[d] = j;
     }
     std::vector<double> kd( gs.strikes, gs.strikes + gs.nk );
+    if ( gs.grouped )
+    {
+        // every equity a single-equity LSV book of its own: two shared factors
+        // (spot and variance noise) instead of two per equity
+        for ( int a = 0; a < in.n_assets; a++ )
+            if ( in.model[a] != THOTH_MODEL_LSV )
+                return SetError( THOTH_ERR_INVALID, Why( bi, "grouped grid books are LSV equities" ) );
+        if ( in.correlate_stochastic )
+            return SetError( THOTH_ERR_INVALID, Why( bi, "grouped grid books are uncorrelated" ) );
+        db.factors = 2;
+        for ( int a = 0; a < in.n_assets; a++ )
+            db.asset[a].var_factor = 1;
+        kd.clear();
+        for ( int a = 0; a < in.n_assets; a++ )
+            kd.insert( kd.end(), gs.strikes_by_equity[a], gs.strikes_by_equity[a] + gs.nk );
+    }
     std::vector<float> kf( kd.begin(), kd.end() );
     db.grid_nm = gs.nm;
     db.grid_nk = gs.nk;
     db.grid_udl = gs.udl;
+    db.grid_grouped = gs.grouped ? 1 : 0;
     of.grid_mat_at = ar.put( at );
     of.grid_kf = ar.put( kf );
     of.grid_kd = ar.put( kd );
@@ -513,6 +530,22 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         }
         B.kind = std::max( kind, 0 );
     }
+    // grouped grid books (every book of the batch, or none): the uniform LSV
+    // arithmetic on the two shared factors
+    bool grouped = false;
+    for ( int b = 0; grids && b < n_books; b++ )
+        grouped |= B.db[b].grid_grouped != 0;
+    if ( grouped )
+    {
+        for ( int b = 0; b < n_books; b++ )
+            if ( !B.db[b].grid_grouped )
+                return SetError( THOTH_ERR_INVALID, "grouped and single grid books in one batch" );
+        if ( cfg->precision != 1 || B.kind != 1 )
+            return SetError( THOTH_ERR_UNSUPPORTED, "grouped grid books: FP32 LSV books with vol-of-vol > 0" );
+        if ( B.depth > 10 )
+            return SetError( THOTH_ERR_UNSUPPORTED, "grouped grid books: bridge stack deeper than 10" );
+        B.kind = 5;
+    }
     // pseudo-random draws: every normal (use_sobol false), or the dimensions
     // of a book past the Sobol table (path_generator.cpp:140-157).  One stream
     // per book, seeded alike: books with the same draws per path T share the
@@ -536,6 +569,8 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             return SetError( THOTH_ERR_UNSUPPORTED,
                              std::string( why ) + " needs every normal from the Sobol table (use_sobol true, "
                                                   "steps x factors <= 4097)" );
+        if ( grouped )
+            return SetError( THOTH_ERR_UNSUPPORTED, "grouped grid books need every normal from the Sobol table" );
         B.kind = 0;
         if ( B.pseudo )
             B.depth = 1;  // no bridge
@@ -578,7 +613,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     B.lds_lev_words = levw;
     // tail areas, 16-byte aligned after the leverage ring (sweep_block's layout)
     B.lds_tail = B.cv_mode == 2 ? (size_t)kCvLdsBytes : any_grid ? (size_t)grid_lds_bytes( cfg->precision == 0 ? 8 : 4 ) : 0;
-    B.nf_inst = 2 * na_inst;
+    B.nf_inst = B.kind == 5 ? 2 : 2 * na_inst;  // grouped books: two shared factors
     int lev_group = 4;
     while ( lev_group > 1 && 2 * ( LdsBytes( B, lev_group ) + kSweepStaticLds ) > (size_t)160 * 1024 )
         lev_group /= 2;
@@ -599,7 +634,7 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
     for ( int b = 0; b < n_books; b++ )
     {
         const DevBook& d = B.db[b];
-        B.grid_cells[b] = d.grid_nm * 2 * d.grid_nk;
+        B.grid_cells[b] = d.grid_nm * 2 * d.grid_nk * ( d.grid_grouped ? d.n_assets : 1 );
         B.grid_off[b] = B.grid_total;
         B.grid_out_off[b] = B.grid_out_total;
         B.grid_total += (size_t)B.replicates * B.nblocks * B.grid_cells[b] * 2;
@@ -872,8 +907,9 @@ static int EnqueueSweep( const DeviceBatch& B, hipStream_t s )
                  : ( B.kind && launch_sweep_f32_uniform( B.kind, B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
                                                          DeviceDirections(), B.q0, B.first, B.last, B.dshifts,
                                                          kcv ) ) ||
-                       launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
-                                         B.first, B.last, B.dshifts, kcv );
+                       ( B.kind != 5 &&  // grouped grid books have no generic form
+                         launch_sweep_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks, DeviceDirections(), B.q0,
+                                         B.first, B.last, B.dshifts, kcv ) );
     }
     if ( !ok )
         return SetError( THOTH_ERR_UNSUPPORTED, "no kernel instantiation for this book shape" );

@@ -135,6 +135,7 @@ struct DevBook
     // vanilla grid (grid_nm = 0: none): calls and puts on one underlying,
     // summed per (replicate, Sobol block, maturity, cell) into grid_part
     int grid_nm, grid_nk, grid_udl;
+    int grid_grouped;           // one grid per equity (sweep kind 5), strikes [equity][nk]
     const int* grid_mat_at;     // [n_dates] maturity index j at the date, else -1
     const float* grid_kf;       // [nk] absolute strikes
     const double* grid_kd;

@@ -44,6 +44,10 @@ struct GridPlan
     int blocks_per_cu = 2, lev_group = 4;
     double tune_ms = 0;
     int candidates = 0;
+    // grouped books: original book b is equity slot_of[b] of device book
+    // group_of[b] (books_per_lane = 1: every book its own device book)
+    int books_per_lane = 1;
+    std::vector<int> group_of, slot_of;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~GridPlan()
@@ -75,9 +79,10 @@ int Enqueue( const GridPlan& P, hipStream_t s )
                                                                      B.dbooks, DeviceDirections(), B.q0, B.first,
                                                                      B.last, P.ditems, P.n_items, B.nblocks,
                                                                      P.dcounter ) ) ||
-                              launch_persistent_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                              ( B.kind != 5 &&  // grouped books have no generic form
+                                launch_persistent_f32( B.na_max, B.depth, grid, s, B.lds_bytes, B.dbooks,
                                                      DeviceDirections(), B.q0, B.first, B.last, P.ditems, P.n_items,
-                                                     B.nblocks, P.dcounter );
+                                                     B.nblocks, P.dcounter ) );
     if ( !ok )
         return SetError( THOTH_ERR_UNSUPPORTED, "no kernel instantiation for this book shape" );
     HIP_OK( hipGetLastError() );
@@ -164,8 +169,86 @@ int Capture( GridPlan& P )
 namespace host
 {
 
+// K single-equity LSV books on the same grid and Sobol points (FP32) can be
+// stepped as the K equities of one device book on two shared factors (sweep
+// kind 5): the same normals and bridge serve all K.  K = 5, else 2, when it
+// divides the book count; 1 otherwise.
+static int GroupSize( const std::vector<thoth_mcl_book>& books, const std::vector<GridSpec>& specs, int precision )
+{
+    const size_t n = books.size();
+    if ( precision != THOTH_FP32 || n < 2 )
+        return 1;
+    const thoth_mcl_book& b0 = books[0];
+    for ( size_t b = 0; b < n; b++ )
+    {
+        const thoth_mcl_book& x = books[b];
+        if ( x.n_assets != 1 || x.model[0] != THOTH_MODEL_LSV || !( x.xi[0] > 1e-12 ) || x.n_contracts != 0 ||
+             x.n_rainbows != 0 || x.n_dates != b0.n_dates || x.lev_points != b0.lev_points ||
+             !std::equal( x.t, x.t + x.n_dates, b0.t ) ||
+             !std::equal( x.disc_rate, x.disc_rate + x.n_dates, b0.disc_rate ) || specs[b].nm != specs[0].nm ||
+             specs[b].nk != specs[0].nk ||
+             !std::equal( specs[b].mat_date, specs[b].mat_date + specs[b].nm, specs[0].mat_date ) )
+            return 1;
+    }
+    return n % 5 == 0 ? 5 : n % 2 == 0 ? 2 : 1;
+}
+
+// the device book of K grouped books: their equities in order, an identity
+// correlation (the factors are shared, not correlated), no contracts, the
+// discount curve they all have
+static void MergeBooks( const std::vector<const BookArrays*>& in, const std::vector<thoth_mcl_book>& books, int first,
+                        BookArrays& M )
+{
+    const int K = (int)in.size();
+    const thoth_mcl_book& b0 = books[first];
+    const int nd = b0.n_dates, P = b0.lev_points;
+    for ( int a = 0; a < K; a++ )
+    {
+        const BookArrays& A = *in[a];
+        M.model.push_back( A.model[0] );
+        M.spot.push_back( A.spot[0] );
+        M.vol.push_back( A.vol[0] );
+        M.v0.push_back( A.v0[0] );
+        M.kappa.push_back( A.kappa[0] );
+        M.theta.push_back( A.theta[0] );
+        M.xi.push_back( A.xi[0] );
+        M.rho.push_back( A.rho[0] );
+        M.drift.insert( M.drift.end(), A.drift.begin(), A.drift.begin() + nd );
+        M.lev_ln_step.insert( M.lev_ln_step.end(), A.lev_ln_step.begin(), A.lev_ln_step.begin() + nd );
+        M.lev_offset.insert( M.lev_offset.end(), A.lev_offset.begin(), A.lev_offset.begin() + nd );
+        M.lev_values.insert( M.lev_values.end(), A.lev_values.begin(), A.lev_values.begin() + (size_t)nd * P );
+    }
+    M.chol.assign( (size_t)K * K, 0.0 );
+    for ( int a = 0; a < K; a++ )
+        M.chol[(size_t)a * K + a] = 1.0;
+    M.disc = in[0]->disc;
+    M.c_mon_off.assign( 1, 0 );
+    thoth_mcl_book& b = M.book;
+    b = thoth_mcl_book{};
+    b.n_dates = nd;
+    b.t = b0.t;
+    b.dt = b0.dt;
+    b.n_assets = K;
+    b.model = M.model.data();
+    b.spot = M.spot.data();
+    b.vol = M.vol.data();
+    b.v0 = M.v0.data();
+    b.kappa = M.kappa.data();
+    b.theta = M.theta.data();
+    b.xi = M.xi.data();
+    b.rho = M.rho.data();
+    b.drift = M.drift.data();
+    b.chol = M.chol.data();
+    b.lev_points = P;
+    b.lev_ln_step = M.lev_ln_step.data();
+    b.lev_offset = M.lev_offset.data();
+    b.lev_values = M.lev_values.data();
+    b.disc_rate = M.disc.data();
+    b.c_mon_offset = M.c_mon_off.data();
+}
+
 int PrepareGridPlan( int n, const thoth_mcl_request* reqs, const thoth_vanilla_grid* grids, bool autotune,
-                     GridPlan& P )
+                     GridPlan& P, bool allow_grouping )
 {
     if ( n < 1 || !reqs || !grids )
         throw Invalid( THOTH_ERR_INVALID, "null argument" );
@@ -272,6 +355,40 @@ int PrepareGridPlan( int n, const thoth_mcl_request* reqs, const thoth_vanilla_g
     std::vector<thoth_mcl_book> books;
     for ( const BookArrays& A : arrays )
         books.push_back( A.book );
+    // grouping of the books into device books of K equities on shared factors
+    const int K = allow_grouping ? GroupSize( books, specs, r0.precision ) : 1;
+    P.books_per_lane = K;
+    P.group_of.assign( books.size(), 0 );
+    P.slot_of.assign( books.size(), 0 );
+    std::vector<BookArrays> merged( K > 1 ? books.size() / K : 0 );
+    if ( K > 1 )
+    {
+        std::vector<thoth_mcl_book> dev;
+        std::vector<GridSpec> dev_specs;
+        for ( size_t g = 0; g < merged.size(); g++ )
+        {
+            std::vector<const BookArrays*> in;
+            GridSpec gs = specs[g * K];
+            gs.grouped = 1;
+            gs.udl = 0;
+            for ( int a = 0; a < K; a++ )
+            {
+                const size_t b = g * K + a;
+                in.push_back( &arrays[b] );
+                std::copy( specs[b].strikes, specs[b].strikes + specs[b].nk, gs.strikes_by_equity[a] );
+                P.group_of[b] = (int)g;
+                P.slot_of[b] = a;
+            }
+            MergeBooks( in, books, (int)( g * K ), merged[g] );
+            dev.push_back( merged[g].book );
+            dev_specs.push_back( gs );
+        }
+        books.swap( dev );
+        specs.swap( dev_specs );
+    }
+    else
+        for ( size_t b = 0; b < books.size(); b++ )
+            P.group_of[b] = (int)b;
     thoth_mcl_sweep_cfg cfg{};
     cfg.paths = r0.mcl.paths;
     cfg.sobol_skip = r0.mcl.sobol_skip;
@@ -325,10 +442,18 @@ int RunGridPlan( GridPlan& P, thoth_grid_result* res, double* device_ms )
     (void)hipEventDestroy( e1 );
     if ( device_ms )
         *device_ms = f;
-    std::vector<std::vector<double>> sums;
-    int rc = CopyGrids( P.batch, sums );
+    std::vector<std::vector<double>> dsums;
+    int rc = CopyGrids( P.batch, dsums );
     if ( rc != THOTH_OK )
         return rc;
+    // per original book: its device book's sums, or its equity's slice of them
+    std::vector<std::vector<double>> sums( P.group_of.size() );
+    for ( size_t b = 0; b < sums.size(); b++ )
+    {
+        const std::vector<double>& d = dsums[P.group_of[b]];
+        const size_t per = d.size() / (size_t)P.books_per_lane;
+        sums[b].assign( d.begin() + P.slot_of[b] * per, d.begin() + ( P.slot_of[b] + 1 ) * per );
+    }
     const double n = (double)P.paths;
     for ( int r = 0; r < P.n_req; r++ )
     {
@@ -368,10 +493,10 @@ int RunGridPlan( GridPlan& P, thoth_grid_result* res, double* device_ms )
 
 using namespace thoth;
 
-extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_prepare( int32_t n,
-                                                                            const thoth_mcl_request* reqs,
-                                                                            const thoth_vanilla_grid* grids,
-                                                                            int32_t autotune, thoth_grid_plan* plan )
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_prepare_ex( int32_t n,
+                                                                               const thoth_mcl_request* reqs,
+                                                                               const thoth_vanilla_grid* grids,
+                                                                               int32_t flags, thoth_grid_plan* plan )
 {
     if ( !plan )
         return SetError( THOTH_ERR_INVALID, "null argument" );
@@ -382,8 +507,17 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_prepare( 
         int rc = EnsureDevice();
         if ( rc != THOTH_OK )
             return rc;
+        const bool autotune = ( flags & THOTH_GRID_AUTOTUNE ) != 0;
         auto* P = new GridPlan;
-        rc = host::PrepareGridPlan( n, reqs, grids, autotune != 0, *P );
+        rc = host::PrepareGridPlan( n, reqs, grids, autotune, *P, ( flags & THOTH_GRID_SEPARATE_BOOKS ) == 0 );
+        if ( rc == THOTH_ERR_UNSUPPORTED && P->books_per_lane > 1 )
+        {
+            // no grouped form for these books (bridge depth, draws past the
+            // Sobol table): one device book per book
+            delete P;
+            P = new GridPlan;
+            rc = host::PrepareGridPlan( n, reqs, grids, autotune, *P, false );
+        }
         if ( rc != THOTH_OK )
         {
             delete P;
@@ -400,6 +534,14 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_prepare( 
     {
         return SetError( THOTH_ERR_INVALID, e.what() );
     }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_prepare( int32_t n,
+                                                                            const thoth_mcl_request* reqs,
+                                                                            const thoth_vanilla_grid* grids,
+                                                                            int32_t autotune, thoth_grid_plan* plan )
+{
+    return thoth_grid_prepare_ex( n, reqs, grids, autotune ? THOTH_GRID_AUTOTUNE : 0, plan );
 }
 
 extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_run( thoth_grid_plan plan,
@@ -422,6 +564,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_grid_get_tunin
     tuning->lev_group = P.lev_group;
     tuning->device_ms = P.tune_ms;
     tuning->candidates = P.candidates;
+    tuning->books_per_lane = P.books_per_lane;
     return THOTH_OK;
 }
 

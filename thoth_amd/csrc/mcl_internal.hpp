@@ -134,6 +134,11 @@ struct GridSpec
     int nm = 0, nk = 0, udl = 0;
     int mat_date[kGridMaxMat] = {};
     double strikes[kGridMaxStrikes] = {};
+    // grouped: the book is several single-equity LSV books on one grid and one
+    // set of Sobol points, one per equity, stepped together on two shared
+    // factors (sweep kind 5); every equity has a grid with its own strikes
+    int grouped = 0;
+    double strikes_by_equity[kMaxAssets][kGridMaxStrikes] = {};
 };
 // sums of P - CV per replicate (mode 2)
 struct CvSums

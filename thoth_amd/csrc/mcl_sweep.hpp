@@ -488,7 +488,11 @@ static_assert( kCvImgBytes == kMlpTiles * 64 * 8 && kCvBiasBytes == kMlpBSize * 
 // 4 any book (the arithmetic of 0) whose normals are not all Sobol points:
 // the dimensions past the Sobol budget read precomputed pseudo-random
 // normals, and pseudo books (use_sobol false) draw every normal from the
-// lane's xoshiro256++ stream.
+// lane's xoshiro256++ stream; 5 NA single-equity LSV books on one grid and
+// one set of Sobol points stepped together as the NA equities of one book:
+// every equity reads the two shared factors (its book's spot and variance
+// noises are the same normals), so the normals and the bridge are computed
+// once for the NA books, and every equity carries its own vanilla grid.
 template <class R, int NA, int DEPTH, int CV, int KIND = 0>
 __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* __restrict__ V, uint32_t bq,
                                              uint32_t first_idx, uint32_t last_idx,
@@ -498,13 +502,14 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     constexpr bool F64 = std::is_same_v<R, double>;
     constexpr bool prng = KIND == 4;  // pseudo-random draws
     constexpr bool uniform = KIND != 0 && !prng;
-    constexpr bool lsv_kind = KIND == 1 || KIND == 3;  // every equity LSV
+    constexpr bool grouped = KIND == 5;                            // NA books sharing their two factors
+    constexpr bool lsv_kind = KIND == 1 || KIND == 3 || grouped;  // every equity LSV
     constexpr bool corr_kind = KIND == 3;              // correlated spot noises
     constexpr bool qe_phase = uniform;
     const int nas = uniform ? NA : B.n_assets;  // row length of the date-major per-step arrays
     static_assert( !uniform || !F64, "uniform-book kernels are FP32" );
     static_assert( !prng || CV == 0, "pseudo-random draws run without the control variate" );
-    constexpr int NF = 2 * NA;
+    constexpr int NF = grouped ? 2 : 2 * NA;
     constexpr int NFV = NF;
     constexpr int NAV = NA;
     typedef R stackv __attribute__( ( ext_vector_type( DEPTH ) ) );
@@ -985,9 +990,10 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     // per (replicate, Sobol block, maturity, cell).  Per wave, 16 cells at a
     // time go lane -> LDS [cell][lane]; lane l then adds a quarter row
     // (cell l/4, lanes 16 (l%4) ..) and two shuffles finish the row.
-    auto grid_observe = [&]( int j ) __attribute__( ( always_inline ) )
+    // one grid (underlying value s, strikes kf / kd) at maturity j into the
+    // partial slot `out`
+    auto grid_observe_one = [&]( R s, const float* kf, const double* kd, double* out ) __attribute__( ( always_inline ) )
     {
-        const R s = udl_value( B.grid_udl );
         const int ncell = 2 * B.grid_nk;
         R* tr = reinterpret_cast<R*>( gl ) + wib * kGridRows * 64;
         double* stage = reinterpret_cast<double*>( gl + kWavesPerBlock * kGridRows * 64 * sizeof( R ) );
@@ -1002,9 +1008,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 {
                     R K;
                     if constexpr ( F64 )
-                        K = B.grid_kd[c >> 1];
+                        K = kd[c >> 1];
                     else
-                        K = B.grid_kf[c >> 1];
+                        K = kf[c >> 1];
                     x = ( c & 1 ) ? K - s : s - K;  // finance.cpp:9-32
                     x = x < R( 0 ) ? R( 0 ) : x;
                 }
@@ -1032,7 +1038,6 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             wave_lds_sync();
         }
         __syncthreads();
-        double* out = B.grid_part + ( ( (size_t)z * ntiles + tile ) * B.grid_nm + j ) * ncell * 2;
         for ( int t = threadIdx.x; t < 2 * ncell; t += kBlock )
         {
             const int c = t >> 1, mom = t & 1;
@@ -1042,6 +1047,23 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             out[t] = v;
         }
         __syncthreads();
+    };
+    // the book's grid at maturity j; grouped books: every equity's grid, the
+    // slot holding [equity][maturity][cell][2]
+    auto grid_observe = [&]( int j ) __attribute__( ( always_inline ) )
+    {
+        const int ncell = 2 * B.grid_nk;
+        if constexpr ( grouped )
+        {
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+                grid_observe_one( spot_of( a ), B.grid_kf + a * B.grid_nk, B.grid_kd + a * B.grid_nk,
+                                  B.grid_part + ( ( ( (size_t)z * ntiles + tile ) * NA + a ) * B.grid_nm + j ) *
+                                                    ncell * 2 );
+        }
+        else
+            grid_observe_one( udl_value( B.grid_udl ), B.grid_kf, B.grid_kd,
+                              B.grid_part + ( ( (size_t)z * ntiles + tile ) * B.grid_nm + j ) * ncell * 2 );
     };
 
     // path recording of an American contract's equity (path_recorder; the
@@ -1263,7 +1285,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             for ( int a = 0; a < NA; a++ )
             {
                 cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
-                const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+                const int vf = grouped ? 1 : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                 const QeMoments q = qe_moments( Var[a], sc );
                 vq[a] = qe_quadratic( q, nz[vf], cv_on ? &dq[a] : nullptr );
                 expo |= q.quad ? 0u : 1u << a;
@@ -1275,7 +1297,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     if ( ( expo >> a ) & 1u )
                     {
                         cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
-                        const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+                        const int vf = grouped ? 1 : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                         vq[a] = qe_exponential( qe_moments( Var[a], sc ), nz[vf], cv_on ? &dq[a] : nullptr );
                     }
             }
@@ -1289,7 +1311,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     continue;
             const int model = lsv_kind ? 2 : KIND == 2 ? 1 : B.asset[a].model;
             // correlated noise (correlated_noise_node.cpp:15-27, lower triangle, name order)
-            R eps = nz[a];
+            R eps = nz[grouped ? 0 : a];
             // the factor of the step into date i (term-structured correlation:
             // one per date, correlation.cpp:860-903), read by scalar loads
             const size_t crow = ( (size_t)i * kMaxAssets + a ) * kMaxAssets;
@@ -1356,7 +1378,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 continue;
             }
             // Heston / LSV (heston_variance_node.cpp, heston_spot_node.cpp)
-            const int vf = __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+            const int vf = grouped ? 1 : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
             const R zv = nz[vf];
             const R vp = Var[a];
             R vc;

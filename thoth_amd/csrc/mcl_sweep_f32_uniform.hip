@@ -39,7 +39,8 @@ static bool persistent_by_depth( int depth, dim3 grid, hipStream_t s, size_t lds
 }
 
 // the persistent (work-queue) form of the same instantiations: one-equity LSV
-// (the vanilla grids) and Heston books here, five-equity LSV books in their own unit
+// (the vanilla grids) and Heston books here, five-equity LSV books in their own
+// unit; kind 5 (grouped single-equity LSV grid books) for 2 and 5 books
 bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
                                     const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first,
                                     uint32_t last, const int2* items, int n_items, uint32_t ntiles, int* counter )
@@ -50,9 +51,12 @@ bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipS
     if ( kind == 2 && na == 1 )
         return persistent_by_depth<1, 2>( depth, grid, s, lds, books, V, bq0, first, last, items, n_items, ntiles,
                                           counter );
-    if ( ( kind == 1 || kind == 3 ) && na == 5 )
+    if ( ( kind == 1 || kind == 3 || kind == 5 ) && na == 5 )
         return launch_persistent_f32_uniform5( kind, depth, grid, s, lds, books, V, bq0, first, last, items, n_items,
                                                ntiles, counter );
+    if ( kind == 5 && na == 2 )  // grouped grid books, two per workgroup lane
+        return persistent_by_depth<2, 5>( depth, grid, s, lds, books, V, bq0, first, last, items, n_items, ntiles,
+                                          counter );
     return false;
 }
 }  // namespace thoth

@@ -24,6 +24,16 @@ bool launch_persistent_f32_uniform5( int kind, int depth, dim3 grid, hipStream_t
 {
     if ( depth > 10 )
         return false;
+    if ( kind == 5 )
+    {
+        if ( depth <= 8 )
+            hipLaunchKernelGGL( ( sweep_persistent<float, 5, 8, 5> ), grid, dim3( kBlock ), lds, s, books, V, bq0,
+                                first, last, items, n_items, ntiles, counter );
+        else
+            hipLaunchKernelGGL( ( sweep_persistent<float, 5, 10, 5> ), grid, dim3( kBlock ), lds, s, books, V, bq0,
+                                first, last, items, n_items, ntiles, counter );
+        return true;
+    }
     if ( kind == 3 )
     {
         if ( depth <= 8 )

@@ -61,6 +61,8 @@ def lib():
         L.thoth_mcl_pool_results.argtypes = [C.c_int32, P(C.c_int64), P(abi.Result), C.c_int32, P(abi.Result)]
         L.thoth_mcl_price_split.argtypes = [P(abi.Request), C.c_int32, P(abi.Result)]
         L.thoth_grid_prepare.argtypes = [C.c_int32, P(abi.Request), P(abi.VanillaGrid), C.c_int32, P(C.c_void_p)]
+        L.thoth_grid_prepare_ex.argtypes = [C.c_int32, P(abi.Request), P(abi.VanillaGrid), C.c_int32,
+                                            P(C.c_void_p)]
         L.thoth_grid_run.argtypes = [C.c_void_p, P(abi.GridResult), P(C.c_double)]
         L.thoth_grid_get_tuning.argtypes = [C.c_void_p, P(abi.GridTuning)]
         L.thoth_grid_release.argtypes = [C.c_void_p]
@@ -445,14 +447,16 @@ class GridPlan:
     """All books (request x bump scenario) of a set of vanilla grids, swept by
     one persistent launch replayed from a hipGraph (thoth_grid_*)."""
 
-    def __init__(self, builts, grids, autotune=True):
+    def __init__(self, builts, grids, autotune=True, group=True):
+        """group=False keeps one device book per book (no shared normals)."""
         self.builts = list(builts)
         self.grids = list(grids)
         reqs = (abi.Request * len(self.builts))(*[b.req for b in self.builts])
         gr = (abi.VanillaGrid * len(self.grids))(*self.grids)
         self._keep = (reqs, gr)
         self.handle = C.c_void_p()
-        _check(lib().thoth_grid_prepare(len(self.builts), reqs, gr, 1 if autotune else 0, C.byref(self.handle)))
+        flags = (abi.GRID_AUTOTUNE if autotune else 0) | (0 if group else abi.GRID_SEPARATE_BOOKS)
+        _check(lib().thoth_grid_prepare_ex(len(self.builts), reqs, gr, flags, C.byref(self.handle)))
 
     def run(self):
         """Returns (list of dicts of [n_maturities, n_strikes] arrays, device ms)."""
