@@ -684,45 +684,34 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     const bool lev_lds = !F64 && KIND != 2 && B.lev_points > 0;  // uniform Heston books have no leverage
     const int P = B.lev_points;
     const int G = B.lev_group;
-    // The group's (layers x equities) rows of P points are spread evenly over
-    // the block's threads, and each thread issues all its loads (up to
-    // kStageLoads) before its first store, so a group costs one memory latency
-    // rather than one per row (a row-by-row copy cost 15 % of the basket sweep).
+    // The ring holds each equity's 2G layer slots contiguously (equity a's
+    // slot q at (a 2G + q) P), so a group of an equity is one contiguous run
+    // of layers x P floats in the book's leverage array and in LDS.  Each
+    // thread takes elements tid + j kBlock of every equity's run and issues
+    // all its loads (up to kStageLoads) before its first store, so a group
+    // costs one memory latency rather than one per row.
     constexpr int kStageLoads = 8;
+    constexpr int kStageRuns = NA >= kStageLoads ? 1 : kStageLoads / NA;
     auto stage_group = [&]( int first ) __attribute__( ( always_inline ) )
     {
-        const int layers = min( first + G - 1, m ) - first + 1;
-        const int n = layers * na * P;
+        const int run = ( min( first + G - 1, m ) - first + 1 ) * P;
         // layer first + l goes to slot (first - 1) mod 2G + l (first - 1 is a multiple of G)
-        float* dst0 = levs + ( ( first - 1 ) % ( 2 * G ) ) * NA * P;
-        const float invP = 1.0f / (float)P, invna = 1.0f / (float)na;
-        for ( int e0 = 0; e0 < n; e0 += kBlock * kStageLoads )
+        const int q0 = ( first - 1 ) % ( 2 * G );
+        for ( int e0 = (int)threadIdx.x; e0 < run; e0 += kBlock * kStageRuns )
         {
-            float v[kStageLoads];
-            int d[kStageLoads];
+            float v[kStageRuns][NA];
 #pragma unroll
-            for ( int j = 0; j < kStageLoads; j++ )
-            {
-                const int e = e0 + (int)threadIdx.x + j * kBlock;
-                d[j] = -1;
-                if ( e < n )
-                {
-                    // row = e / P and l = row / na, estimated in FP32 and corrected
-                    int row = (int)( (float)e * invP );
-                    row -= row * P > e ? 1 : 0;
-                    row += ( row + 1 ) * P <= e ? 1 : 0;
-                    int l = (int)( (float)row * invna );
-                    l -= l * na > row ? 1 : 0;
-                    l += ( l + 1 ) * na <= row ? 1 : 0;
-                    const int a = row - l * na, k = e - row * P;
-                    v[j] = B.levf[( (size_t)a * nd + first + l ) * P + k];
-                    d[j] = ( l * NA + a ) * P + k;
-                }
-            }
+            for ( int j = 0; j < kStageRuns; j++ )
 #pragma unroll
-            for ( int j = 0; j < kStageLoads; j++ )
-                if ( d[j] >= 0 )
-                    dst0[d[j]] = v[j];
+                for ( int a = 0; a < NA; a++ )
+                    if ( e0 + j * kBlock < run && ( uniform || a < na ) )
+                        v[j][a] = B.levf[( (size_t)a * nd + first ) * P + e0 + j * kBlock];
+#pragma unroll
+            for ( int j = 0; j < kStageRuns; j++ )
+#pragma unroll
+                for ( int a = 0; a < NA; a++ )
+                    if ( e0 + j * kBlock < run && ( uniform || a < na ) )
+                        levs[( a * 2 * G + q0 ) * P + e0 + j * kBlock] = v[j][a];
         }
     };
     if constexpr ( block_table )
@@ -1424,7 +1413,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 }
                 else
                 {
-                    const float* lv = levs + ( ( i - 1 ) % ( 2 * G ) ) * NA * P + a * P;
+                    const float* lv = levs + ( a * 2 * G + ( i - 1 ) % ( 2 * G ) ) * P;
                     const float fc = fminf( fmaxf( f, 0.0f ), (float)last );
                     const int k = min( (int)fc, last - 1 );
                     const float w = fc - (float)k;
@@ -1573,8 +1562,11 @@ sweep_kernel( const DevBook* __restrict__ books, const uint32_t* __restrict__ V,
 // counter is zeroed by a kernel of the same graph).  Every item writes its
 // own partial-sum slot, so the result does not depend on which workgroup
 // took which item.
+// grouped grid books (kind 5) are held to 128 VGPRs: four waves per SIMD ran
+// the vanilla grids 8 % faster than three at the unconstrained 145
 template <class R, int NA, int DEPTH, int KIND = 0>
 __global__ void __launch_bounds__( kBlock )
+__attribute__( ( amdgpu_waves_per_eu( KIND == 5 ? 4 : 1 ) ) )
 sweep_persistent( const DevBook* __restrict__ books, const uint32_t* __restrict__ V, uint32_t bq0, uint32_t first_idx,
                   uint32_t last_idx, const int2* __restrict__ items, int n_items, uint32_t ntiles,
                   int* __restrict__ counter )
