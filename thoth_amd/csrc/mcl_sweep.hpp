@@ -975,7 +975,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     };
 
     // vanilla grid at maturity j: call and put payoffs of every strike, summed
-    // over the block's 256 paths (FP64 from the first addition), one partial
+    // over the block's 256 paths (FP64 kernels in FP64 from the first
+    // addition, FP32 kernels in FP64 past groups of sixteen), one partial
     // per (replicate, Sobol block, maturity, cell).  Per wave, 16 cells at a
     // time go lane -> LDS [cell][lane]; lane l then adds a quarter row
     // (cell l/4, lanes 16 (l%4) ..) and two shuffles finish the row.
@@ -983,37 +984,56 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     // partial slot `out`
     auto grid_observe_one = [&]( R s, const float* kf, const double* kd, double* out ) __attribute__( ( always_inline ) )
     {
-        const int ncell = 2 * B.grid_nk;
+        const int nk = B.grid_nk, ncell = 2 * nk;
         R* tr = reinterpret_cast<R*>( gl ) + wib * kGridRows * 64;
         double* stage = reinterpret_cast<double*>( gl + kWavesPerBlock * kGridRows * 64 * sizeof( R ) );
+        // a lane past the book's paths pays nothing: its calls see a zero
+        // spot, its puts an infinite one
+        const R s_call = valid ? s : R( 0 ), s_put = valid ? s : R( INFINITY );
         for ( int c0 = 0; c0 < ncell; c0 += kGridRows )
         {
+            // cell c0 + r: the call (r even; c0 is even) or the put of strike
+            // (c0 + r) / 2, read by scalar loads; rows past the grid repeat
+            // the last strike and are never staged
 #pragma unroll
             for ( int r = 0; r < kGridRows; r++ )
             {
-                const int c = c0 + r;
-                R x = R( 0 );
-                if ( c < ncell )
-                {
-                    R K;
-                    if constexpr ( F64 )
-                        K = kd[c >> 1];
-                    else
-                        K = kf[c >> 1];
-                    x = ( c & 1 ) ? K - s : s - K;  // finance.cpp:9-32
-                    x = x < R( 0 ) ? R( 0 ) : x;
-                }
-                tr[r * 64 + lane] = valid ? x : R( 0 );
+                const int k = min( ( c0 + r ) >> 1, nk - 1 );
+                R K;
+                if constexpr ( F64 )
+                    K = as_const( kd )[k];
+                else
+                    K = as_const( kf )[k];
+                const R x = ( r & 1 ) ? K - s_put : s_call - K;  // finance.cpp:9-32
+                tr[r * 64 + lane] = x < R( 0 ) ? R( 0 ) : x;
             }
             wave_lds_sync();
             const int row = lane >> 2, q = lane & 3;
-            double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-            for ( int t = 0; t < 16; t++ )
+            double s1, s2;
+            if constexpr ( F64 )
             {
-                const double v = (double)tr[row * 64 + q * 16 + t];
-                s1 += v;
-                s2 = fma( v, v, s2 );
+                s1 = s2 = 0.0;
+#pragma unroll
+                for ( int t = 0; t < 16; t++ )
+                {
+                    const double v = tr[row * 64 + q * 16 + t];
+                    s1 += v;
+                    s2 = fma( v, v, s2 );
+                }
+            }
+            else
+            {
+                // sixteen payoffs in FP32 (relative error below 1e-6), then FP64
+                float f1 = 0.0f, f2 = 0.0f;
+#pragma unroll
+                for ( int t = 0; t < 16; t++ )
+                {
+                    const float v = tr[row * 64 + q * 16 + t];
+                    f1 += v;
+                    f2 = fmaf( v, v, f2 );
+                }
+                s1 = f1;
+                s2 = f2;
             }
             s1 += __shfl_xor( s1, 1, 64 );
             s2 += __shfl_xor( s2, 1, 64 );
