@@ -295,11 +295,12 @@ __device__ __forceinline__ QeMoments qe_moments( float v, P sc )
     return { m, m2, s2, !( m > 0.0f ) || s2 <= 1.5f * m2 };
 }
 
-// psi <= 1.5: v' = a (sqrt(b2) + z)^2
+// psi <= 1.5: v' = a (sqrt(b2) + z)^2 with c = 2 m^2 / s^2 = 2 / psi,
+// b2 = c - 1 + sqrt(c (c - 1)) and c (c - 1) = (c - 1)^2 + (c - 1)
 __device__ __forceinline__ float qe_quadratic( const QeMoments& q, float z, QeDraw* draw )
 {
-    const float c = 2.0f * q.m2 * fast_rcp( q.s2 );
-    const float b2 = c - 1.0f + __builtin_amdgcn_sqrtf( c * ( c - 1.0f ) );
+    const float cm1 = fmaf( q.m2 + q.m2, fast_rcp( q.s2 ), -1.0f );
+    const float b2 = cm1 + __builtin_amdgcn_sqrtf( fmaf( cm1, cm1, cm1 ) );
     const float a = fast_div( q.m, 1.0f + b2 );
     const float sb = __builtin_amdgcn_sqrtf( b2 );
     const float bz = sb + z;
@@ -308,18 +309,18 @@ __device__ __forceinline__ float qe_quadratic( const QeMoments& q, float z, QeDr
     return a * bz * bz;
 }
 
-// psi > 1.5: p = (psi-1)/(psi+1), beta = (1-p)/m; v' = log((1-p)/(1-u)) / beta
+// psi > 1.5: p = (psi - 1) / (psi + 1), beta = (1 - p) / m, and
+// v' = log((1 - p) / (1 - u)) / beta, 0 where u <= p.  With h = (psi + 1) / 2,
+// 1 - p = 1 / h and 1 / beta = m h, so v' = -log(h (1 - u)) m h, 0 where
+// h (1 - u) >= 1; 1 - u = erfc(z / sqrt 2) / 2.  E[v'] = (1 - p) / beta = m.
 __device__ __forceinline__ float qe_exponential( const QeMoments& q, float z, QeDraw* draw )
 {
-    const float psi = q.s2 * fast_rcp( q.m2 );
-    const float inv = fast_rcp( psi + 1.0f );
-    const float one_minus_p = 2.0f * inv;
+    const float h4 = fmaf( q.s2, fast_rcp( q.m2 ), 1.0f ) * 0.25f;  // h / 2
     if ( draw )
-        draw->mean = one_minus_p * q.m * 0.5f * ( psi + 1.0f );  // (1 - p) / beta
-    const float one_minus_u = 0.5f * erfc_fast( z * 0.70710678118654752f );
-    return ( one_minus_u >= one_minus_p )
-               ? 0.0f
-               : fast_log( fast_div( one_minus_p, one_minus_u ) ) * q.m * 0.5f * ( psi + 1.0f );
+        draw->mean = q.m;
+    const float A = h4 * erfc_fast( z * 0.70710678118654752f );  // h (1 - u)
+    return A >= 1.0f ? 0.0f
+                     : __builtin_amdgcn_logf( A ) * ( -0.693147180559945309f * 2.0f ) * ( q.m * h4 );
 }
 
 template <class P>
@@ -684,6 +685,8 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
     const bool lev_lds = !F64 && KIND != 2 && B.lev_points > 0;  // uniform Heston books have no leverage
     const int P = B.lev_points;
     const int G = B.lev_group;
+    // the largest float below the last leverage point P - 1 (P >= 2)
+    [[maybe_unused]] const float lev_top = __uint_as_float( __float_as_uint( (float)( P - 1 ) ) - 1u );
     // The ring holds each equity's 2G layer slots contiguously (equity a's
     // slot q at (a 2G + q) P), so a group of an equity is one contiguous run
     // of layers x P floats in the book's leverage array and in LDS.  Each
@@ -1434,8 +1437,10 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 else
                 {
                     const float* lv = levs + ( a * 2 * G + ( i - 1 ) % ( 2 * G ) ) * P;
-                    const float fc = fminf( fmaxf( f, 0.0f ), (float)last );
-                    const int k = min( (int)fc, last - 1 );
+                    // clamped to [0, last) so that k <= last - 1: past the last
+                    // point w = 1 - 2^-24 last rather than 1
+                    const float fc = __builtin_amdgcn_fmed3f( f, 0.0f, lev_top );
+                    const int k = (int)fc;
                     const float w = fc - (float)k;
                     L = fmaf( w, lv[k + 1] - lv[k], lv[k] );
                 }
