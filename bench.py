@@ -8,7 +8,8 @@ with FP64 accumulation), the book prepared and resident in HBM before the
 timed region (grid, LSV leverage calibration and packing are setup).  Prints
 ONE JSON line: paths/s of that sweep with its roofline, beside the other
 workloads (legs): the basket with Cholesky-correlated spots and in FP64, the
-Heston down-and-out call with bump and pathwise Greeks, the vanilla grids in
+basket's bump and pathwise Greeks, the Heston down-and-out call with bump and
+pathwise Greeks, the vanilla grids in
 one persistent graph-replayed launch, the learned control variate as
 specified, the time to a 1bp confidence interval measured on the host clock,
 and the CPU oracle timed on this host.  `--leg` runs only the named legs
@@ -370,6 +371,46 @@ def heston_barrier(paths=1 << 20, steps=252, repeats=3):
     return out
 
 
+def basket_greeks(paths=1 << 20, repeats=2, fp64_paths=1 << 14):
+    """Greeks of the 5-asset LSV worst-of basket (365 steps, 100 x 50 leverage
+    surface, FP32 stepping): the reference's bump Greeks (+1 % spot per
+    equity, +-10 % for gamma, +1 vol point, the leverage recalibrated in every
+    scenario: 17 books in one launch) beside the pathwise basket Greeks (the
+    five deltas and the vega from one book, gamma from the pathwise deltas of
+    the +-10 % books: 11 books in one launch); device time of each launch and
+    host setup (calibrations, packing, upload), and the FP32 - FP64 difference
+    of the pathwise figures on the same points (fp64_paths)."""
+    greeks = abi.GREEK_DELTA | abi.GREEK_GAMMA | abi.GREEK_VEGA
+    b = basket(paths, greeks=greeks)
+    t0 = time.perf_counter()
+    plan = mcl.Plan(b)
+    setup = time.perf_counter() - t0
+    plan.run()
+    ms = [plan.run().sweep_ms for _ in range(repeats)]
+    r = plan.run()
+    pw = [mcl.price_pathwise(b) for _ in range(repeats)]
+    p = pw[-1]
+    n = p.n_assets
+    out = {"workload": "lsv_worst_of_basket_greeks", "paths": paths, "steps": 365, "assets": n,
+           "bump": {"premium": r.premium, "delta": r.delta, "gamma": r.gamma, "vega": r.vega,
+                    "ms": min(ms), "books": r.n_scenarios, "setup_s": setup},
+           "pathwise": {"premium": p.premium, "delta": p.delta, "delta_trust": p.delta_trust,
+                        "asset_delta": [p.asset_delta[a] for a in range(n)],
+                        "asset_delta_trust": [p.asset_delta_trust[a] for a in range(n)],
+                        "gamma": p.gamma, "asset_gamma": [p.asset_gamma[a] for a in range(n)],
+                        "vega": p.vega, "vega_trust": p.vega_trust,
+                        "ms": min(x.sweep_ms for x in pw), "books": p.n_books,
+                        "setup_s": min(x.setup_ms for x in pw) * 1e-3}}
+    small = {prec: mcl.price_pathwise(basket(fp64_paths, precision=prec, greeks=greeks))
+             for prec in (abi.FP32, abi.FP64)}
+    a, c = small[abi.FP32], small[abi.FP64]
+    out["pathwise_fp32_minus_fp64"] = {
+        "paths": fp64_paths, "premium": a.premium - c.premium, "delta": a.delta - c.delta,
+        "asset_delta": [a.asset_delta[k] - c.asset_delta[k] for k in range(n)],
+        "gamma": a.gamma - c.gamma, "vega": a.vega - c.vega}
+    return out
+
+
 def american_leg(repeats=3):
     """The American put of the reference's tests (S = K = 100, r 8 %, vol 30 %,
     1y in 30-day steps, pseudo-random draws, 300,000 paths) with delta, gamma,
@@ -448,7 +489,7 @@ def vanilla_grids(paths, replays=5):
             "setup_s": setup}
 
 
-LEGS = ("c3", "c3corr", "c3fp64", "c2", "c4", "c5", "t1bp", "american", "prng", "cpu", "traffic")
+LEGS = ("c3", "c3corr", "c3fp64", "c3greeks", "c2", "c4", "c5", "t1bp", "american", "prng", "cpu", "traffic")
 
 
 def progress(msg):
@@ -528,6 +569,9 @@ def main():
     if "c3fp64" in legs:
         progress("c3 fp64")
         line["basket_fp64"] = basket_leg(args.paths, 3, 1, precision=abi.FP64)
+    if "c3greeks" in legs:
+        progress("c3 greeks: bump and pathwise")
+        line["basket_greeks"] = basket_greeks(args.paths)
     if "c4" in legs:
         progress("c4 vanilla grids")
         line["vanilla_grids"] = vanilla_grids(args.grid_paths)

@@ -153,6 +153,17 @@ typedef struct thoth_mcl_request {
     int32_t n_corr_pillars;
     const double* corr_maturities;
     const double* corr_matrices;
+    /* Scenario inputs (engine extensions, zero / NULL: the reference's base
+     * scenario).  vol_shift: a parallel volatility shift of every equity, the
+     * shift the reference's vega scenario applies (Volatility::ApplyShift of
+     * RISK_FACTOR_VOL, tasks/pricer.cpp:325-333): Black-Scholes vol + shift,
+     * Heston sqrt(v0) and sqrt(theta) + shift, LSV target implied vols + shift
+     * (the leverage recalibrated); the vega bump adds its own 0.01 on top.
+     * rebase_spots: n_assets rebasing spots S_i0 of the rainbow members
+     * (rainbow.cpp:22-27 captures them at load, so a spot bump does not move
+     * them); NULL: the equities' spots. */
+    double vol_shift;
+    const double* rebase_spots;
 } thoth_mcl_request;
 
 typedef struct thoth_mcl_result {
@@ -225,23 +236,42 @@ typedef struct thoth_rqmc_result {
 int thoth_mcl_price_rqmc(const thoth_mcl_request* req, int32_t replicates, uint64_t rqmc_seed, thoth_rqmc_result* res);
 
 /* Pathwise Greeks (no reference counterpart; the reference computes bump
- * Greeks only, pricer_mcl_greeks.cpp:43-299) of a single-equity Black-Scholes
- * or Heston book holding one vanilla or one knock-out barrier contract, on
- * the same Sobol points.  Knock-out books are simulated with one-step
- * survival conditioning (each monitored step draws the spot conditioned on
- * survival and carries the survival probability as a weight): an unbiased
- * estimator of the same discretely monitored price whose pathwise derivative
- * exists.  delta = dP/dS0, vega per vol point in the reference's bump
+ * Greeks only, pricer_mcl_greeks.cpp:43-299) on the same Sobol points, of
+ *  - a single-equity Black-Scholes or Heston book holding one vanilla or one
+ *    knock-out barrier contract.  Knock-out books are simulated with one-step
+ *    survival conditioning (each monitored step draws the spot conditioned on
+ *    survival and carries the survival probability as a weight): an unbiased
+ *    estimator of the same discretely monitored price whose pathwise
+ *    derivative exists;
+ *  - any other book of 1..8 Black-Scholes, Heston or LSV equities (correlated
+ *    as the sweep correlates them) holding one european vanilla on an equity
+ *    or on a best-of / worst-of rainbow.  LSV equities carry the tangent
+ *    through the leverage lookup: its slope in log-spot, and the leverage
+ *    function's own sensitivity to the spot and to the vol shift (the
+ *    reference recalibrates the leverage in every bump scenario), taken as
+ *    that recalibration at the reference's bump (+1 % spot, +1 vol point) over
+ *    the bump (thoth_mcl_leverage_tangent): pathwise along the paths, the
+ *    reference's finite bump for the particle calibration, whose
+ *    infinitesimal derivative is dominated by its bin kinks.
+ * delta = sum over equities of dP/dS0_a (the reference's book delta sums its
+ * per-underlying bumps, pricer.cpp:287-323), each in asset_delta; vega per vol
+ * point of the parallel shift of every equity in the reference's bump
  * parametrisation (Black-Scholes vol + 1pt; Heston sqrt(v0) and sqrt(theta)
- * + 0.01), gamma = (delta(S0 (1 + 10%)) - delta(S0 (1 - 10%))) / (20% S0)
- * when THOTH_GREEK_GAMMA is requested (the reference's gamma bump applied to
- * the pathwise delta). */
+ * + 0.01; LSV target implied vols + 1pt); gamma (THOTH_GREEK_GAMMA) = sum
+ * over equities of (delta_a(S0_a (1 + 10%)) - delta_a(S0_a (1 - 10%))) /
+ * (20% S0_a), the reference's gamma bump applied to the pathwise delta. */
 typedef struct thoth_pathwise_result {
     double premium, premium_trust;   /* the survival-weighted estimator */
-    double delta, delta_trust;
-    double gamma;
+    double delta, delta_trust;       /* book delta: sum of the per-equity deltas */
+    double gamma;                    /* book gamma: sum of the per-equity gammas */
     double vega, vega_trust;
     double sweep_ms;
+    int32_t n_assets;
+    double asset_delta[THOTH_MCL_MAX_ASSETS];        /* dP / dS0_a */
+    double asset_delta_trust[THOTH_MCL_MAX_ASSETS];
+    double asset_gamma[THOTH_MCL_MAX_ASSETS];        /* the +-10 % bump of equity a applied to its pathwise delta */
+    int32_t n_books;                 /* books swept (base, and two per equity for gamma) */
+    double setup_ms;                 /* host clock: calibrations and their tangents, packing, upload */
 } thoth_pathwise_result;
 int thoth_mcl_price_pathwise(const thoth_mcl_request* req, thoth_pathwise_result* res);
 
@@ -259,6 +289,20 @@ int thoth_mcl_grid(const thoth_mcl_request* req, int32_t cap, double* days, doub
  * count or a negative THOTH_* status. */
 int thoth_mcl_calibrate_leverage(const thoth_mcl_request* req, int32_t asset, double spot, int32_t cap_layers,
                                  double* t, double* ln_step, int64_t* offset, double* levels);
+
+/* No reference counterpart (host only, no device): the sensitivity of an LSV
+ * equity's calibrated leverage to its spot (which = 0, per unit of spot) or
+ * to the parallel vol shift (which = 1, per unit of shift) that the pathwise
+ * basket Greeks carry, on the request's diffusion dates: the leverage
+ * recalibrated at spot + 1 % (helpers/constants.hpp:36) or at vol shift + 1
+ * vol point (tasks/pricer.hpp:25), as the reference's bump scenarios
+ * recalibrate it, read at the nodes of the base per-date layers (points and
+ * offsets of thoth_mcl_calibrate_leverage's per-date layers) the way the
+ * sweep reads a layer, minus the base values, over the bump:
+ * dv[i * points + k] for node k of date i.  Writes at most cap_dates dates and
+ * returns the date count, or a negative THOTH_* status. */
+int thoth_mcl_leverage_tangent(const thoth_mcl_request* req, int32_t asset, int32_t which, int32_t cap_dates,
+                               double* dv);
 
 /* The cluster master's path split (reference: modes/run_cluster.cpp:308-339):
  * `total` paths cut into min(parts, total) disjoint Sobol blocks, the first

@@ -58,8 +58,11 @@ int thoth_gpu_device_info(char* buf, size_t len);
  *   F_T = Forward * exp(-0.5 Vol^2 T + Vol sqrt(T) Z), payoff max(+-(F_T-K),0),
  *   premium = Df * mean, trust = Df * sqrt(max(E[x^2]-mean^2,0) / Paths).
  * Z comes from the Joe-Kuo Sobol sequence through Acklam's inverse normal
- * (point index 1 + Seed*Paths, so a fixed Seed reproduces the same points and
- * bumps share them).  Paths <= 0 returns premium = trust = 0 as the reference. */
+ * (point indices 1 + Seed*Paths .. (Seed+1)*Paths while they stay below
+ * 2^32 - 1; for larger seeds, a negative seed wrapped to 2^64 - k included,
+ * points 0 .. Paths-1 under one random digital shift seeded by Seed), so a
+ * fixed Seed reproduces the same points and bumps share them.  Paths <= 0
+ * returns premium = trust = 0 as the reference. */
 int thoth_gpu_price_european_gbm(double forward, double strike, double t, double vol,
                                  double df, int is_call, long long paths, uint64_t seed,
                                  double* premium, double* trust);

@@ -981,6 +981,7 @@ Market BuildMarket( const thoth_mcl_request& req )
         e.rho = in.rho_spot_var;
         e.has_div = in.dividend_yield != 0;
         e.div = in.dividend_yield * 0.01;                        // curve.cpp:35-39
+        e.vol_shift = req.vol_shift;                             // scenario shift (Volatility::_vol_shift)
         for ( int i = 0; i < in.surface.n; i++ )
         {
             e.surface.maturities.push_back( in.surface.maturities[i] );
@@ -1181,8 +1182,9 @@ ResolvedBook ResolveBook( const thoth_mcl_request& req, const Market& mk, const 
         {
             int m = req.rainbows[r].members[j];
             rb.members.push_back( m );
-            // rainbow.cpp:56-57: weight 100 / reference spot captured at load (base spot)
-            rb.weight.push_back( 100.0 / req.assets[m].spot );
+            // rainbow.cpp:56-57: weight 100 / reference spot captured at load (base
+            // spot, or the caller's rebasing spot)
+            rb.weight.push_back( 100.0 / ( req.rebase_spots ? req.rebase_spots[m] : req.assets[m].spot ) );
         }
         b.rainbows.push_back( rb );
     }
@@ -1746,7 +1748,7 @@ int Price( const thoth_mcl_request& req, thoth_mcl_result* res )
     {
         Market mk = base;
         for ( Equity& e : mk.eq )
-            e.vol_shift = GREEK_VOL_BUMP;
+            e.vol_shift += GREEK_VOL_BUMP;
         const Scen s = run( mk, false );
         res->n_scenarios++;
         res->vega = ( s.book - p0 ) / GREEK_VOL_BUMP * 0.01;

@@ -61,6 +61,25 @@ def test_gbm_entry_point(gpu):
     assert gpu.price_european_gbm(F, K, T, v, 1.0, True, 0, 0) == (0.0, 0.0)
 
 
+@pytest.mark.parametrize("seed,paths", [(3, 1 << 16), ((1 << 64) - 1, 1 << 16), ((1 << 31) - 1, 1 << 16),
+                                        (1000, 10_000_000)])
+def test_gbm_entry_point_prices_every_seed(gpu, seed, paths):
+    """Every unsigned long seed prices (mcl_configuration.cpp:24 reads any int,
+    cast to unsigned long): a disjoint run of points while it fits below
+    2^32, a seeded digital shift past it (a negative seed wraps to 2^64 - k)."""
+    F, K, T, v, r = 100 * math.exp(0.03), 100.0, 1.0, 0.2, 0.03
+    df = math.exp(-r * T)
+    p, t = gpu.price_european_gbm(F, K, T, v, df, True, paths, seed)
+    assert abs(p - 9.4134033839) < 5 * t + 1e-3 and t > 0
+    assert gpu.price_european_gbm(F, K, T, v, df, True, paths, seed) == (p, t)  # a fixed point set per seed
+    if seed == 3:
+        # reference mode: points 1 + seed paths .. (seed + 1) paths
+        b = gpu.sweep([_c1_book(gpu)], paths, precision=abi.FP64, sobol_skip=seed * paths)[0]
+        assert p == pytest.approx(b.book_sum / paths, rel=1e-9, abs=0)  # the same points, discounted in the book
+    else:
+        assert p != gpu.price_european_gbm(F, K, T, v, df, True, paths, seed - 1)[0]
+
+
 def test_heston_down_out_fp64(gpu, oracle):
     spec = configs.heston_down_out_call(paths=1 << 12, precision=abi.FP64, greeks=abi.GREEK_DELTA | abi.GREEK_GAMMA | abi.GREEK_VEGA)
     compare(gpu, oracle, spec)

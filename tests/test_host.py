@@ -60,3 +60,46 @@ def test_request_shape_is_checked_before_use(name, change):
     assert e.value.code == abi.ERR_INVALID, name
     with pytest.raises(mcl.McError):
         mcl.calibrate_leverage(b, 0)
+
+
+def _layer_at(levels, ln_step, offset, x):
+    """LeverageSurface::LayerAt (leverage_surface.cpp:19-40) at log-spot x."""
+    f = x / ln_step - offset
+    last = levels.size - 1
+    if f <= 0:
+        return levels[0]
+    if f >= last:
+        return levels[last]
+    k = int(f)
+    w = f - k
+    return (1 - w) * levels[k] + w * levels[k + 1]
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_leverage_sensitivity_is_the_bump_recalibration_at_the_base_nodes(which):
+    """The leverage sensitivity the pathwise basket Greeks carry: the leverage
+    recalibrated at the reference's bump (+1 % spot, helpers/constants.hpp:36;
+    +1 vol point, tasks/pricer.hpp:25), read at the base layers' nodes the way
+    the sweep reads a layer, minus the base, over the bump -- here rebuilt from
+    two calibrations of the public entry point."""
+    spec = _lsv(days=20)
+    b = BuiltRequest(spec)
+    base = mcl.calibrate_leverage(b, 0)
+    dv = mcl.leverage_tangent(b, 0, which)
+    assert dv.shape == base["levels"].shape
+    if which == 0:
+        h = 0.01 * 100.0
+        up = mcl.calibrate_leverage(b, 0, spot=100.0 + h)
+    else:
+        h = 0.01
+        spec.vol_shift = h
+        up = mcl.calibrate_leverage(BuiltRequest(spec), 0)
+    # the bump moves the axis: the spot moves its centre, the vol shift its width
+    assert not np.array_equal(up["offset"], base["offset"]) or not np.array_equal(up["ln_step"], base["ln_step"])
+    for i in range(len(base["t"])):
+        lv, st, off = base["levels"][i], base["ln_step"][i], base["offset"][i]
+        for k in range(lv.size):
+            x = (k + off) * st
+            want = (_layer_at(up["levels"][i], up["ln_step"][i], up["offset"][i], x) - lv[k]) / h
+            assert abs(dv[i, k] - want) <= 1e-12 * (1 + abs(want)), (i, k)
+    assert np.abs(dv).max() > 0.01

@@ -71,7 +71,7 @@ class Request(C.Structure):
                 ("precision", _i32), ("correlate_stochastic", _i32), ("uniform_steps", _i32),
                 ("leverage_spot_points", _i32), ("leverage_time_layers", _i32),
                 ("discount", Curve), ("n_corr_pillars", _i32), ("corr_maturities", _pd),
-                ("corr_matrices", _pd)]
+                ("corr_matrices", _pd), ("vol_shift", _d), ("rebase_spots", _pd)]
 
 
 class Result(C.Structure):
@@ -173,4 +173,6 @@ class GridTuning(C.Structure):
 
 class PathwiseResult(C.Structure):
     _fields_ = [("premium", _d), ("premium_trust", _d), ("delta", _d), ("delta_trust", _d), ("gamma", _d),
-                ("vega", _d), ("vega_trust", _d), ("sweep_ms", _d)]
+                ("vega", _d), ("vega_trust", _d), ("sweep_ms", _d), ("n_assets", _i32),
+                ("asset_delta", _d * MAX_ASSETS), ("asset_delta_trust", _d * MAX_ASSETS),
+                ("asset_gamma", _d * MAX_ASSETS), ("n_books", _i32), ("setup_ms", _d)]

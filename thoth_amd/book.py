@@ -104,6 +104,11 @@ class BookSpec:
     # n x n matrix per pillar; empty: the constant `correlation`
     correlation_maturities: Sequence[float] = ()
     correlation_pillars: Sequence[Sequence[Sequence[float]]] = ()
+    # scenario inputs: a parallel volatility shift of every equity (decimal,
+    # the vega scenario's shift) and the rainbow rebasing spots S_i0 (None:
+    # the equities' spots)
+    vol_shift: float = 0.0
+    rebase_spots: Optional[Sequence[float]] = None
 
 
 class BuiltRequest:
@@ -194,6 +199,12 @@ class BuiltRequest:
             r.n_corr_pillars = npl
             r.corr_maturities = self.corr_mats
             r.corr_matrices = self.corr_pillars
+        r.vol_shift = spec.vol_shift
+        if spec.rebase_spots is not None:
+            if len(spec.rebase_spots) != n:
+                raise ValueError("one rebasing spot per equity")
+            self.rebase = (C.c_double * n)(*[float(x) for x in spec.rebase_spots])
+            r.rebase_spots = self.rebase
         self.req = r
 
     def result(self):

@@ -47,6 +47,8 @@ struct Offsets
 {
     size_t t, dt, sqrt_dt, bsq, binv, step_ops, op_b, op_left, op_w, op_wf, drift, sc, lev, levf, lnstep, offset,
         offsetf, invstep, mon, flow, cv_start, cv_end, cv_scale, grid_mat_at, grid_kf, grid_kd, chol, cholf;
+    size_t pw[4];  // leverage sensitivities: ds, de (FP64), then their FP32 copies
+    bool has_pw;
 };
 
 std::string Why( int b, const std::string& s ) { return "book " + std::to_string( b ) + ": " + s; }
@@ -412,12 +414,13 @@ static size_t LdsBytes( const DeviceBatch& B, int G )
 {
     // the pathwise kernel reads a Sobol block table; the sweep a block table
     // (not in the in-loop inference kernels) and a ring
+    // (the pathwise kernels read the leverage from device memory)
     size_t ring = 0;
     if ( !B.pathwise )
         for ( const std::vector<int>& so : B.step_ops )
             ring = std::max( ring, (size_t)RingSlots( so, G ) * B.nf_inst * 32 );
     const size_t table = B.pathwise ? B.lds_dims : B.cv_mode == 2 ? 0 : ( B.lds_dims + 3 ) & ~size_t( 3 );
-    const size_t words = table + ring + (size_t)G * B.lds_lev_words;
+    const size_t words = table + ring + ( B.pathwise ? 0 : (size_t)G * B.lds_lev_words );
     if ( !B.lds_tail )
         return words * sizeof( uint32_t );
     return ( ( words + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + B.lds_tail;
@@ -427,8 +430,10 @@ static size_t LdsBytes( const DeviceBatch& B, int G )
 // the hedge network runs in the loop
 static int SlotsOf( const DeviceBatch& B, int b )
 {
-    if ( B.pathwise )
+    if ( B.pathwise == 1 )
         return 3;  // survival-weighted payoff, pathwise delta, pathwise vega
+    if ( B.pathwise == 2 )
+        return 3 + B.db[b].n_assets;  // payoff, book delta, vega, per-equity deltas
     return 1 + B.db[b].n_contracts + ( B.cv_mode == 2 ? 1 : 0 );
 }
 
@@ -443,8 +448,8 @@ DeviceBatch::~DeviceBatch()
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv, const GridSpec* grids, bool pathwise, char* ext_base, size_t ext_cap,
-                  const RecordSpec* rec )
+                  const CvSpec* cv, const GridSpec* grids, int pathwise, char* ext_base, size_t ext_cap,
+                  const RecordSpec* rec, const PathwiseLayers* pwl )
 {
     B.pathwise = pathwise;
     int rc = EnsureDevice();
@@ -485,6 +490,22 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         rc = PackBook( b, books[b], B.db[b], of[b], ar, B.depth, B.step_ops[b] );
         if ( rc != THOTH_OK )
             return rc;
+        of[b].has_pw = false;
+        if ( pwl && pathwise == 2 && B.db[b].lev_points > 0 )
+        {
+            // leverage sensitivities, zeros where the caller gives none
+            const size_t n = (size_t)B.db[b].n_assets * B.db[b].n_dates * B.db[b].lev_points;
+            const double* src[2] = { pwl[b].ds, pwl[b].de };
+            for ( int k = 0; k < 2; k++ )
+            {
+                std::vector<double> v( n, 0.0 );
+                if ( src[k] )
+                    std::copy( src[k], src[k] + n, v.begin() );
+                of[b].pw[k] = ar.put( v );
+                of[b].pw[2 + k] = ar.put( std::vector<float>( v.begin(), v.end() ) );
+            }
+            of[b].has_pw = true;
+        }
         if ( cv && cv->J > 0 )
         {
             if ( cfg->precision != 1 || n_books != 1 )
@@ -788,6 +809,13 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
             d.lev_offset = At<double>( dbase, o.offset );
             d.lev_offset_f = At<float>( dbase, o.offsetf );
             d.lev_inv_step = At<float>( dbase, o.invstep );
+        }
+        if ( o.has_pw )
+        {
+            d.lev_ds = At<double>( dbase, o.pw[0] );
+            d.lev_de = At<double>( dbase, o.pw[1] );
+            d.lev_dsf = At<float>( dbase, o.pw[2] );
+            d.lev_def = At<float>( dbase, o.pw[3] );
         }
         d.mon_mask = At<uint32_t>( dbase, o.mon );
         d.flow_mask = At<uint32_t>( dbase, o.flow );

@@ -107,6 +107,13 @@ struct DevBook
     const double* lev_offset;   // [n_assets * n_dates] (integral values)
     const float* lev_offset_f;  // FP32 copy (exact: |offset| < 2^24), date-major [n_dates][n_assets]
     const float* lev_inv_step;  // [n_dates][n_assets] 1 / ln_step
+    // pathwise basket Greeks: sensitivity of the LSV leverage layers to the
+    // equity's own spot (s) and to the parallel vol shift (e) at each node,
+    // [n_assets * n_dates * lev_points] (null when the book has none)
+    const double* lev_ds;
+    const double* lev_de;
+    const float* lev_dsf;
+    const float* lev_def;
     // payoffs
     int c_kind[kMaxContracts], c_call[kMaxContracts], c_udl[kMaxContracts];
     int c_up[kMaxContracts], c_in[kMaxContracts];

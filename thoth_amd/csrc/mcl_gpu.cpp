@@ -382,10 +382,23 @@ __attribute__( ( visibility( "default" ) ) ) int thoth_gpu_price_european_gbm( d
     b.c_level = &zero;
     b.c_mon_offset = mon_off;
     b.c_mon_index = nullptr;
+    // The reference prices every unsigned long seed (pricer/src/tasks/mcl_configuration.cpp:24).
+    // While points 1 + Seed Paths .. (Seed + 1) Paths stay below 2^32 - 1 the
+    // seed selects that disjoint run of Sobol points; past it (large seeds,
+    // a negative YAML seed wrapped to 2^64 - k) the seed instead seeds one
+    // random digital shift of points 0 .. Paths - 1: still a fixed,
+    // reproducible point set per seed, shared by every bump.
     thoth_mcl_sweep_cfg c{};
     c.paths = paths;
-    c.sobol_skip = (int64_t)( seed * (uint64_t)paths );
     c.precision = 0;
+    const unsigned __int128 end = ( (unsigned __int128)seed + 1u ) * (unsigned __int128)(uint64_t)paths;
+    if ( end < 0xffffffffull )
+        c.sobol_skip = (int64_t)( seed * (uint64_t)paths );
+    else
+    {
+        c.replicates = 1;
+        c.rqmc_seed = seed;
+    }
     thoth_mcl_book_out o{};
     int rc = thoth_mcl_sweep( 1, &b, &c, &o );
     if ( rc != THOTH_OK )

@@ -331,7 +331,7 @@ int PrepareGridPlan( int n, const thoth_mcl_request* reqs, const thoth_vanilla_g
         {
             Market m = base;
             for ( Equity& e : m.eq )
-                e.vol_shift = kVolBump;
+                e.vol_shift += kVolBump;
             mks.push_back( m );
             kinds.push_back( 4 );
         }

@@ -10,8 +10,12 @@
 // the reference does.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <functional>
 #include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <set>
@@ -205,8 +209,12 @@ void Resolve( const thoth_mcl_request& req, const Market& mk, const Grid& g, con
             if ( m < 0 || m >= n )
                 throw Invalid( THOTH_ERR_INVALID, "rainbow member index" );
             A.rb_member.push_back( m );
-            // rainbow.cpp:56-57: 100 / reference spot captured at load (the unbumped spot)
-            A.rb_weight.push_back( j < rb.n ? 100.0 / req.assets[m].spot : 0.0 );
+            // rainbow.cpp:56-57: 100 / reference spot captured at load (the unbumped
+            // spot, or the caller's rebasing spot)
+            const double s0 = req.rebase_spots ? req.rebase_spots[m] : req.assets[m].spot;
+            if ( j < rb.n && !( s0 > 0 ) )
+                throw Invalid( THOTH_ERR_INVALID, "rainbow rebasing spot must be positive" );
+            A.rb_weight.push_back( j < rb.n ? 100.0 / s0 : 0.0 );
         }
     }
     auto index_of = [&]( double day )
@@ -415,7 +423,7 @@ int PreparePlan( const thoth_mcl_request& req, Plan& plan )
     {
         Scenario s{ base, 4, -1, kVolBump };
         for ( Equity& e : s.mk.eq )
-            e.vol_shift = kVolBump;
+            e.vol_shift += kVolBump;
         sc.push_back( s );
     }
     if ( req.request_greeks & THOTH_GREEK_RHO )
@@ -835,24 +843,94 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_price_rqmc
 namespace thoth::host
 {
 
+namespace
+{
+
+// Run independent host tasks (each may use threads of its own) on a few
+// threads; the first exception is rethrown after every task has ended.
+void RunConcurrently( std::vector<std::function<void()>>& tasks )
+{
+    const size_t nt = std::min<size_t>( tasks.size(), std::max( 1u, std::min( 8u, std::thread::hardware_concurrency() ) ) );
+    std::atomic<size_t> next{ 0 };
+    std::exception_ptr err;
+    std::mutex mu;
+    auto worker = [&] {
+        for ( size_t i; ( i = next.fetch_add( 1 ) ) < tasks.size(); )
+            try
+            {
+                tasks[i]();
+            }
+            catch ( ... )
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                if ( !err )
+                    err = std::current_exception();
+            }
+    };
+    std::vector<std::thread> th;
+    for ( size_t t = 1; t < nt; t++ )
+        th.emplace_back( worker );
+    worker();
+    for ( auto& t : th )
+        t.join();
+    if ( err )
+        std::rethrow_exception( err );
+}
+
+// Kahan sums over blocks in block order of every estimator of every book
+// ([books][ne][sum, sum2]) from the per-block partials of a pathwise batch
+std::vector<std::vector<double>> BlockSums( const DeviceBatch& B, const std::vector<double>& part, int nbooks, int ne )
+{
+    std::vector<std::vector<double>> sums( nbooks, std::vector<double>( 2 * ne, 0.0 ) );
+    for ( int b = 0; b < nbooks; b++ )
+    {
+        std::vector<double> cp( 2 * ne, 0.0 );
+        std::vector<double>& sm = sums[b];
+        for ( uint32_t k = 0; k < B.nblocks; k++ )
+            for ( int j = 0; j < 2 * ne; j++ )
+            {
+                const double y = part[B.part_off[b] + (size_t)k * 2 * ne + j] - cp[j];
+                const double t = sm[j] + y;
+                cp[j] = ( t - sm[j] ) - y;
+                sm[j] = t;
+            }
+    }
+    return sums;
+}
+
+// launch one pathwise batch on the library stream: device time (ms) and the partials
+int RunPathwise( const DeviceBatch& B, int kernel, std::vector<double>& part, float& ms )
+{
+    hipStream_t s = Stream();
+    hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
+    (void)hipEventRecord( e0, s );
+    const dim3 grid( B.nblocks, B.n_books );
+    const bool ok = kernel == 1 ? launch_pathwise( B.precision, B.depth, grid, s, B.lds_bytes, B.dbooks,
+                                                   DeviceDirections(), B.q0, B.first, B.last )
+                                : launch_pathwise_basket( B.precision, B.na_max, B.depth, grid, s, B.lds_bytes,
+                                                          B.dbooks, DeviceDirections(), B.q0, B.first, B.last );
+    if ( !ok )
+        return SetError( THOTH_ERR_UNSUPPORTED, "bridge deeper than the pathwise kernels support" );
+    if ( hipGetLastError() != hipSuccess )
+        return SetError( THOTH_ERR_DEVICE, "pathwise launch failed" );
+    (void)hipEventRecord( e1, s );
+    part.assign( B.part_total, 0.0 );
+    if ( hipMemcpyAsync( part.data(), B.dpart, part.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) !=
+             hipSuccess ||
+         hipStreamSynchronize( s ) != hipSuccess )
+        return SetError( THOTH_ERR_DEVICE, "pathwise copy failed" );
+    ms = 0;
+    (void)hipEventElapsedTime( &ms, e0, e1 );
+    return THOTH_OK;
+}
+
+}  // namespace
+
 // single-equity BS / Heston book with one vanilla or knock-out contract:
 // base and (gamma) +-10 % spot books swept by the pathwise kernel
-int PricePathwise( const thoth_mcl_request& req, thoth_pathwise_result* res )
+int PricePathwiseSingle( const thoth_mcl_request& req, thoth_pathwise_result* res )
 {
-    if ( req.n_assets != 1 || !req.assets || req.n_contracts != 1 || !req.contracts )
-        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: one equity and one contract" );
-    if ( req.contracts[0].american )
-        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: european exercise" );
-    const thoth_asset& a0 = req.assets[0];
-    if ( a0.vol_kind == THOTH_VOL_LSV )
-        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: Black-Scholes or Heston equities" );
-    const thoth_contract& k0 = req.contracts[0];
-    if ( k0.kind == THOTH_BARRIER && ( k0.barrier_type == THOTH_UP_IN || k0.barrier_type == THOTH_DOWN_IN ) )
-        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: knock-out barriers (knock-in = vanilla - knock-out)" );
-    if ( k0.underlying != 0 )
-        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: the contract's underlying is the equity" );
-    if ( req.mcl.paths <= 1 || req.mcl.paths > 200000000 || !req.mcl.use_sobol )
-        throw Invalid( THOTH_ERR_INVALID, "paths in (1, 2e8], Sobol mode" );
+    const auto t0 = std::chrono::steady_clock::now();
     const Market base = BuildMarket( req );
     const Grid g = BuildGrid( req, base );
     std::vector<Market> mks{ base };
@@ -878,45 +956,17 @@ int PricePathwise( const thoth_mcl_request& req, thoth_pathwise_result* res )
     cfg.sobol_skip = req.mcl.sobol_skip;
     cfg.precision = req.precision;
     DeviceBatch B;
-    int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, B, nullptr, nullptr, true );
+    int rc = PrepareBatch( (int32_t)books.size(), books.data(), &cfg, B, nullptr, nullptr, 1 );
     if ( rc != THOTH_OK )
         return rc;
-    hipStream_t s = Stream();
-    hipEvent_t e0, e1;
-    if ( hipEventCreate( &e0 ) != hipSuccess || hipEventCreate( &e1 ) != hipSuccess )
-        return SetError( THOTH_ERR_DEVICE, "hipEventCreate" );
-    (void)hipEventRecord( e0, s );
-    if ( !launch_pathwise( B.precision, B.depth, dim3( B.nblocks, B.n_books ), s, B.lds_bytes, B.dbooks,
-                           DeviceDirections(), B.q0, B.first, B.last ) )
-        return SetError( THOTH_ERR_UNSUPPORTED, "bridge deeper than the pathwise kernels support" );
-    if ( hipGetLastError() != hipSuccess )
-        return SetError( THOTH_ERR_DEVICE, "pathwise launch failed" );
-    (void)hipEventRecord( e1, s );
-    std::vector<double> part( B.part_total );
-    if ( hipMemcpyAsync( part.data(), B.dpart, part.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) !=
-             hipSuccess ||
-         hipStreamSynchronize( s ) != hipSuccess )
-        return SetError( THOTH_ERR_DEVICE, "pathwise copy failed" );
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<double> part;
     float ms = 0;
-    (void)hipEventElapsedTime( &ms, e0, e1 );
-    (void)hipEventDestroy( e0 );
-    (void)hipEventDestroy( e1 );
-    // Kahan over blocks in block order, per book: [Y, delta, vega] x [sum, sum2]
+    if ( ( rc = RunPathwise( B, 1, part, ms ) ) != THOTH_OK )
+        return rc;
+    // [Y, delta, vega] x [sum, sum2] per book
     const double n = (double)req.mcl.paths;
-    std::vector<std::array<double, 6>> sums( books.size() );
-    for ( size_t b = 0; b < books.size(); b++ )
-    {
-        std::array<double, 6> sm{}, cp{};
-        for ( uint32_t k = 0; k < B.nblocks; k++ )
-            for ( int j = 0; j < 6; j++ )
-            {
-                const double y = part[B.part_off[b] + (size_t)k * 6 + j] - cp[j];
-                const double t = sm[j] + y;
-                cp[j] = ( t - sm[j] ) - y;
-                sm[j] = t;
-            }
-        sums[b] = sm;
-    }
+    const auto sums = BlockSums( B, part, (int)books.size(), 3 );
     auto mean = [&]( int b, int e ) { return sums[b][2 * e] / n; };
     auto trust = [&]( int b, int e ) {
         const double s1 = sums[b][2 * e], s2 = sums[b][2 * e + 1];
@@ -929,8 +979,166 @@ int PricePathwise( const thoth_mcl_request& req, thoth_pathwise_result* res )
     res->vega = mean( 0, 2 );
     res->vega_trust = trust( 0, 2 );
     res->gamma = gamma ? ( mean( 1, 1 ) - mean( 2, 1 ) ) / ( 2 * h ) : 0.0;
+    res->n_assets = 1;
+    res->asset_delta[0] = res->delta;
+    res->asset_delta_trust[0] = res->delta_trust;
+    res->asset_gamma[0] = res->gamma;
+    res->n_books = (int32_t)books.size();
     res->sweep_ms = ms;
+    res->setup_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count();
     return THOTH_OK;
+}
+
+// books of 1..8 BS / Heston / LSV equities with one european vanilla on an
+// equity or a rainbow: the base book and, for gamma, the +-10 % books of every
+// equity (leverage recalibrated at the bumped spot, as the reference's bump
+// scenario rebuilds it), swept by the pathwise basket kernel with the LSV
+// leverage tangents of each book
+int PricePathwiseBasket( const thoth_mcl_request& req, thoth_pathwise_result* res )
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = req.n_assets;
+    const Market base = BuildMarket( req );
+    const Grid g = BuildGrid( req, base );
+    const std::vector<double> chol = SpotCholesky( req, base );
+    const bool gamma = ( req.request_greeks & THOTH_GREEK_GAMMA ) != 0;
+    std::vector<Market> mks{ base };
+    std::vector<int> bumped{ -1 };
+    std::vector<double> hs( n, 0.0 );
+    if ( gamma )
+        for ( int a = 0; a < n; a++ )
+        {
+            hs[a] = kGammaBump * base.eq[a].spot;
+            Market u = base, d = base;
+            u.eq[a].spot += hs[a];
+            d.eq[a].spot -= hs[a];
+            mks.push_back( u );
+            bumped.push_back( a );
+            mks.push_back( d );
+            bumped.push_back( a );
+        }
+    const size_t nb = mks.size(), nd = g.t.size();
+    // leverage per (book, equity): its own calibration where the book moved
+    // the equity (every equity of the base book), else the base book's; the
+    // calibrations are independent and run concurrently
+    std::vector<std::vector<LeverageGrid>> levs( nb, std::vector<LeverageGrid>( n ) );
+    auto own = [&]( size_t k, int a ) { return mks[k].eq[a].kind == THOTH_VOL_LSV && ( k == 0 || a == bumped[k] ); };
+    {
+        std::vector<std::function<void()>> tasks;
+        for ( size_t k = 0; k < nb; k++ )
+            for ( int a = 0; a < n; a++ )
+                if ( own( k, a ) )
+                    tasks.push_back( [&, k, a] {
+                        if ( CalibrateLeverage( mks[k], a, g, levs[k][a] ) != THOTH_OK )
+                            throw Invalid( THOTH_ERR_INVALID, "leverage calibration failed" );
+                    } );
+        RunConcurrently( tasks );
+        for ( size_t k = 1; k < nb; k++ )
+            for ( int a = 0; a < n; a++ )
+                if ( mks[k].eq[a].kind == THOTH_VOL_LSV && !own( k, a ) )
+                    levs[k][a] = levs[0][a];
+    }
+    std::vector<BookArrays> arrays( nb );
+    std::vector<thoth_mcl_book> books;
+    for ( size_t k = 0; k < nb; k++ )
+    {
+        Resolve( req, mks[k], g, chol, levs[k], arrays[k] );
+        books.push_back( arrays[k].book );
+    }
+    // leverage sensitivities: to the spot (every equity of the base book, the
+    // bumped equity of a gamma book) and to the vol shift (base book), zeros
+    // elsewhere (those Greeks are not read from those books)
+    std::vector<std::vector<double>> ds( nb ), de( nb );
+    std::vector<PathwiseLayers> pwl( nb );
+    std::vector<std::function<void()>> tasks;
+    for ( size_t k = 0; k < nb; k++ )
+    {
+        const int P = arrays[k].book.lev_points;
+        if ( P <= 0 )
+            continue;
+        ds[k].assign( (size_t)n * nd * P, 0.0 );
+        de[k].assign( (size_t)n * nd * P, 0.0 );
+        pwl[k] = { ds[k].data(), de[k].data() };
+        for ( int a = 0; a < n; a++ )
+        {
+            if ( !own( k, a ) )
+                continue;
+            if ( levs[k][a].points != P )
+                throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: LSV equities with one leverage layout" );
+            for ( int which = 0; which < ( k == 0 ? 2 : 1 ); which++ )
+                tasks.push_back( [&, k, a, which, P] {
+                    std::vector<double> dv;
+                    LeverageTangent( mks[k], a, g, levs[k][a], which, dv );
+                    std::copy( dv.begin(), dv.end(), ( which == 0 ? ds[k] : de[k] ).begin() + (size_t)a * nd * P );
+                } );
+        }
+    }
+    RunConcurrently( tasks );
+    thoth_mcl_sweep_cfg cfg{};
+    cfg.paths = req.mcl.paths;
+    cfg.sobol_skip = req.mcl.sobol_skip;
+    cfg.precision = req.precision;
+    DeviceBatch B;
+    int rc = PrepareBatch( (int32_t)nb, books.data(), &cfg, B, nullptr, nullptr, 2, nullptr, 0, nullptr, pwl.data() );
+    if ( rc != THOTH_OK )
+        return rc;
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<double> part;
+    float ms = 0;
+    if ( ( rc = RunPathwise( B, 2, part, ms ) ) != THOTH_OK )
+        return rc;
+    // [Y, sum of deltas, vega, delta_0 .. delta_{n-1}] x [sum, sum2] per book
+    const int ne = 3 + n;
+    const double N = (double)req.mcl.paths;
+    const auto sums = BlockSums( B, part, (int)nb, ne );
+    auto mean = [&]( size_t b, int e ) { return sums[b][2 * e] / N; };
+    auto trust = [&]( size_t b, int e ) {
+        const double s1 = sums[b][2 * e], s2 = sums[b][2 * e + 1];
+        return std::sqrt( std::max( 0.0, ( s2 - s1 * s1 / N ) / ( N - 1 ) / N ) );
+    };
+    res->premium = mean( 0, 0 );
+    res->premium_trust = trust( 0, 0 );
+    res->delta = mean( 0, 1 );
+    res->delta_trust = trust( 0, 1 );
+    res->vega = mean( 0, 2 );
+    res->vega_trust = trust( 0, 2 );
+    res->gamma = 0;
+    res->n_assets = n;
+    for ( int a = 0; a < n; a++ )
+    {
+        res->asset_delta[a] = mean( 0, 3 + a );
+        res->asset_delta_trust[a] = trust( 0, 3 + a );
+        // books 1 + 2a (up) and 2 + 2a (down) of equity a
+        res->asset_gamma[a] = gamma ? ( mean( 1 + 2 * a, 3 + a ) - mean( 2 + 2 * a, 3 + a ) ) / ( 2 * hs[a] ) : 0.0;
+        res->gamma += res->asset_gamma[a];
+    }
+    res->n_books = (int32_t)nb;
+    res->sweep_ms = ms;
+    res->setup_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count();
+    return THOTH_OK;
+}
+
+int PricePathwise( const thoth_mcl_request& req, thoth_pathwise_result* res )
+{
+    if ( req.n_assets < 1 || req.n_assets > THOTH_MCL_MAX_ASSETS || !req.assets || req.n_contracts != 1 ||
+         !req.contracts )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: 1..8 equities and one contract" );
+    if ( req.contracts[0].american )
+        throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: european exercise" );
+    if ( req.mcl.paths <= 1 || req.mcl.paths > 200000000 || !req.mcl.use_sobol )
+        throw Invalid( THOTH_ERR_INVALID, "paths in (1, 2e8], Sobol mode" );
+    *res = thoth_pathwise_result{};
+    const thoth_contract& k0 = req.contracts[0];
+    const bool single = req.n_assets == 1 && req.assets[0].vol_kind != THOTH_VOL_LSV && k0.underlying == 0;
+    if ( k0.kind == THOTH_BARRIER )
+    {
+        if ( !single )
+            throw Invalid( THOTH_ERR_UNSUPPORTED,
+                           "pathwise Greeks of barrier contracts: single-equity Black-Scholes or Heston books" );
+        if ( k0.barrier_type == THOTH_UP_IN || k0.barrier_type == THOTH_DOWN_IN )
+            throw Invalid( THOTH_ERR_UNSUPPORTED, "pathwise Greeks: knock-out barriers (knock-in = vanilla - knock-out)" );
+    }
+    return single ? PricePathwiseSingle( req, res ) : PricePathwiseBasket( req, res );
 }
 
 }  // namespace thoth::host
@@ -1065,6 +1273,41 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_grid( cons
             dt[i] = g.dt[i];
         }
         return (int)g.t.size();
+    }
+    catch ( const thoth::host::Invalid& e )
+    {
+        return -thoth::SetError( e.code, e.what() );
+    }
+    catch ( const std::exception& e )
+    {
+        return -thoth::SetError( THOTH_ERR_INVALID, e.what() );
+    }
+}
+
+extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_leverage_tangent(
+    const thoth_mcl_request* req, int32_t asset, int32_t which, int32_t cap_dates, double* dv )
+{
+    if ( !req || asset < 0 || asset >= req->n_assets || ( which != 0 && which != 1 ) || !dv )
+        return -thoth::SetError( THOTH_ERR_INVALID, "asset index, tangent kind or null output" );
+    try
+    {
+        const thoth::host::Market mk = thoth::host::BuildMarket( *req );
+        if ( mk.eq[asset].kind != THOTH_VOL_LSV )
+            return -thoth::SetError( THOTH_ERR_INVALID, "equity is not LSV" );
+        const thoth::host::Grid g = thoth::host::BuildGrid( *req, mk );
+        thoth::host::LeverageGrid L;
+        int rc = thoth::host::CalibrateLeverage( mk, asset, g, L );
+        if ( rc != THOTH_OK )
+            return -rc;
+        std::vector<double> d;
+        thoth::host::LeverageTangent( mk, asset, g, L, which, d );
+        const int nd = (int)g.t.size();
+        for ( int i = 0; i < nd && i < cap_dates; i++ )
+        {
+            std::copy( d.begin() + (size_t)i * L.points, d.begin() + (size_t)( i + 1 ) * L.points,
+                       dv + (size_t)i * L.points );
+        }
+        return nd;
     }
     catch ( const thoth::host::Invalid& e )
     {

@@ -134,6 +134,13 @@ bool CholeskyLower( std::vector<double>& a, int n );  // maths.cpp:354-381
 // Invalid when the matrix is missing or not positive-definite
 std::vector<double> SpotCholesky( const thoth_mcl_request& req, const Market& mk );
 int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& out );
+// Sensitivity of equity a's calibrated leverage to its spot (which = 0, per
+// unit of spot) or to the parallel vol shift (which = 1, per unit of shift),
+// as the reference's bump sees it: the leverage recalibrated at spot + 1 % or
+// at vol shift + 1 vol point, read at the base layers' nodes, minus the base,
+// over the bump.  dv[i P + k] for node k of date i's layer.
+void LeverageTangent( const Market& mk, int a, const Grid& g, const LeverageGrid& L0, int which,
+                      std::vector<double>& dv );
 // the base scenario's device book (grid, Cholesky, LSV calibration, payoffs);
 // throws on invalid input like the other host entry points
 int ResolveBaseBook( const thoth_mcl_request& req, BookArrays& A, Grid& g );

@@ -62,7 +62,9 @@ struct DeviceBatch
     int kind = 0;  // uniform-book kernel: 1 all LSV, 2 all Heston (see sweep_block), 0 generic
     int cv_mode = 0;  // 0 none, 1 control-variate samples, 2 in-loop hedge inference
     int cv_act = 0;   // hidden activation of the in-loop network: THOTH_CV_SILU or THOTH_CV_GATE
-    bool pathwise = false;  // pathwise-Greek sweep: three estimators per block
+    // pathwise-Greek sweep: 1 the single-equity kernel (three estimators per
+    // block), 2 the basket kernel (3 + n_assets estimators per block)
+    int pathwise = 0;
     // vanilla grids: per-(replicate, block) partials and their per-book sums
     // [nm][2 nk][2] (device), offsets per book; grid_cells[b] = nm * 2 nk
     double* dgrid = nullptr;
@@ -152,12 +154,20 @@ struct RecordSpec
 {
     int udl = -1, n = 0;
 };
+// Leverage sensitivities of one book for the pathwise basket kernel (host
+// arrays laid out as DevBook::lev_ds / lev_de; null for zeros)
+struct PathwiseLayers
+{
+    const double* ds = nullptr;
+    const double* de = nullptr;
+};
 // ext_base / ext_cap: device memory of the caller to hold the whole batch
 // (the no-allocation entry point's scratch) instead of a hipMalloc; a batch
 // that needs more than ext_cap bytes is refused before any device work
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
-                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, bool pathwise = false,
-                  char* ext_base = nullptr, size_t ext_cap = 0, const RecordSpec* rec = nullptr );
+                  const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, int pathwise = 0,
+                  char* ext_base = nullptr, size_t ext_cap = 0, const RecordSpec* rec = nullptr,
+                  const PathwiseLayers* pwl = nullptr );
 int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
 // Launch shape of a plan's batch (reference-mode sweeps without the control
 // variate): steps per LDS barrier G and workgroups per CU of the persistent

@@ -115,6 +115,12 @@ bool launch_pathwise( int precision, int depth, dim3 grid, hipStream_t s, size_t
                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 bool launch_pathwise_f32( int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,
                           uint32_t bq0, uint32_t first, uint32_t last );
+// the pathwise basket kernel (3 + n_assets estimators per block); false when
+// no instantiation covers the equity count or bridge depth
+bool launch_pathwise_basket( int precision, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
+                             const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
+bool launch_pathwise_basket_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                                 const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slot, double* out );
 void launch_grid_sum( hipStream_t s, const double* part, const long long* table, int n_books, int max_per_slot,
                       int nslots, double* out );

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -277,6 +278,7 @@ Market BuildMarket( const thoth_mcl_request& req )
         e.rho = in.rho_spot_var;
         e.has_div = in.dividend_yield != 0;
         e.div = in.dividend_yield * 0.01;               // curve.cpp:35-39
+        e.vol_shift = req.vol_shift;                    // scenario shift (Volatility::_vol_shift)
         for ( int i = 0; i < in.surface.n; i++ )
         {
             e.surface.mats.push_back( in.surface.maturities[i] );
@@ -516,23 +518,27 @@ double Acklam( double P )
 // scenario (a fresh Rng(LSV_CALIB_SEED) per calibration, single.cpp:415): it
 // is generated once and cached.  Entry 2*(P*(k-1)+p) is the variance normal of
 // particle p into date k, the next entry its spot normal (single.cpp:465-466).
+// Calibrations may run concurrently: a longer table is built beside the
+// current one, and every table built stays allocated, so a reference handed
+// out earlier stays valid.
 const std::vector<double>& CalibNormals( size_t steps )
 {
-    static std::vector<double> z;
+    static std::vector<std::unique_ptr<std::vector<double>>> tables;
     static std::mutex mu;
     std::lock_guard<std::mutex> lk( mu );
     const size_t need = steps * 2 * kParticles;
-    if ( z.size() < need )
+    if ( tables.empty() || tables.back()->size() < need )
     {
+        auto z = std::make_unique<std::vector<double>>( need );
         Xoshiro r( kCalibSeed );
-        z.resize( need );
         for ( size_t i = 0; i < need; i++ )
         {
             const double u = ( r.next() >> 11 ) * ( 1.0 / 9007199254740992.0 );
-            z[i] = Acklam( u > 0.0 ? u : ( 1.0 / 9007199254740992.0 ) );
+            ( *z )[i] = Acklam( u > 0.0 ? u : ( 1.0 / 9007199254740992.0 ) );
         }
+        tables.push_back( std::move( z ) );
     }
-    return z;
+    return *tables.back();
 }
 
 double QeStep( double V, double Dt, double Z, const StochParams& H )  // single.cpp:346-365
@@ -793,6 +799,48 @@ int CalibrateLeverage( const Market& mk, int a, const Grid& g, LeverageGrid& lev
             out[j] = ( 1.0 - w ) * layers[lo * P + j] + w * layers[hi * P + j];
     }
     return THOTH_OK;
+}
+
+void LeverageTangent( const Market& mk, int a, const Grid& g, const LeverageGrid& L0, int which,
+                      std::vector<double>& dv )
+{
+    // the reference's bumps: +1 % of the spot (helpers/constants.hpp:36), +1
+    // vol point (tasks/pricer.hpp:25), each recalibrating the leverage
+    const double h = which == 0 ? kSpotBump * mk.eq[a].spot : kVolBump;
+    Market up = mk;
+    if ( which == 0 )
+        up.eq[a].spot += h;
+    else
+        up.eq[a].vol_shift += h;
+    LeverageGrid Lu;
+    if ( CalibrateLeverage( up, a, g, Lu ) != THOTH_OK || Lu.points != L0.points )
+        throw Invalid( THOTH_ERR_INVALID, "leverage tangent: calibration failed" );
+    const int P = L0.points, last = P - 1;
+    const size_t nd = g.t.size();
+    dv.assign( nd * P, 0.0 );
+    for ( size_t i = 0; i < nd; i++ )
+    {
+        const double* lu = &Lu.values[i * P];
+        for ( int k = 0; k < P; k++ )
+        {
+            // the bumped layer read at the base layer's node k, as the sweep
+            // reads a layer (LeverageSurface::LayerAt, leverage_surface.cpp:19-40)
+            const double x = (double)( k + L0.offset[i] ) * L0.ln_step[i];
+            const double f = x / Lu.ln_step[i] - (double)Lu.offset[i];
+            double v;
+            if ( f <= 0.0 )
+                v = lu[0];
+            else if ( f >= (double)last )
+                v = lu[last];
+            else
+            {
+                const int j = (int)f;
+                const double w = f - (double)j;
+                v = ( 1.0 - w ) * lu[j] + w * lu[j + 1];
+            }
+            dv[i * P + k] = ( v - L0.values[i * P + k] ) / h;
+        }
+    }
 }
 
 }  // namespace thoth::host

@@ -411,4 +411,418 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pathwise delta and vega of a book of NA Black-Scholes, Heston or LSV
+// equities (correlated as the sweep correlates them) holding one european
+// vanilla on an equity or on a best-of / worst-of rainbow.  The path is the
+// generic sweep's (the same arithmetic, so the premium estimator is the
+// sweep's to rounding); each equity carries the forward tangents of its log
+// spot x_a with respect to its own spot (gd) and to the parallel vol shift e
+// (ge), and a Heston equity the vega tangent of its variance (dv).  Equities
+// depend on each other only through the noise, so d x_a / d S0_b = 0 for
+// a != b and the per-equity deltas come from one tangent each.
+// LSV: the step x' = x + cd + K0 + K1 vp + K2 vc + sqrt(K3 (vp + vc)) z with
+// K0..K3 linear and quadratic in the leverage L(t_i, x), so
+//   dx' = dx + G dL,  G = dK0/dL + dK1/dL vp + dK2/dL vc + sqrt(var) / L z,
+//   dL = L_x dx + L_theta,
+// L_x the slope of the layer in log-spot and L_theta the leverage function's
+// own response at fixed x to the spot or the vol shift (the calibration's,
+// read linearly between the layer's nodes from DevBook::lev_ds / lev_de).
+// Heston vega: as pathwise_kernel, through the QE variance map.
+// Black-Scholes: d expo / d vol = -vol dt + dW.
+// Payoff: vanilla on s = S_u, or s = the rainbow's extreme w_j S_j (weights
+// fixed at the rebasing spots, rainbow.cpp:54-57); d f / d x_j* = f' w S_j*.
+// Per block: [Y, sum_a delta_a, vega, delta_0 .. delta_{na-1}] x [sum, sum2].
+// ---------------------------------------------------------------------------
+template <class R, int NA, int DEPTH>
+__global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBook* __restrict__ books,
+                                                                   const uint32_t* __restrict__ V, uint32_t bq0,
+                                                                   uint32_t first_idx, uint32_t last_idx )
+{
+    constexpr bool F64 = std::is_same_v<R, double>;
+    constexpr int NF = 2 * NA;
+    typedef R facv __attribute__( ( ext_vector_type( NF ) ) );
+    const DevBook& B = books[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const uint32_t bq = bq0 + blockIdx.x;
+    const uint32_t idx = bq * 256u + (uint32_t)threadIdx.x;
+    const bool valid = idx >= first_idx && idx <= last_idx;
+    const int m = B.m, nd = B.n_dates, na = B.n_assets;
+
+    extern __shared__ uint32_t lds[];
+    __shared__ double acc[kWavesPerBlock][3 + kMaxAssets][2];
+    sobol_block_table( lds, V, m * B.factors, bq, nullptr );
+    __syncthreads();
+    SobolLane sob;
+    sob.V = V;
+    sob.tab = lds;
+    sob.set_lane( wib, lane );
+    BridgeWalk<R, NF, DEPTH> bw;
+    bw.start( B, sob );
+
+    const int udl = B.c_udl[0];
+    const R K = (R)B.c_strike[0];
+    const bool call = B.c_call[0] != 0;
+    const int P = B.lev_points;
+    const int last = P - 1;
+
+    R S[NA], Var[NA], W[NA], gd[NA], ge[NA], dv[NA], dl[NA];
+#pragma unroll
+    for ( int a = 0; a < NA; a++ )
+    {
+        S[a] = Var[a] = W[a] = gd[a] = ge[a] = dv[a] = dl[a] = R( 0 );
+        if ( a < na )
+        {
+            S[a] = (R)B.asset[a].spot;
+            Var[a] = (R)B.asset[a].v0;
+            gd[a] = R( 1 ) / (R)B.asset[a].spot;
+            // Heston vega seeds d v0 / de = 2 sqrt(v0) (heston_volatility.hpp:45-47)
+            dv[a] = B.asset[a].model == 1 ? R( 2 ) * sqrtr( (R)B.asset[a].v0 ) : R( 0 );
+        }
+    }
+    R y = R( 0 ), vg = R( 0 );
+
+    facv nz = (facv)( R( 0 ) );
+    for ( int p = 0; p < m; p++ )
+    {
+        const int i = p + 1;
+        bw.step( B, sob, p, nz );
+#pragma unroll
+        for ( int a = 0; a < NA; a++ )
+        {
+            if ( a >= na )
+                continue;
+            const DevAsset& A = B.asset[a];
+            const int model = A.model;
+            // correlated noise, as the sweep (correlated_noise_node.cpp:15-27)
+            R eps = nz[a];
+            const size_t crow = ( (size_t)i * kMaxAssets + a ) * kMaxAssets;
+            if ( na > 1 && ( model == 0 || B.corr_stoch ) )
+            {
+                eps = R( 0 );
+#pragma unroll
+                for ( int v = 0; v <= a; v++ )
+                {
+                    if constexpr ( F64 )
+                    {
+                        const double c = as_const( B.chol_t )[crow + v];
+                        if ( c != 0.0 )
+                            eps += nz[v] * c;
+                    }
+                    else
+                        eps += nz[v] * as_const( B.cholf_t )[crow + v];
+                }
+            }
+            cptr<float> sc = as_const( B.sc ) + ( (size_t)i * na + a ) * SC_COUNT;
+            R cd;
+            if constexpr ( F64 )
+                cd = B.drift[(size_t)a * nd + i] * B.t[i] - B.drift[(size_t)a * nd + i - 1] * B.t[i - 1];
+            else
+                cd = sc[SC_CARRY];
+            if ( model == 0 )
+            {
+                if constexpr ( F64 )
+                {
+                    const double w1 = W[a] + B.sqrt_dt[i] * eps;
+                    const double dw = w1 - W[a];
+                    W[a] = w1;
+                    const double v = A.vol;
+                    const double expo = cd - v * v / 2 * B.dt[i] + v * dw;
+                    S[a] = ( S[a] - 0 ) * exp( expo ) + 0;
+                    ge[a] += -v * B.dt[i] + dw;
+                }
+                else
+                {
+                    const float w1 = W[a] + sc[SC_SQDT] * eps;
+                    const float dw = w1 - W[a];
+                    W[a] = w1;
+                    const float expo = cd - sc[SC_HALF_V2DT] + sc[SC_VOL] * dw;
+                    S[a] = S[a] * __expf( expo );
+                    ge[a] += -sc[SC_VOL] * sc[SC_DT] + dw;
+                }
+                continue;
+            }
+            const int vf = __builtin_amdgcn_readfirstlane( A.var_factor );
+            const R zv = nz[vf];
+            const R vp = Var[a];
+            R vc, dvc = R( 0 );
+            const R theta = (R)A.theta;
+            const R dth = model == 1 ? R( 2 ) * sqrtr( theta ) : R( 0 );  // d theta / de
+            if constexpr ( F64 )
+                vc = qe_f64( vp, B.dt[i], zv, A.kappa, A.theta, A.xi );
+            else
+                vc = qe_f32( vp, zv, sc );
+            if ( model == 1 )
+            {
+                R e, M0, S1, S0;
+                if constexpr ( F64 )
+                {
+                    e = exp( -A.kappa * B.dt[i] );
+                    M0 = A.theta * ( 1 - e );
+                    S1 = A.xi * A.xi * e / A.kappa * ( 1 - e );
+                    S0 = A.theta * A.xi * A.xi / ( 2 * A.kappa ) * ( 1 - e ) * ( 1 - e );
+                }
+                else
+                {
+                    e = sc[SC_E];
+                    M0 = sc[SC_M0];
+                    S1 = sc[SC_S1];
+                    S0 = sc[SC_S0];
+                }
+                dvc = qe_tangent<R>( vp, zv, e, M0, S1, S0, theta, dv[a], dth );
+            }
+            // ---- leverage at the previous spot (local_volatility_node.cpp:19-47)
+            // and its tangents: slope in log-spot, derivatives at fixed x
+            R L = R( 1 ), Lx = R( 0 ), Ls = R( 0 ), Le = R( 0 );
+            if ( model == 2 )
+            {
+                const size_t li = (size_t)a * nd + i;
+                R f;  // layer coordinate
+                if constexpr ( F64 )
+                    f = log( S[a] ) / B.lev_ln_step[li] - B.lev_offset[li];
+                else
+                {
+                    const size_t lf = (size_t)i * na + a;
+                    f = fast_log( S[a] ) * as_const( B.lev_inv_step )[lf] - as_const( B.lev_offset_f )[lf];
+                }
+                const R* lv;
+                const R* ds;
+                const R* de;
+                R inv_step;
+                if constexpr ( F64 )
+                {
+                    lv = B.lev + li * P;
+                    ds = B.lev_ds + li * P;
+                    de = B.lev_de + li * P;
+                    inv_step = 1.0 / B.lev_ln_step[li];
+                }
+                else
+                {
+                    lv = B.levf + li * P;
+                    ds = B.lev_dsf + li * P;
+                    de = B.lev_def + li * P;
+                    inv_step = as_const( B.lev_inv_step )[(size_t)i * na + a];
+                }
+                if constexpr ( F64 )
+                {
+                    if ( f <= 0.0 )
+                    {
+                        L = lv[0];
+                        Ls = ds[0];
+                        Le = de[0];
+                    }
+                    else if ( f >= (double)last )
+                    {
+                        L = lv[last];
+                        Ls = ds[last];
+                        Le = de[last];
+                    }
+                    else
+                    {
+                        const int k = (int)f;
+                        const double w = f - (double)k;
+                        L = ( 1.0 - w ) * lv[k] + w * lv[k + 1];
+                        Lx = ( lv[k + 1] - lv[k] ) * inv_step;
+                        Ls = ( 1.0 - w ) * ds[k] + w * ds[k + 1];
+                        Le = ( 1.0 - w ) * de[k] + w * de[k + 1];
+                    }
+                }
+                else
+                {
+                    // the sweep's clamp to [0, last) (k <= last - 1, w < 1)
+                    const float lev_top = __uint_as_float( __float_as_uint( (float)last ) - 1u );
+                    const float fc = __builtin_amdgcn_fmed3f( f, 0.0f, lev_top );
+                    const int k = (int)fc;
+                    const float w = fc - (float)k;
+                    const float sl = lv[k + 1] - lv[k];
+                    L = fmaf( w, sl, lv[k] );
+                    const bool inside = f > 0.0f && f < (float)last;
+                    Lx = inside ? sl * inv_step : 0.0f;
+                    Ls = fmaf( w, ds[k + 1] - ds[k], ds[k] );
+                    Le = fmaf( w, de[k + 1] - de[k], de[k] );
+                }
+            }
+            // ---- the step (the sweep's arithmetic) and d step / d L
+            const R z = eps;
+            R step, G, dmu = R( 0 ), dsig = R( 0 );
+            if constexpr ( F64 )
+            {
+                const double xi = A.xi, rho = A.rho, kappa = A.kappa, dt = B.dt[i];
+                if ( xi > 1e-12 )
+                {
+                    const double gg = 0.5;
+                    const double K0 = -L * rho * kappa * theta / xi * dt;
+                    const double aa = gg * dt * ( L * kappa * rho / xi - 0.5 * L * L );
+                    const double K1 = aa - L * rho / xi;
+                    const double K2 = aa + L * rho / xi;
+                    const double K3 = gg * dt * ( 1.0 - rho * rho ) * L * L;
+                    const double K4 = K3;
+                    double var = K3 * vp + K4 * vc;
+                    if ( var < 0 )
+                        var = 0;
+                    const double sq = sqrt( var );
+                    step = cd + K0 + K1 * vp + K2 * vc + sq * z;
+                    const double dK0 = -rho * kappa * theta / xi * dt;
+                    const double daa = gg * dt * ( kappa * rho / xi - L );
+                    G = dK0 + ( daa - rho / xi ) * vp + ( daa + rho / xi ) * vc + ( L > 0 ? sq / L : 0.0 ) * z;
+                    if ( model == 1 )
+                    {
+                        dmu = -rho * kappa / xi * dt * dth + K1 * dv[a] + K2 * dvc;
+                        dsig = sq > 0 ? K3 * ( dv[a] + dvc ) / ( 2 * sq ) : 0.0;
+                    }
+                }
+                else
+                {
+                    const double sv = sqrt( vp * dt );
+                    step = cd - 0.5 * L * L * vp * dt + L * sv * z;
+                    G = -L * vp * dt + sv * z;
+                    if ( model == 1 )
+                    {
+                        dmu = -0.5 * dt * dv[a];
+                        dsig = sv > 0 ? dt * dv[a] / ( 2 * sv ) : 0.0;
+                    }
+                }
+                S[a] = S[a] * exp( step );
+            }
+            else
+            {
+                float det, bvol;
+                if ( A.xi > 1e-12 )
+                {
+                    const float sv = vp + vc;
+                    const float u = fmaf( sc[SC_RX], vc - vp, fmaf( sc[SC_A1], sv, sc[SC_K0] ) );
+                    const float l2s = L * L * sv;
+                    det = fmaf( -sc[SC_A2], l2s, fmaf( L, u, cd ) );
+                    bvol = __builtin_amdgcn_sqrtf( fmaxf( sc[SC_K3] * l2s, 0.0f ) );
+                    G = u - 2.0f * sc[SC_A2] * L * sv + ( L > 0.0f ? bvol / L : 0.0f ) * z;
+                    if ( model == 1 )
+                    {
+                        const float aa = sc[SC_A1] - sc[SC_A2];
+                        const float dK0 = theta > 0.0f ? sc[SC_K0] / theta * dth : 0.0f;
+                        dmu = dK0 + ( aa - sc[SC_RX] ) * dv[a] + ( aa + sc[SC_RX] ) * dvc;
+                        dsig = bvol > 0.0f ? sc[SC_K3] * ( dv[a] + dvc ) / ( 2.0f * bvol ) : 0.0f;
+                    }
+                }
+                else
+                {
+                    const float sv = __builtin_amdgcn_sqrtf( vp * sc[SC_DT] );
+                    det = cd - 0.5f * L * L * vp * sc[SC_DT];
+                    bvol = L * sv;
+                    G = -L * vp * sc[SC_DT] + sv * z;
+                    if ( model == 1 )
+                    {
+                        dmu = -0.5f * sc[SC_DT] * dv[a];
+                        dsig = sv > 0.0f ? sc[SC_DT] * dv[a] / ( 2.0f * sv ) : 0.0f;
+                    }
+                }
+                step = det + bvol * z;
+                S[a] = S[a] * __expf( step );
+            }
+            // ---- tangents of the log-spot (the leverage reads the previous spot)
+            if ( model == 2 )
+            {
+                gd[a] += G * ( Lx * gd[a] + Ls );
+                ge[a] += G * ( Lx * ge[a] + Le );
+            }
+            else
+            {
+                ge[a] += dmu + dsig * z;
+                dv[a] = dvc;
+            }
+            Var[a] = vc;
+        }
+        // ---- the contract's flow (finance.cpp:9-32, contract_node.cpp:33-47)
+        if ( as_const( B.flow_mask )[i] & 1u )
+        {
+            R s = R( 0 ), w = R( 1 );
+            int js = 0;
+            if ( udl >= 0 )
+            {
+#pragma unroll
+                for ( int a = 0; a < NA; a++ )
+                    if ( a == udl )
+                    {
+                        s = S[a];
+                        js = a;
+                    }
+            }
+            else
+            {
+                const int r = -1 - udl;  // rainbow_node.cpp:14-27
+                const bool best = B.rb_best[r] != 0;
+                s = best ? R( -INFINITY ) : R( INFINITY );
+                for ( int j = 0; j < B.rb_n[r]; j++ )
+                {
+                    const int mj = B.rb_member[r][j];
+                    R sj = R( 0 );
+#pragma unroll
+                    for ( int a = 0; a < NA; a++ )
+                        if ( a == mj )
+                            sj = S[a];
+                    const R perf = sj * R( B.rb_weight[r][j] );
+                    if ( best ? ( s < perf ) : ( perf < s ) )
+                    {
+                        s = perf;
+                        js = mj;
+                        w = R( B.rb_weight[r][j] );
+                    }
+                }
+            }
+            const R f0 = call ? s - K : K - s;
+            const R f = f0 > R( 0 ) ? f0 : R( 0 );
+            const R fp = f0 > R( 0 ) ? ( call ? R( 1 ) : R( -1 ) ) : R( 0 );
+            const R df = (R)B.c_df[0];
+            y = df * f;
+            R sj = R( 0 ), gdj = R( 0 ), gej = R( 0 );
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+                if ( a == js )
+                {
+                    sj = S[a];
+                    gdj = gd[a];
+                    gej = ge[a];
+                }
+            const R chain = df * fp * w * sj;  // d payoff / d x_j*
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+                dl[a] = a == js ? chain * gdj : R( 0 );
+            vg = chain * gej * R( 0.01 );  // per vol point
+        }
+    }
+    // ---- block sums of the estimators (FP64)
+    double e[3 + NA];
+    double dsum = 0.0;
+#pragma unroll
+    for ( int a = 0; a < NA; a++ )
+    {
+        e[3 + a] = valid && a < na ? (double)dl[a] : 0.0;
+        dsum += e[3 + a];
+    }
+    e[0] = valid ? (double)y : 0.0;
+    e[1] = dsum;
+    e[2] = valid ? (double)vg : 0.0;
+    const int ne = 3 + na;
+#pragma unroll
+    for ( int k = 0; k < 3 + NA; k++ )
+    {
+        const double s1 = wave_sum<double>( e[k] );
+        const double s2 = wave_sum<double>( e[k] * e[k] );
+        if ( lane == 0 && k < ne )
+        {
+            acc[wib][k][0] = s1;
+            acc[wib][k][1] = s2;
+        }
+    }
+    __syncthreads();
+    if ( (int)threadIdx.x < 2 * ne )
+    {
+        const int k = threadIdx.x >> 1, mom = threadIdx.x & 1;
+        double s = 0.0;
+        for ( int w = 0; w < kWavesPerBlock; w++ )
+            s += acc[w][k][mom];
+        B.partial[(size_t)blockIdx.x * 2 * ne + threadIdx.x] = s;
+    }
+}
+
 }  // namespace thoth

@@ -12,6 +12,27 @@ void launch_pw( dim3 grid, hipStream_t s, size_t lds, const DevBook* books, cons
 {
     hipLaunchKernelGGL( ( pathwise_kernel<float, D> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first, last );
 }
+template <int NA, int D>
+void launch_pwb( dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V, uint32_t bq0,
+                 uint32_t first, uint32_t last )
+{
+    hipLaunchKernelGGL( ( pathwise_basket_kernel<float, NA, D> ), grid, dim3( kBlock ), lds, s, books, V, bq0, first,
+                        last );
+}
+template <int NA>
+bool launch_pwb_depth( int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,
+                       uint32_t bq0, uint32_t first, uint32_t last )
+{
+    if ( depth <= 8 )
+        launch_pwb<NA, 8>( grid, s, lds, books, V, bq0, first, last );
+    else if ( depth <= 10 )
+        launch_pwb<NA, 10>( grid, s, lds, books, V, bq0, first, last );
+    else if ( depth <= 16 )
+        launch_pwb<NA, 16>( grid, s, lds, books, V, bq0, first, last );
+    else
+        return false;
+    return true;
+}
 }  // namespace
 
 bool launch_pathwise_f32( int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,
@@ -26,5 +47,19 @@ bool launch_pathwise_f32( int depth, dim3 grid, hipStream_t s, size_t lds, const
     else
         return false;
     return true;
+}
+
+bool launch_pathwise_basket_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                                 const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last )
+{
+    if ( na <= 1 )
+        return launch_pwb_depth<1>( depth, grid, s, lds, books, V, bq0, first, last );
+    if ( na <= 2 )
+        return launch_pwb_depth<2>( depth, grid, s, lds, books, V, bq0, first, last );
+    if ( na <= 5 )
+        return launch_pwb_depth<5>( depth, grid, s, lds, books, V, bq0, first, last );
+    if ( na <= 8 )
+        return launch_pwb_depth<8>( depth, grid, s, lds, books, V, bq0, first, last );
+    return false;
 }
 }  // namespace thoth

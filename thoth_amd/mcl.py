@@ -55,6 +55,7 @@ def lib():
         L.thoth_mcl_grid.argtypes = [P(abi.Request), C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.thoth_mcl_calibrate_leverage.argtypes = [P(abi.Request), C.c_int32, C.c_double, C.c_int32,
                                                    P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_double)]
+        L.thoth_mcl_leverage_tangent.argtypes = [P(abi.Request), C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]
         L.thoth_mcl_price_pathwise.argtypes = [P(abi.Request), P(abi.PathwiseResult)]
         L.thoth_mcl_price_rqmc.argtypes = [P(abi.Request), C.c_int32, C.c_uint64, P(abi.RqmcResult)]
         L.thoth_mcl_split_paths.argtypes = [C.c_int64, C.c_int32, P(C.c_int64), P(C.c_int64)]
@@ -170,8 +171,10 @@ def price_rqmc(built, replicates=16, rqmc_seed=1, per_replicate=False):
 
 
 def price_pathwise(built):
-    """Pathwise delta / vega (and gamma from pathwise deltas) of a single-equity
-    Black-Scholes or Heston book with one vanilla or knock-out contract."""
+    """Pathwise delta / vega (and gamma from pathwise deltas): a single-equity
+    Black-Scholes or Heston book with one vanilla or knock-out contract, or a
+    book of 1..8 Black-Scholes / Heston / LSV equities with one european vanilla
+    on an equity or a rainbow (per-equity deltas in asset_delta[:n_assets])."""
     res = abi.PathwiseResult()
     _check(lib().thoth_mcl_price_pathwise(C.byref(built.req), C.byref(res)))
     return res
@@ -244,6 +247,22 @@ def calibrate_leverage(built, asset, spot=None):
     if k < 0:
         _check(-k)
     return dict(t=t[:k], ln_step=ln_step[:k], offset=offset[:k], levels=levels[:k])
+
+
+def leverage_tangent(built, asset, which):
+    """Sensitivity of an LSV equity's per-date calibrated leverage to its spot
+    (which 0) or to the parallel vol shift (which 1): the reference's bump
+    recalibration read at the base nodes, per unit of bump (host only),
+    [dates, points]."""
+    days, _, _ = grid(built)
+    n = days.size
+    points = built.req.leverage_spot_points if built.req.leverage_time_layers > 0 else 201
+    dv = np.zeros((n, points))
+    k = lib().thoth_mcl_leverage_tangent(C.byref(built.req), asset, which, n,
+                                         dv.ctypes.data_as(C.POINTER(C.c_double)))
+    if k < 0:
+        _check(-k)
+    return dv[:k]
 
 
 class BookArrays:
