@@ -279,16 +279,24 @@ def time_to_1bp(cfg, w, train_ms, target=0.01, batch=64, seed=101):
     pricings after which it has paid for itself."""
     b64 = lambda n: basket(n, abi.FP64)
     b32 = lambda n: basket(n, abi.FP32)
+    cfg2 = mcl.cv_config(hedge_dates=cfg.hedge_dates, train_paths=cfg.train_paths,
+                         train_sobol_skip=cfg.train_sobol_skip, adam_steps=cfg.adam_steps,
+                         batch_paths=cfg.batch_paths, learning_rate=cfg.learning_rate, init_seed=cfg.init_seed,
+                         eval_mode=abi.CV_TWO_PHASE, hedge_spacing=cfg.hedge_spacing, activation=cfg.activation)
     runs = {
         "plain_fp64": lambda n, R, s: mcl.price_rqmc(b64(n), R, s, per_replicate=True),
         "plain_fp32": lambda n, R, s: mcl.price_rqmc(b32(n), R, s, per_replicate=True),
         "learned_cv_fp32": lambda n, R, s: mcl.cv_price(b32(n), cfg, w, R, s, per_replicate=True),
+        # the same estimator with the network evaluated after the sweep on its
+        # samples (the sweep of a pricing this small runs in the equity-wave form)
+        "learned_cv_fp32_two_phase": lambda n, R, s: mcl.cv_price(b32(n), cfg2, w, R, s, per_replicate=True),
     }
 
     def result(name, out):
-        if name == "learned_cv_fp32":
+        if name.startswith("learned_cv_fp32"):
             r, _, cm = out
-            return r, cm, r.run_wall_ms, r.sweep_ms, r.setup_ms
+            wall = r.run_wall_ms if r.run_wall_ms > 0 else r.sweep_ms + r.eval_ms
+            return r, cm, wall, r.sweep_ms + r.eval_ms, r.setup_ms
         r, m = out
         return r, m, r.run_wall_ms, r.sweep_ms, r.setup_ms
 
@@ -298,14 +306,19 @@ def time_to_1bp(cfg, w, train_ms, target=0.01, batch=64, seed=101):
         def expected_hw(n):
             _, means, _, _, _ = result(name, run(n, 64, seed))
             return float(np.sqrt(np.mean(_hw16(means) ** 2)))
-        pilot = 4096
-        k = max(0, math.ceil(4 * math.log2(pilot / 256 * (expected_hw(pilot) / target) ** 2)) - 6)
-        while True:
-            n = grid_n(k)
-            hw_expected = expected_hw(n)
-            if hw_expected <= target or n >= 1 << 22:
-                break
-            k += 1
+        if name == "learned_cv_fp32_two_phase":
+            # the same estimator as the in-loop one: the same paths per replicate
+            n, hw_expected = res["learned_cv_fp32"]["single"]["paths_per_replicate"], \
+                res["learned_cv_fp32"]["single"]["expected_halfwidth"]
+        else:
+            pilot = 4096
+            k = max(0, math.ceil(4 * math.log2(pilot / 256 * (expected_hw(pilot) / target) ** 2)) - 6)
+            while True:
+                n = grid_n(k)
+                hw_expected = expected_hw(n)
+                if hw_expected <= target or n >= 1 << 22:
+                    break
+                k += 1
         walls = []
         for k in range(5):
             r, means, wall, dev, setup = result(name, run(n, 16, seed + 1 + k))
@@ -313,6 +326,9 @@ def time_to_1bp(cfg, w, train_ms, target=0.01, batch=64, seed=101):
         single = {"paths_per_replicate": n, "paths": 16 * n, "expected_halfwidth": hw_expected,
                   "halfwidth": float(_hw16(means)[0]), "wall_ms": float(np.median(walls)), "device_ms": dev,
                   "book_setup_ms": setup}
+        if name == "learned_cv_fp32_two_phase":
+            res[name] = {"single": single}
+            continue
         r, means, wall, dev, _ = result(name, run(n, 16 * batch, seed + 100))
         hws = _hw16(means)
         batched = {"pricings": batch, "wall_ms_per_pricing": wall / batch, "device_ms_per_pricing": dev / batch,
@@ -320,8 +336,14 @@ def time_to_1bp(cfg, w, train_ms, target=0.01, batch=64, seed=101):
                    "fraction_at_target": float(np.mean(hws <= target))}
         res[name] = {"single": single, "batched": batched}
     cv, p64, p32 = res["learned_cv_fp32"], res["plain_fp64"], res["plain_fp32"]
+    # a single pricing takes the faster evaluation of the network (in the loop,
+    # or after the sweep on its samples); a batch the in-loop one
+    cv2 = res["learned_cv_fp32_two_phase"]
+    res["single_cv_evaluation"] = "two_phase" if cv2["single"]["wall_ms"] < cv["single"]["wall_ms"] else "in_loop"
     for regime, key in (("single", "wall_ms"), ("batched", "wall_ms_per_pricing")):
         t64, t32, tcv = p64[regime][key], p32[regime][key], cv[regime][key]
+        if regime == "single":
+            tcv = min(tcv, cv2["single"]["wall_ms"])
         res["speedup_" + regime] = {
             "learned_cv_fp32_vs_plain_fp64": t64 / tcv,
             "learned_cv_fp32_vs_plain_fp32": t32 / tcv,

@@ -199,8 +199,15 @@ void thoth_mcl_release(thoth_mcl_plan plan);
  * choose): the launch shape of a plan's sweep.  lev_group: time steps per LDS
  * barrier (leverage layers and Sobol ops staged per group); workgroups_per_cu:
  * 0 launches one workgroup per Sobol block of 256 paths, W > 0 a persistent
- * grid of W workgroups per CU pulling (book, block) items.  Every shape gives
- * the same per-block partial sums, hence the same result bit for bit.
+ * grid of W workgroups per CU pulling (book, block) items, -1 the equity-wave
+ * form (FP32 books whose equities are all LSV with uncorrelated spots and
+ * whose contracts are vanillas: the equities of 64 paths stepped by as many
+ * waves, for pricings too small to fill the device; chosen by itself where
+ * the other forms would leave the SIMDs under two waves each).  The forms
+ * 0 and W > 0 give the same per-block partial sums, hence the same result
+ * bit for bit, and are thoth_mcl_tune's candidates; the equity-wave form
+ * sums the same path values in quarters of a block (the same to FP64
+ * rounding of the sums) and is set by itself or by thoth_mcl_set_shape.
  * thoth_mcl_tune times every candidate (lev_group 1, 2, 4, 8 x the launch
  * forms that exist for the plan's books) twice on the plan's own books, keeps
  * the fastest and caches it by book shape for the process (candidates = 0:

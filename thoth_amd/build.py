@@ -41,6 +41,7 @@ UNITS = [
     ("mcl_sweep_f32_uniform5.hip", "hipcc", ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]),
     ("mcl_util.hip", "hipcc", []),
     ("mcl_cv.hip", "hipcc", []),
+    ("mcl_sweep_waves.hip", "hipcc", []),
     ("mcl_pathwise.hip", "hipcc", ["-ffp-contract=off"]),
     ("mcl_pathwise_f32.hip", "hipcc", []),
     ("mcl_noalloc.hip", "hipcc", []),

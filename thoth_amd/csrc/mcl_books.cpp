@@ -426,6 +426,22 @@ static size_t LdsBytes( const DeviceBatch& B, int G )
     return ( ( words + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + B.lds_tail;
 }
 
+// dynamic LDS of the equity-wave form at G steps per barrier: Sobol block
+// table, ring of the 2 na factors, every equity's 2G leverage layers and the
+// spots exchanged at a flow date
+static size_t WavesLds( const DeviceBatch& B, int G )
+{
+    size_t ring = 0;
+    int P = 0;
+    for ( size_t b = 0; b < B.step_ops.size(); b++ )
+    {
+        ring = std::max( ring, (size_t)RingSlots( B.step_ops[b], G ) * 2 * B.na_max * 32 );
+        P = std::max( P, B.db[b].lev_points );
+    }
+    const size_t words = ( ( B.lds_dims + 3 ) & ~size_t( 3 ) ) + ring + (size_t)2 * G * B.na_max * P + 64 * B.na_max;
+    return words * sizeof( uint32_t );
+}
+
 // result slots per block of book b: the book, its contracts, and P - CV when
 // the hedge network runs in the loop
 static int SlotsOf( const DeviceBatch& B, int b )
@@ -661,12 +677,34 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         B.grid_total += (size_t)B.replicates * B.nblocks * B.grid_cells[b] * 2;
         B.grid_out_total += (size_t)B.grid_cells[b] * 2;
     }
+    // the equity-wave latency form: FP32 uniform LSV books (one equity count),
+    // vanilla contracts only, plain sweeps or control-variate samples
+    B.waves_ok = cfg->precision == 1 && B.kind == 1 && !B.prng && !B.recording && !any_grid && !pathwise &&
+                 ( B.cv_mode == 0 || B.cv_mode == 1 ) && sweep_waves_available( B.na_max, B.depth, B.cv_mode ) &&
+                 WavesLds( B, lev_group ) + kSweepStaticLds <= (size_t)160 * 1024;
+    for ( int b = 0; b < n_books && B.waves_ok; b++ )
+        for ( int c = 0; c < B.db[b].n_contracts; c++ )
+            B.waves_ok = B.waves_ok && B.db[b].c_kind[c] == 0;
+    B.wave_split = false;
+    if ( B.waves_ok )
+    {
+        // where the lane form gives fewer than two waves per SIMD (measured on
+        // the five-equity basket: 16 replicates of 1536 paths 0.58 ms against
+        // 1.30 ms; at 16 x 8192, two lane waves per SIMD, the lane form is
+        // the faster, 1.35 against 1.61 ms)
+        hipDeviceProp_t prop;
+        if ( hipGetDeviceProperties( &prop, 0 ) != hipSuccess )
+            return SetError( THOTH_ERR_DEVICE, "hipGetDeviceProperties" );
+        const size_t lane_waves = (size_t)B.nblocks * n_books * B.replicates * kWavesPerBlock;
+        B.wave_split = lane_waves < (size_t)2 * 4 * prop.multiProcessorCount;
+    }
+    const int tiles = B.waves_ok ? 4 : 1;  // partial-sum tiles per Sobol block
     B.part_off.assign( n_books, 0 );
     B.part_total = 0;
     for ( int b = 0; b < n_books; b++ )
     {
         B.part_off[b] = B.part_total;
-        B.part_total += (size_t)B.replicates * B.nblocks * SlotsOf( B, b ) * 2;
+        B.part_total += (size_t)B.replicates * B.nblocks * tiles * SlotsOf( B, b ) * 2;
     }
     // random digital shifts: one 32-bit word per (replicate, dimension) from
     // xoshiro256++ seeded by splitmix64(rqmc_seed) (upper 32 bits of each draw)
@@ -907,7 +945,14 @@ static int EnqueueSweep( const DeviceBatch& B, hipStream_t s )
             return SetError( THOTH_ERR_DEVICE, "pseudo-random stream set-up launch failed" );
     }
     bool ok;
-    if ( B.persist_wpc > 0 )
+    if ( B.wave_split )
+    {
+        const dim3 grid( 4 * B.nblocks, B.n_books, B.replicates );
+        const int G = B.db.empty() ? 1 : B.db[0].lev_group;
+        ok = launch_sweep_waves( B.na_max, B.depth, B.cv_mode, grid, s, WavesLds( B, G ), B.dbooks,
+                                 DeviceDirections(), B.q0, B.first, B.last, B.dshifts );
+    }
+    else if ( B.persist_wpc > 0 )
     {
         // work queue of (book, Sobol block) items; the counter is cleared by a kernel
         launch_counter_reset( s, B.dcounter );
@@ -945,7 +990,8 @@ static int EnqueueSweep( const DeviceBatch& B, hipStream_t s )
     return THOTH_OK;
 }
 
-int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out,
+              const std::function<int( hipStream_t )>& then )
 {
     hipStream_t s = Stream();
     hipEvent_t e0 = SweepEvent( 0 ), e1 = SweepEvent( 1 );
@@ -954,19 +1000,23 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out )
     if ( rc != THOTH_OK )
         return rc;
     HIP_OK( hipEventRecord( e1, s ) );
+    if ( then && ( rc = then( s ) ) != THOTH_OK )
+        return rc;
     std::vector<double> part( B.part_total );
     HIP_OK( hipMemcpyAsync( part.data(), B.dpart, B.part_total * sizeof( double ), hipMemcpyDeviceToHost, s ) );
     HIP_OK( hipStreamSynchronize( s ) );
     float ms = 0;
     HIP_OK( hipEventElapsedTime( &ms, e0, e1 ) );
+    // partial-sum tiles of the form that ran (4 per Sobol block in the equity-wave form)
+    const uint32_t ntiles = B.nblocks * ( B.wave_split ? 4u : 1u );
     for ( int b = 0; b < B.n_books; b++ )
       for ( int r = 0; r < B.replicates; r++ )
     {
         const int nc = B.db[b].n_contracts;
         const int ne = SlotsOf( B, b );
         std::vector<double> sum( ne * 2, 0.0 ), comp( ne * 2, 0.0 );
-        const double* pb = part.data() + B.part_off[b] + (size_t)r * B.nblocks * ne * 2;
-        for ( uint32_t k = 0; k < B.nblocks; k++ )
+        const double* pb = part.data() + B.part_off[b] + (size_t)r * ntiles * ne * 2;
+        for ( uint32_t k = 0; k < ntiles; k++ )
             for ( int j = 0; j < ne * 2; j++ )
             {
                 // Kahan over blocks in block order (deterministic)
@@ -1029,13 +1079,21 @@ static bool PersistentAvailable( const DeviceBatch& B )
 
 int SetShape( DeviceBatch& B, int lev_group, int wpc )
 {
-    if ( wpc < 0 || wpc > 16 )
-        return SetError( THOTH_ERR_INVALID, "workgroups per CU must be in [0, 16]" );
+    if ( wpc < -1 || wpc > 16 )
+        return SetError( THOTH_ERR_INVALID, "workgroups per CU must be in [-1, 16]" );
     if ( wpc > 0 && !PersistentAvailable( B ) )
         return SetError( THOTH_ERR_UNSUPPORTED, "no persistent form of the sweep for these books" );
+    if ( wpc == -1 && ( !B.waves_ok || WavesLds( B, lev_group ) + kSweepStaticLds > (size_t)160 * 1024 ) )
+        return SetError( THOTH_ERR_UNSUPPORTED, "no equity-wave form of the sweep for these books" );
     int rc = SetLevGroup( B, lev_group );
     if ( rc != THOTH_OK )
         return rc;
+    B.wave_split = wpc == -1;
+    if ( wpc == -1 )
+    {
+        B.persist_wpc = 0;
+        return THOTH_OK;
+    }
     if ( wpc > 0 && !B.ditems )
     {
         // items book-major, Sobol blocks in order
@@ -1105,7 +1163,10 @@ int TuneShape( DeviceBatch& B, ShapeChoice& out )
         return THOTH_OK;
     }
     const int groups[] = { 1, 2, 4, 8 };
-    const int forms[] = { 0, 1, 2, 3, 4 };  // 0: one workgroup per Sobol block; W > 0: persistent, W per CU
+    // 0: one workgroup per Sobol block; W > 0: persistent, W per CU (the
+    // forms with the same partial sums; the equity-wave form is not a
+    // candidate: it sums in quarter blocks, and is taken by itself where it pays)
+    const int forms[] = { 0, 1, 2, 3, 4 };
     ShapeChoice best;
     best.ms = 1e300;
     const int g0 = B.db.empty() ? 4 : B.db[0].lev_group, w0 = B.persist_wpc;

@@ -76,47 +76,59 @@ __global__ void mlp_pack_kernel( const float* __restrict__ w, uint16_t* __restri
 
 // Forward of every sample: per path Y = P - CV; block partial sums
 // (sum Y, sum Y^2, sum P, sum P^2) in FP64 for each replicate.
-// One wave = 16 paths (one sample column tile), looping over the J dates.
+// Workgroup = 16 paths (one sample column tile) x kEvalChunks waves, wave c
+// taking the c-th chunk of the J dates: the dates' inferences are
+// independent, so the chunks run side by side (kEvalChunks times the waves,
+// each with kEvalChunks times fewer dependent inferences) and meet in LDS,
+// summed in chunk order.
+constexpr int kEvalChunks = 8;
 template <int ACT>
-__global__ void __launch_bounds__( 256 ) cv_eval_kernel( const uint2* __restrict__ wimg, const float* __restrict__ bias,
-                                                        const uint16_t* __restrict__ feat, const float* __restrict__ dm,
-                                                        const float* __restrict__ pay, int64_t N, int J,
-                                                        double* __restrict__ partial )
+__global__ void __launch_bounds__( 64 * kEvalChunks ) cv_eval_kernel( const uint2* __restrict__ wimg,
+                                                                     const float* __restrict__ bias,
+                                                                     const uint16_t* __restrict__ feat,
+                                                                     const float* __restrict__ dm,
+                                                                     const float* __restrict__ pay, int64_t N, int J,
+                                                                     double* __restrict__ partial )
 {
     const int lane = threadIdx.x & 63;
-    const int wib = threadIdx.x >> 6;
+    const int chunk = threadIdx.x >> 6;
     const int r = blockIdx.y;
-    const int64_t p = ( (int64_t)blockIdx.x * 4 + wib ) * 16 + ( lane & 15 );
+    const int64_t p = (int64_t)blockIdx.x * 16 + ( lane & 15 );
     const bool valid = p < N;
     const int64_t pp = valid ? p : 0;
+    const int dpc = ( J + kEvalChunks - 1 ) / kEvalChunks;
+    const int j0 = chunk * dpc, j1 = min( J, j0 + dpc );
     float cv = 0.0f;
-    for ( int j = 0; j < J; j++ )
+    for ( int j = j0; j < j1; j++ )
     {
         const size_t row = ( (size_t)r * J + j ) * N + pp;
         cv += mlp_dot( mlp_infer<ACT>( wimg, bias, load_feat( feat, row, lane ), lane ), load_dm( dm, row, lane ) );
     }
+    __shared__ float part[kEvalChunks][16];
+    if ( lane < 16 )
+        part[chunk][lane] = cv;
+    __syncthreads();
+    if ( chunk != 0 )
+        return;
+    float tot = 0.0f;
+    if ( lane < 16 )
+        for ( int c = 0; c < kEvalChunks; c++ )
+            tot += part[c][lane];
     const float P = pay[(size_t)r * N + pp];
-    double y = ( valid && lane < 16 ) ? (double)P - (double)cv : 0.0;
+    double y = ( valid && lane < 16 ) ? (double)P - (double)tot : 0.0;
     double pv = ( valid && lane < 16 ) ? (double)P : 0.0;
     double v[4] = { y, y * y, pv, pv * pv };
-    __shared__ double red[4][4];
 #pragma unroll
     for ( int k = 0; k < 4; k++ )
     {
         double x = v[k];
         for ( int off = 32; off >= 1; off >>= 1 )
             x += __shfl_xor( x, off, 64 );
-        if ( lane == 0 )
-            red[wib][k] = x;
+        v[k] = x;
     }
-    __syncthreads();
-    if ( threadIdx.x < 4 )
-    {
-        double s = 0.0;
-        for ( int w = 0; w < 4; w++ )
-            s += red[w][threadIdx.x];
-        partial[( (size_t)r * gridDim.x + blockIdx.x ) * 4 + threadIdx.x] = s;
-    }
+    if ( lane < 4 )
+        partial[( (size_t)r * gridDim.x + blockIdx.x ) * 4 + lane] =
+            lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : v[3];
 }
 
 // Training step, part 1: the control variate of the minibatch split by date
@@ -317,16 +329,17 @@ void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bw
     hipLaunchKernelGGL( mlp_pack_kernel, dim3( ( kMlpWSize + 255 ) / 256 ), dim3( 256 ), 0, s, w, fwd, bwd );
 }
 
+uint32_t cv_eval_blocks( int64_t N ) { return (uint32_t)( ( N + 15 ) / 16 ); }
+
 void launch_cv_eval( hipStream_t s, int act, const uint16_t* fwd, const float* bias, const uint16_t* feat,
-                     const float* dm, const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks )
+                     const float* dm, const float* pay, int64_t N, int J, int R, double* partial )
 {
-    const uint32_t nb = (uint32_t)( ( N + 63 ) / 64 );
-    *nblocks = nb;
+    const uint32_t nb = cv_eval_blocks( N );
     if ( act == kActGate )
-        hipLaunchKernelGGL( cv_eval_kernel<kActGate>, dim3( nb, R ), dim3( 256 ), 0, s,
+        hipLaunchKernelGGL( cv_eval_kernel<kActGate>, dim3( nb, R ), dim3( 64 * kEvalChunks ), 0, s,
                             reinterpret_cast<const uint2*>( fwd ), bias, feat, dm, pay, N, J, partial );
     else
-        hipLaunchKernelGGL( cv_eval_kernel<kActSilu>, dim3( nb, R ), dim3( 256 ), 0, s,
+        hipLaunchKernelGGL( cv_eval_kernel<kActSilu>, dim3( nb, R ), dim3( 64 * kEvalChunks ), 0, s,
                             reinterpret_cast<const uint2*>( fwd ), bias, feat, dm, pay, N, J, partial );
 }
 

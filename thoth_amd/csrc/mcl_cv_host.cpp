@@ -85,9 +85,10 @@ int CheckConfig( const thoth_cv_config& c )
     return THOTH_OK;
 }
 
-// FP32 CV-data sweep of the base book: `paths` paths per replicate
-int CvSweep( const thoth_mcl_request& req, int J, float spacing, int64_t paths, int64_t skip, int R, uint64_t rqmc_seed,
-             Dev& dev, CvSpec& cv, std::vector<thoth_mcl_book_out>& out, double& ms )
+// FP32 CV-data sweep of the base book, `paths` paths per replicate: the
+// sample buffers and the batch, resident and ready to sweep
+int CvPrepare( const thoth_mcl_request& req, int J, float spacing, int64_t paths, int64_t skip, int R,
+               uint64_t rqmc_seed, Dev& dev, CvSpec& cv, DeviceBatch& B )
 {
     host::BookArrays A;
     host::Grid g;
@@ -109,11 +110,17 @@ int CvSweep( const thoth_mcl_request& req, int J, float spacing, int64_t paths, 
     cfg.precision = 1;
     cfg.replicates = R;
     cfg.rqmc_seed = rqmc_seed;
+    return PrepareBatch( 1, &A.book, &cfg, B, &cv );
+}
+
+int CvSweep( const thoth_mcl_request& req, int J, float spacing, int64_t paths, int64_t skip, int R, uint64_t rqmc_seed,
+             Dev& dev, CvSpec& cv, std::vector<thoth_mcl_book_out>& out, double& ms )
+{
     DeviceBatch B;
-    rc = PrepareBatch( 1, &A.book, &cfg, B, &cv );
+    int rc = CvPrepare( req, J, spacing, paths, skip, R, rqmc_seed, dev, cv, B );
     if ( rc != THOTH_OK )
         return rc;
-    out.assign( Rr, thoth_mcl_book_out{} );
+    out.assign( std::max( 1, R ), thoth_mcl_book_out{} );
     rc = RunBatch( B, out.data() );
     ms = out[0].device_ms;
     return rc;
@@ -364,19 +371,21 @@ int PriceInLoop( const thoth_mcl_request& req, const thoth_cv_config& cfg, const
 int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, const thoth_cv_weights& w, int R,
                    uint64_t rqmc_seed, thoth_cv_result* res )
 {
+    // set-up: the book resident, the sample and evaluation buffers allocated,
+    // the network packed; then the timed run: sweep, evaluation, reduction
+    const auto t0 = std::chrono::steady_clock::now();
     Dev dev;
     CvSpec cv;
-    std::vector<thoth_mcl_book_out> out;
-    double sweep_ms = 0;
     const int J = cfg.hedge_dates;
     const int64_t N = req.mcl.paths;
-    int rc = CvSweep( req, J, cfg.hedge_spacing, N, 0, R, rqmc_seed, dev, cv, out, sweep_ms );
+    DeviceBatch B;
+    int rc = CvPrepare( req, J, cfg.hedge_spacing, N, 0, R, rqmc_seed, dev, cv, B );
     if ( rc != THOTH_OK )
         return rc;
     float* dw = dev.alloc<float>( kN );
     uint16_t* fwd = dev.alloc<uint16_t>( kWSize );
     uint16_t* bwd = dev.alloc<uint16_t>( kWSize );
-    const uint32_t nb = (uint32_t)( ( N + 63 ) / 64 );
+    const uint32_t nb = cv_eval_blocks( N );
     double* part = dev.alloc<double>( (size_t)R * nb * 4 );
     if ( !dw || !fwd || !bwd || !part )
         return SetError( THOTH_ERR_DEVICE, "cannot allocate the evaluation buffers" );
@@ -386,14 +395,23 @@ int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, con
     hipEvent_t e0, e1;
     HIP_OK( hipEventCreate( &e0 ) );
     HIP_OK( hipEventCreate( &e1 ) );
-    HIP_OK( hipEventRecord( e0, s ) );
-    uint32_t nbl = 0;
-    launch_cv_eval( s, cfg.activation, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part, &nbl );
-    HIP_OK( hipGetLastError() );
-    HIP_OK( hipEventRecord( e1, s ) );
     std::vector<double> hp( (size_t)R * nb * 4 );
-    HIP_OK( hipMemcpyAsync( hp.data(), part, hp.size() * sizeof( double ), hipMemcpyDeviceToHost, s ) );
     HIP_OK( hipStreamSynchronize( s ) );
+    const auto t1 = std::chrono::steady_clock::now();
+    // the sweep, then (same stream, no synchronisation between) the evaluation
+    std::vector<thoth_mcl_book_out> out( R );
+    rc = RunBatch( B, out.data(), nullptr, [&]( hipStream_t st ) {
+        if ( hipEventRecord( e0, st ) != hipSuccess )
+            return SetError( THOTH_ERR_DEVICE, "hipEventRecord" );
+        launch_cv_eval( st, cfg.activation, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part );
+        if ( hipGetLastError() != hipSuccess || hipEventRecord( e1, st ) != hipSuccess ||
+             hipMemcpyAsync( hp.data(), part, hp.size() * sizeof( double ), hipMemcpyDeviceToHost, st ) != hipSuccess )
+            return SetError( THOTH_ERR_DEVICE, "control-variate evaluation launch failed" );
+        return (int)THOTH_OK;
+    } );
+    if ( rc != THOTH_OK )
+        return rc;
+    const double sweep_ms = out[0].device_ms;
     float ems = 0;
     HIP_OK( hipEventElapsedTime( &ems, e0, e1 ) );
     (void)hipEventDestroy( e0 );
@@ -413,8 +431,11 @@ int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, con
             }
     }
     FinishCi( sums, R, N, res );
+    const auto t2 = std::chrono::steady_clock::now();
     res->sweep_ms = sweep_ms;
     res->eval_ms = ems;
+    res->setup_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count();
+    res->run_wall_ms = std::chrono::duration<double, std::milli>( t2 - t1 ).count();
     return THOTH_OK;
 }
 

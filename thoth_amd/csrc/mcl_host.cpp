@@ -1244,7 +1244,7 @@ extern "C" __attribute__( ( visibility( "default" ) ) ) int thoth_mcl_get_shape(
     std::lock_guard<std::mutex> lk( thoth::DeviceMutex() );
     const auto* p = reinterpret_cast<const thoth::host::Plan*>( plan );
     shape->lev_group = p->batch.db.empty() ? 0 : p->batch.db[0].lev_group;
-    shape->workgroups_per_cu = p->batch.persist_wpc;
+    shape->workgroups_per_cu = p->batch.wave_split ? -1 : p->batch.persist_wpc;
     shape->device_ms = p->tuned_ms;
     shape->candidates = 0;
     shape->default_ms = 0;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -103,6 +104,12 @@ struct DeviceBatch
     // (book, block) items (the same per-block partial sums, so the same
     // result bit for bit); the item list and counter are device-resident
     int persist_wpc = 0, cus = 0, n_items = 0;
+    // the equity-wave latency form (FP32 uniform LSV books with vanilla
+    // contracts): waves_ok when it applies (the partial sums then have room
+    // for its 4 tiles per Sobol block), wave_split when it is the form in
+    // effect (chosen where the lane form would leave the SIMDs under two
+    // waves each, or by the launch shape)
+    bool waves_ok = false, wave_split = false;
     int2* ditems = nullptr;
     int* dcounter = nullptr;
     DeviceBatch() = default;
@@ -168,10 +175,14 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
                   const CvSpec* cv = nullptr, const GridSpec* grids = nullptr, int pathwise = 0,
                   char* ext_base = nullptr, size_t ext_cap = 0, const RecordSpec* rec = nullptr,
                   const PathwiseLayers* pwl = nullptr );
-int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr );
+// `then`, when given, enqueues further work on the library stream right after
+// the sweep (before the one synchronisation that returns the sums)
+int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out = nullptr,
+              const std::function<int( hipStream_t )>& then = {} );
 // Launch shape of a plan's batch (reference-mode sweeps without the control
 // variate): steps per LDS barrier G and workgroups per CU of the persistent
-// form (0: one workgroup per Sobol block).  SetShape applies one (refusing
+// form (0: one workgroup per Sobol block, -1: the equity-wave form, where it
+// applies).  SetShape applies one (refusing
 // shapes whose LDS does not fit or that have no kernel instantiation);
 // TuneShape times every candidate (G in {1, 2, 4, 8} x launch forms) twice
 // on the batch itself, keeps the fastest, and caches it by book shape.

@@ -65,8 +65,10 @@ bool launch_persistent_f32_uniform( int kind, int na, int depth, dim3 grid, hipS
                                     uint32_t last, const int2* items, int n_items, uint32_t ntiles, int* counter );
 void launch_mlp_pack( hipStream_t s, const float* w, uint16_t* fwd, uint16_t* bwd );
 // act: the hidden activation of the network (kActSilu 0, kActGate 1, mcl_mlp.hpp)
+// the two-phase evaluation: partial [R][cv_eval_blocks(N)][4] (16 paths per block)
+uint32_t cv_eval_blocks( int64_t N );
 void launch_cv_eval( hipStream_t s, int act, const uint16_t* fwd, const float* bias, const uint16_t* feat,
-                     const float* dm, const float* pay, int64_t N, int J, int R, double* partial, uint32_t* nblocks );
+                     const float* dm, const float* pay, int64_t N, int J, int R, double* partial );
 // one Adam step of the hedge network: minibatch of B paths x J dates split
 // into `chunks` date chunks of dpc dates; one gradient slice per block
 struct CvTrainLayout
@@ -115,6 +117,12 @@ bool launch_pathwise( int precision, int depth, dim3 grid, hipStream_t s, size_t
                       const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 bool launch_pathwise_f32( int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books, const uint32_t* V,
                           uint32_t bq0, uint32_t first, uint32_t last );
+// the equity-wave latency form of the FP32 uniform-LSV sweep (sweep_waves):
+// whether an instantiation exists, and its launch, grid (4 x Sobol blocks,
+// books, replicates), 64 na threads per workgroup
+bool sweep_waves_available( int na, int depth, int cv_mode );
+bool launch_sweep_waves( int na, int depth, int cv_mode, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
+                         const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
 // the pathwise basket kernel (3 + n_assets estimators per block); false when
 // no instantiation covers the equity count or bridge depth
 bool launch_pathwise_basket( int precision, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
