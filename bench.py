@@ -303,6 +303,8 @@ def time_to_1bp(cfg, w, train_ms, target=0.01, batch=64, seed=101):
     grid_n = lambda k: 256 * math.ceil(2 ** (k / 4))
     res = {"target_halfwidth": target, "replicates": 16}
     for name, run in runs.items():
+        progress("time to 1bp: " + name)
+
         def expected_hw(n):
             _, means, _, _, _ = result(name, run(n, 64, seed))
             return float(np.sqrt(np.mean(_hw16(means) ** 2)))
@@ -423,6 +425,15 @@ def basket_greeks(paths=1 << 20, repeats=2, fp64_paths=1 << 14):
                         "vega": p.vega, "vega_trust": p.vega_trust,
                         "ms": min(x.sweep_ms for x in pw), "books": p.n_books,
                         "setup_s": min(x.setup_ms for x in pw) * 1e-3}}
+    # delta and vega alone: one pathwise book against the 7 bump books
+    dv = abi.GREEK_DELTA | abi.GREEK_VEGA
+    b2 = basket(paths, greeks=dv)
+    plan2 = mcl.Plan(b2)
+    plan2.run()
+    ms2 = [plan2.run().sweep_ms for _ in range(repeats)]
+    pw2 = [mcl.price_pathwise(b2) for _ in range(repeats)]
+    out["delta_vega_only"] = {"bump_ms": min(ms2), "bump_books": plan2.run().n_scenarios,
+                              "pathwise_ms": min(x.sweep_ms for x in pw2), "pathwise_books": pw2[-1].n_books}
     small = {prec: mcl.price_pathwise(basket(fp64_paths, precision=prec, greeks=greeks))
              for prec in (abi.FP32, abi.FP64)}
     a, c = small[abi.FP32], small[abi.FP64]

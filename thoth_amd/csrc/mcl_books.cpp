@@ -461,6 +461,8 @@ DeviceBatch::~DeviceBatch()
         (void)hipFree( ditems );
     if ( dcounter )
         (void)hipFree( dcounter );
+    if ( hpart )
+        (void)hipHostFree( hpart );
 }
 
 int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_sweep_cfg* cfg, DeviceBatch& B,
@@ -813,6 +815,10 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
         hipError_t e = hipMalloc( &B.dbase, total );
         if ( e != hipSuccess )
             return SetError( THOTH_ERR_DEVICE, std::string( "hipMalloc: " ) + hipGetErrorString( e ) );
+        // pinned host memory for the partial sums (the no-allocation entry
+        // point keeps a pageable copy per run)
+        if ( hipHostMalloc( &B.hpart, std::max<size_t>( 1, B.part_total ) * sizeof( double ) ) != hipSuccess )
+            B.hpart = nullptr;
     }
     char* dbase = B.dbase;
     B.dpart = reinterpret_cast<double*>( dbase + arena_bytes + books_bytes );
@@ -1002,8 +1008,11 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out,
     HIP_OK( hipEventRecord( e1, s ) );
     if ( then && ( rc = then( s ) ) != THOTH_OK )
         return rc;
-    std::vector<double> part( B.part_total );
-    HIP_OK( hipMemcpyAsync( part.data(), B.dpart, B.part_total * sizeof( double ), hipMemcpyDeviceToHost, s ) );
+    // into the batch's pinned buffer where it has one (a truly asynchronous
+    // copy, no staging), else into pageable memory
+    std::vector<double> pageable( B.hpart ? 0 : B.part_total );
+    double* part = B.hpart ? B.hpart : pageable.data();
+    HIP_OK( hipMemcpyAsync( part, B.dpart, B.part_total * sizeof( double ), hipMemcpyDeviceToHost, s ) );
     HIP_OK( hipStreamSynchronize( s ) );
     float ms = 0;
     HIP_OK( hipEventElapsedTime( &ms, e0, e1 ) );
@@ -1015,7 +1024,7 @@ int RunBatch( const DeviceBatch& B, thoth_mcl_book_out* out, CvSums* cv_out,
         const int nc = B.db[b].n_contracts;
         const int ne = SlotsOf( B, b );
         std::vector<double> sum( ne * 2, 0.0 ), comp( ne * 2, 0.0 );
-        const double* pb = part.data() + B.part_off[b] + (size_t)r * ntiles * ne * 2;
+        const double* pb = part + B.part_off[b] + (size_t)r * ntiles * ne * 2;
         for ( uint32_t k = 0; k < ntiles; k++ )
             for ( int j = 0; j < ne * 2; j++ )
             {

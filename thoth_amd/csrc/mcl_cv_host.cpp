@@ -395,7 +395,15 @@ int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, con
     hipEvent_t e0, e1;
     HIP_OK( hipEventCreate( &e0 ) );
     HIP_OK( hipEventCreate( &e1 ) );
-    std::vector<double> hp( (size_t)R * nb * 4 );
+    // pinned host copy of the evaluation's partial sums
+    double* hp = nullptr;
+    const size_t nhp = (size_t)R * nb * 4;
+    HIP_OK( hipHostMalloc( &hp, nhp * sizeof( double ) ) );
+    struct PinnedFree
+    {
+        double* p;
+        ~PinnedFree() { (void)hipHostFree( p ); }
+    } hp_free{ hp };
     HIP_OK( hipStreamSynchronize( s ) );
     const auto t1 = std::chrono::steady_clock::now();
     // the sweep, then (same stream, no synchronisation between) the evaluation
@@ -405,7 +413,7 @@ int PriceTwoPhase( const thoth_mcl_request& req, const thoth_cv_config& cfg, con
             return SetError( THOTH_ERR_DEVICE, "hipEventRecord" );
         launch_cv_eval( st, cfg.activation, fwd, dw + kWSize, cv.feat, cv.dm, cv.pay, N, J, R, part );
         if ( hipGetLastError() != hipSuccess || hipEventRecord( e1, st ) != hipSuccess ||
-             hipMemcpyAsync( hp.data(), part, hp.size() * sizeof( double ), hipMemcpyDeviceToHost, st ) != hipSuccess )
+             hipMemcpyAsync( hp, part, nhp * sizeof( double ), hipMemcpyDeviceToHost, st ) != hipSuccess )
             return SetError( THOTH_ERR_DEVICE, "control-variate evaluation launch failed" );
         return (int)THOTH_OK;
     } );

@@ -51,6 +51,7 @@ struct DeviceBatch
     char* dbase = nullptr;
     bool external = false;  // dbase is caller-provided memory (not freed here)
     double* dpart = nullptr;
+    double* hpart = nullptr;  // pinned host copy of the partial sums (null: a pageable one per run)
     const DevBook* dbooks = nullptr;
     const uint32_t* dshifts = nullptr;  // [replicates][kSobolDims] in RQMC mode
     int replicates = 1;

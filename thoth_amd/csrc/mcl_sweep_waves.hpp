@@ -101,7 +101,9 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
         const int base = ( ( o & ( nslots - 1 ) ) * NF + f ) * 32;
         return rlo[base] ^ rhi[base];
     };
-    auto so = [&]( int i ) __attribute__( ( always_inline ) ) { return B.step_ops[min( i, m )]; };
+    // per-step tables by scalar loads at wave-uniform indices
+    auto so = [&]( int i ) __attribute__( ( always_inline ) )
+    { return __builtin_amdgcn_readfirstlane( as_const( B.step_ops )[min( i, m )] ); };
     // leverage layers G at a time into a ring of 2G per equity (layer i in
     // slot (i - 1) mod 2G), every equity's group one contiguous run
     const float lev_top = __uint_as_float( __float_as_uint( (float)( P - 1 ) ) - 1u );
@@ -172,7 +174,7 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
     auto bridge_step = [&]( int p, float* nz ) __attribute__( ( always_inline ) )
     {
         const int i = p + 1;
-        const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
+        const int o0 = so( p ), o1 = so( p + 1 );
         if ( o1 - o0 == 1 && sp >= 1 )
         {
             const int o = o0;
@@ -254,7 +256,7 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
             {
                 for ( int e = NA; e < 5; e++ )
                     row[e] = row[5 + e] = 0;
-                row[10] = (uint16_t)to_bf16( fmaf( 2.0f * (float)B.t[i], B.cv_inv_T, -1.0f ) );
+                row[10] = (uint16_t)to_bf16( fmaf( 2.0f * (float)as_const( B.t )[i], B.cv_inv_T, -1.0f ) );
                 for ( int e = 11; e < 16; e++ )
                     row[e] = 0;
             }
