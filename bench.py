@@ -203,10 +203,11 @@ def _profile_counter(counter, cmd, kernel, workdir):
     return _counter_kib(files[0], kernel) if files else None
 
 
-def hbm_traffic(paths, kernel="sweep_kernel"):
+def hbm_traffic(paths, shape=None, kernel="sweep_kernel"):
     """HBM bytes per launch of the basket sweep: rocprofv3 FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch) of a one-step run, each scaled by bytes /
-    counter of a device-to-device copy of known size."""
+    WRITE_SIZE (KiB per dispatch) of a one-step run at the measured launch
+    shape (forced, so that no tuning candidates enter the average), each
+    scaled by bytes / counter of a device-to-device copy of known size."""
     import shutil
     import tempfile
     if not shutil.which("rocprofv3"):
@@ -214,6 +215,8 @@ def hbm_traffic(paths, kernel="sweep_kernel"):
     probe_mib = 256
     me = [sys.executable, str(Path(__file__).resolve())]
     run = me + ["--leg", "c3", "--paths", str(paths), "--steps", "1", "--warmup", "0"]
+    if shape:
+        run += ["--shape", "%d:%d" % shape]
     probe = me + ["--copy-probe", str(probe_mib)]
     detail = {}
     total = 0.0
@@ -628,7 +631,8 @@ def main():
         line["pseudo_random"] = prng_leg(args.paths)
     if "traffic" in legs and "roofline" in line:
         progress("hbm traffic")
-        traffic, detail = hbm_traffic(args.paths)
+        sh = line.get("launch_shape") or {}
+        traffic, detail = hbm_traffic(args.paths, (sh["lev_group"], sh["workgroups_per_cu"]) if sh else None)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_detail"] = detail
         if traffic:

@@ -313,6 +313,29 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
         cv_date( 0 );
     pay( 0 );
 
+    // the equity's step constants of step i, loaded one step ahead: the
+    // table (dates x equities x 16 floats) is larger than the scalar cache,
+    // so a load at the step that reads it would wait on L2 every step
+    struct StepConsts
+    {
+        float sc[SC_COUNT];
+        float inv_step, offset, cv_scale;
+    };
+    auto step_consts = [&]( int i ) __attribute__( ( always_inline ) )
+    {
+        StepConsts c;
+        const size_t lf = (size_t)min( i, m ) * NA + a;
+        cptr<float> sc = as_const( B.sc ) + lf * SC_COUNT;
+#pragma unroll
+        for ( int k = 0; k < SC_COUNT; k++ )
+            c.sc[k] = sc[k];
+        c.inv_step = as_const( B.lev_inv_step )[lf];
+        c.offset = as_const( B.lev_offset_f )[lf];
+        c.cv_scale = cv_samples ? as_const( B.cv_scale )[lf] : 0.0f;
+        return c;
+    };
+    StepConsts cur = step_consts( 1 );
+
     float nz[2];
     if ( m >= 1 )
         bridge_step( 0, nz );
@@ -326,7 +349,8 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
                 stage_group( i + G );
             stage_ops( so( i + G ), so( i + 2 * G ) );
         }
-        cptr<float> sc = as_const( B.sc ) + ( (size_t)i * NA + a ) * SC_COUNT;
+        const StepConsts nxt = step_consts( i + 1 );
+        const float* sc = cur.sc;
         // QE variance draw (quadratic branch for every lane, the exponential
         // lanes redone), heston_variance_node.cpp:17-57
         const QeMoments q = qe_moments( v, sc );
@@ -337,8 +361,7 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
                 vc = qe_exponential( q, nz[1], cv_samples ? &dq : nullptr );
         // leverage at the previous spot (local_volatility_node.cpp:19-47)
         const float* lv = levs + ( a * 2 * G + ( i - 1 ) % ( 2 * G ) ) * P;
-        const size_t lf = (size_t)i * NA + a;
-        const float f = x * as_const( B.lev_inv_step )[lf] - as_const( B.lev_offset_f )[lf];
+        const float f = x * cur.inv_step - cur.offset;
         const float fc = __builtin_amdgcn_fmed3f( f, 0.0f, lev_top );
         const int k = (int)fc;
         const float w = fc - (float)k;
@@ -357,10 +380,11 @@ __global__ void __launch_bounds__( 64 * NA ) sweep_waves( const DevBook* __restr
             const float sold = __expf( x );
             const float A = L * fmaf( -L, sc[SC_CVB], sc[SC_CVA] );
             const float M = fmaf( A, vc - dq.mean, bvol * zs ) * sold;
-            dmacc = fmaf( as_const( B.cv_scale )[lf], M, dmacc );
+            dmacc = fmaf( cur.cv_scale, M, dmacc );
         }
         x += det + bvol * zs;
         v = vc;
+        cur = nxt;
         if constexpr ( cv_samples )
             cv_date( i );
         pay( i );
