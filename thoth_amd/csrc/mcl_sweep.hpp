@@ -537,7 +537,9 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 prs.s[q] = B.states[(size_t)q * B.tail_stride + prow];
         }
 
-    const int F = B.factors;
+    // uniform books: every equity stochastic, so 2 NA factors (2 shared
+    // ones when grouped) and equity a's variance noise is factor NA + a
+    const int F = uniform ? NF : B.factors;
     const int m = B.m;
     const int na = B.n_assets;
     const int nd = B.n_dates;
@@ -1297,7 +1299,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
             for ( int a = 0; a < NA; a++ )
             {
                 cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
-                const int vf = grouped ? 1 : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+                const int vf = grouped ? 1 : uniform ? NA + a : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                 const QeMoments q = qe_moments( Var[a], sc );
                 vq[a] = qe_quadratic( q, nz[vf], cv_on ? &dq[a] : nullptr );
                 expo |= q.quad ? 0u : 1u << a;
@@ -1309,7 +1311,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     if ( ( expo >> a ) & 1u )
                     {
                         cptr<float> sc = as_const( B.sc ) + ( (size_t)i * nas + a ) * SC_COUNT;
-                        const int vf = grouped ? 1 : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+                        const int vf = grouped ? 1 : uniform ? NA + a : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
                         vq[a] = qe_exponential( qe_moments( Var[a], sc ), nz[vf], cv_on ? &dq[a] : nullptr );
                     }
             }
@@ -1390,7 +1392,7 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                 continue;
             }
             // Heston / LSV (heston_variance_node.cpp, heston_spot_node.cpp)
-            const int vf = grouped ? 1 : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
+            const int vf = grouped ? 1 : uniform ? NA + a : __builtin_amdgcn_readfirstlane( B.asset[a].var_factor );
             const R zv = nz[vf];
             const R vp = Var[a];
             R vc;
