@@ -461,8 +461,7 @@ struct BridgeWalk
                 else
                     nz[f] = ( w - prev[f] ) * as_const( B.binv )[i];
                 prev[f] = w;
-                if ( sp >= 2 )
-                    top[f] = stk_get( f, pos );
+                top[f] = stk_get( f, pos );  // stale only after the last pop
             }
         }
         sp--;
@@ -1264,8 +1263,10 @@ __device__ __forceinline__ void sweep_block( const DevBook& B, const uint32_t* _
                     else
                         nz[f] = ( w - prev[f] ) * as_const( B.binv )[i];
                     prev[f] = w;
-                    if ( sp >= 2 )
-                        top[f] = stk_get( f, pos );
+                    // sp = 1 only at the last pop (W at the final date, the
+                    // bridge's first point, is pending until then), after
+                    // which top is never read: no select on sp
+                    top[f] = stk_get( f, pos );
                 }
             }
             sp--;
