@@ -169,6 +169,35 @@ def test_pathwise_basket_is_the_derivative_of_its_estimator(gpu, kinds, corr_sto
     assert abs(pw.vega - fdv * 0.01) <= 1e-6 * abs(fdv * 0.01) + 1e-9
 
 
+@pytest.mark.parametrize("days,step", [(60, 2), (730, 2)])
+def test_lsv_staged_form_equals_the_generic_kernel(gpu, days, step):
+    """The FP32 staged-leverage form of the pathwise basket kernel (books of
+    five LSV equities with uncorrelated stochastic spots) against the generic
+    FP32 kernel on the same book and Sobol points: identity correlation, once
+    without correlated stochastic spots (the staged form) and once with them
+    (the generic form, whose Cholesky row is then exactly the unit vector, so
+    both step the same noises).  Both evaluate the same arithmetic on the same
+    leverage nodes (LDS ring against device memory): premium, per-equity
+    deltas and vega agree to FP32 rounding of the compiled forms (1e-6
+    relative, 1e-9 absolute); 730 days is the full basket grid (365 steps,
+    the ring recycled 90 times)."""
+    out = {}
+    for corr_stoch in (False, True):
+        spec = configs.lsv_worst_of_basket(paths=1 << 14, precision=abi.FP32, maturity_days=days,
+                                           max_day_step=step, correlate_stochastic=corr_stoch)
+        spec.correlation = np.eye(5).tolist()
+        out[corr_stoch] = gpu.price_pathwise(BuiltRequest(spec))
+    a, b = out[False], out[True]
+    print("staged %.10f generic %.10f | vega %.10f %.10f" % (a.premium, b.premium, a.vega, b.vega))
+    assert a.n_assets == b.n_assets == 5
+    assert abs(a.premium - b.premium) <= 1e-6 * abs(b.premium) + 1e-9
+    for k in range(5):
+        print("delta[%d] %.10f %.10f" % (k, a.asset_delta[k], b.asset_delta[k]))
+        assert abs(a.asset_delta[k] - b.asset_delta[k]) <= 1e-6 * abs(b.asset_delta[k]) + 1e-9, k
+    assert abs(a.vega - b.vega) <= 1e-6 * abs(b.vega) + 1e-9
+    assert abs(a.gamma - b.gamma) <= 1e-6 * abs(b.gamma) + 1e-9
+
+
 def test_lsv_basket_pathwise_fp32_against_fp64(gpu):
     """FP32 stepping of the five-equity basket against FP64 on the same points
     (2^14 paths, 60 days): premium within 1e-4 relative, per-equity deltas and

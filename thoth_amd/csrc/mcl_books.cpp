@@ -413,14 +413,19 @@ static int RingSlots( const std::vector<int>& so, int G )
 static size_t LdsBytes( const DeviceBatch& B, int G )
 {
     // the pathwise kernel reads a Sobol block table; the sweep a block table
-    // (not in the in-loop inference kernels) and a ring
-    // (the pathwise kernels read the leverage from device memory)
+    // (not in the in-loop inference kernels) and a ring.  The leverage ring:
+    // 2G layers per equity, of floats in the sweep, of float4 nodes (layer
+    // and its two sensitivities) in the staged pathwise form, none in the
+    // other pathwise kernels (they read it from device memory)
     size_t ring = 0;
     if ( !B.pathwise )
         for ( const std::vector<int>& so : B.step_ops )
             ring = std::max( ring, (size_t)RingSlots( so, G ) * B.nf_inst * 32 );
-    const size_t table = B.pathwise ? B.lds_dims : B.cv_mode == 2 ? 0 : ( B.lds_dims + 3 ) & ~size_t( 3 );
-    const size_t words = table + ring + ( B.pathwise ? 0 : (size_t)G * B.lds_lev_words );
+    const size_t table = B.pathwise && !B.pw_lsv ? B.lds_dims
+                         : B.cv_mode == 2        ? 0
+                                                 : ( B.lds_dims + 3 ) & ~size_t( 3 );
+    const size_t lev = !B.pathwise ? (size_t)G * B.lds_lev_words : B.pw_lsv ? (size_t)G * 4 * B.lds_lev_words : 0;
+    const size_t words = table + ring + lev;
     if ( !B.lds_tail )
         return words * sizeof( uint32_t );
     return ( ( words + 3 ) & ~size_t( 3 ) ) * sizeof( uint32_t ) + B.lds_tail;
@@ -632,6 +637,10 @@ int PrepareBatch( int32_t n_books, const thoth_mcl_book* books, const thoth_mcl_
                 return SetError( THOTH_ERR_UNSUPPORTED, "American exercise and vanilla grids are separate sweeps" );
         B.kind = 0;
     }
+    // pathwise Greeks of FP32 books of LSV equities with uncorrelated
+    // stochastic spots (the uniform kind 1): the staged-leverage form
+    B.pw_lsv = pathwise == 2 && pwl && cfg->precision == 1 && B.kind == 1 &&
+               pathwise_basket_lsv_available( B.na_max, B.depth );
     // dynamic LDS: the sweep's Sobol ring (the pathwise kernel: the Sobol block
     // table of the widest book), plus (FP32 LSV) a ring of 2G leverage layers
     // per equity of the kernel instantiation (G = 4, halved while two blocks

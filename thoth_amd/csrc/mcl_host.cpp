@@ -907,8 +907,9 @@ int RunPathwise( const DeviceBatch& B, int kernel, std::vector<double>& part, fl
     const dim3 grid( B.nblocks, B.n_books );
     const bool ok = kernel == 1 ? launch_pathwise( B.precision, B.depth, grid, s, B.lds_bytes, B.dbooks,
                                                    DeviceDirections(), B.q0, B.first, B.last )
-                                : launch_pathwise_basket( B.precision, B.na_max, B.depth, grid, s, B.lds_bytes,
-                                                          B.dbooks, DeviceDirections(), B.q0, B.first, B.last );
+                                : launch_pathwise_basket( B.precision, B.na_max, B.depth, B.pw_lsv, grid, s,
+                                                          B.lds_bytes, B.dbooks, DeviceDirections(), B.q0, B.first,
+                                                          B.last );
     if ( !ok )
         return SetError( THOTH_ERR_UNSUPPORTED, "bridge deeper than the pathwise kernels support" );
     if ( hipGetLastError() != hipSuccess )

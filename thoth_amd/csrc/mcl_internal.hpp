@@ -111,6 +111,9 @@ struct DeviceBatch
     // effect (chosen where the lane form would leave the SIMDs under two
     // waves each, or by the launch shape)
     bool waves_ok = false, wave_split = false;
+    // pathwise basket Greeks of uniform FP32 LSV books: the staged-leverage
+    // form (its ring of float4 nodes counted in the LDS bytes)
+    bool pw_lsv = false;
     int2* ditems = nullptr;
     int* dcounter = nullptr;
     DeviceBatch() = default;

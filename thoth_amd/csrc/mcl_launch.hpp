@@ -124,11 +124,14 @@ bool sweep_waves_available( int na, int depth, int cv_mode );
 bool launch_sweep_waves( int na, int depth, int cv_mode, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
                          const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last, const uint32_t* shifts );
 // the pathwise basket kernel (3 + n_assets estimators per block); false when
-// no instantiation covers the equity count or bridge depth
-bool launch_pathwise_basket( int precision, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
+// no instantiation covers the equity count or bridge depth.  lsv: the FP32
+// form for books of na LSV equities with uncorrelated stochastic spots and
+// the leverage ring in LDS (where pathwise_basket_lsv_available)
+bool pathwise_basket_lsv_available( int na, int depth );
+bool launch_pathwise_basket( int precision, int na, int depth, bool lsv, dim3 grid, hipStream_t s, size_t lds,
                              const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
-bool launch_pathwise_basket_f32( int na, int depth, dim3 grid, hipStream_t s, size_t lds, const DevBook* books,
-                                 const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
+bool launch_pathwise_basket_f32( int na, int depth, bool lsv, dim3 grid, hipStream_t s, size_t lds,
+                                 const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last );
 void launch_slot_sum( hipStream_t s, const double* part, int nslots, int per_slot, double* out );
 void launch_grid_sum( hipStream_t s, const double* part, const long long* table, int n_books, int max_per_slot,
                       int nslots, double* out );

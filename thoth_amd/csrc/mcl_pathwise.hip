@@ -51,11 +51,13 @@ bool launch_pathwise( int precision, int depth, dim3 grid, hipStream_t s, size_t
     return true;
 }
 
-bool launch_pathwise_basket( int precision, int na, int depth, dim3 grid, hipStream_t s, size_t lds,
+bool launch_pathwise_basket( int precision, int na, int depth, bool lsv, dim3 grid, hipStream_t s, size_t lds,
                              const DevBook* books, const uint32_t* V, uint32_t bq0, uint32_t first, uint32_t last )
 {
     if ( precision != 0 )
-        return launch_pathwise_basket_f32( na, depth, grid, s, lds, books, V, bq0, first, last );
+        return launch_pathwise_basket_f32( na, depth, lsv, grid, s, lds, books, V, bq0, first, last );
+    if ( lsv )
+        return false;
     if ( na <= 1 )
         return launch_pwb_depth<1>( depth, grid, s, lds, books, V, bq0, first, last );
     if ( na <= 2 )

@@ -187,7 +187,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
     sob.tab = lds;
     sob.set_lane( wib, lane );
     BridgeWalk<R, 2, DEPTH> bw;
-    bw.start( B, sob );
+    bw.start( B, sob, B.factors );
 
     const DevAsset& A = B.asset[0];
     const bool heston = A.model == 1;
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
     for ( int p = 0; p < m; p++ )
     {
         const int i = p + 1;
-        bw.step( B, sob, p, nz );
+        bw.step( B, sob, p, nz, B.factors );
         cptr<float> sc = as_const( B.sc ) + (size_t)i * SC_COUNT;
         R eps = nz[0];
         R cd;
@@ -433,13 +433,24 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
 // Payoff: vanilla on s = S_u, or s = the rainbow's extreme w_j S_j (weights
 // fixed at the rebasing spots, rainbow.cpp:54-57); d f / d x_j* = f' w S_j*.
 // Per block: [Y, sum_a delta_a, vega, delta_0 .. delta_{na-1}] x [sum, sum2].
+//
+// LSV = true: the FP32 form for books of exactly NA LSV equities with
+// uncorrelated stochastic spots (the uniform sweep's kind 1): the model
+// branches and the variance-factor lookups are compile-time, and the
+// leverage with its two sensitivity layers is staged in LDS as one float4
+// {L, L_spot, L_vol, 0} per node, G dates at a time into a ring of 2G date
+// slots per equity (the sweep's leverage ring and barrier scheme), instead of
+// six gathers from device memory per equity and step.  Same arithmetic on
+// the same values, so the same results as the generic form.
 // ---------------------------------------------------------------------------
-template <class R, int NA, int DEPTH>
+template <class R, int NA, int DEPTH, bool LSV = false>
 __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBook* __restrict__ books,
                                                                    const uint32_t* __restrict__ V, uint32_t bq0,
                                                                    uint32_t first_idx, uint32_t last_idx )
 {
     constexpr bool F64 = std::is_same_v<R, double>;
+    constexpr bool lsv = LSV && !F64;
+    static_assert( !LSV || !F64, "the staged LSV form is FP32" );
     constexpr int NF = 2 * NA;
     typedef R facv __attribute__( ( ext_vector_type( NF ) ) );
     const DevBook& B = books[blockIdx.y];
@@ -453,19 +464,48 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
     extern __shared__ uint32_t lds[];
     __shared__ double acc[kWavesPerBlock][3 + kMaxAssets][2];
     sobol_block_table( lds, V, m * B.factors, bq, nullptr );
+    const int P = B.lev_points;
+    const int last = P - 1;
+    // LSV: the leverage ring after the block table (16-byte aligned); layer
+    // date i of equity a in slot (a 2G + (i - 1) mod 2G), node k at + k
+    const int G = B.lev_group;
+    [[maybe_unused]] float4* lring = reinterpret_cast<float4*>( lds + ( ( m * B.factors + 3 ) & ~3 ) );
+    [[maybe_unused]] auto stage_group = [&]( int first ) __attribute__( ( always_inline ) )
+    {
+        // dates first .. first + G - 1 are contiguous in every layer array
+        // and in the ring (first - 1 is a multiple of G)
+        const int run = ( min( first + G - 1, m ) - first + 1 ) * P;
+        const int q0 = ( first - 1 ) % ( 2 * G );
+        for ( int e = (int)threadIdx.x; e < run; e += kBlock )
+        {
+            float v[NA][3];
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+            {
+                const size_t at = ( (size_t)a * nd + first ) * P + e;
+                v[a][0] = B.levf[at];
+                v[a][1] = B.lev_dsf[at];
+                v[a][2] = B.lev_def[at];
+            }
+#pragma unroll
+            for ( int a = 0; a < NA; a++ )
+                lring[( a * 2 * G + q0 ) * P + e] = make_float4( v[a][0], v[a][1], v[a][2], 0.0f );
+        }
+    };
+    if constexpr ( lsv )
+        if ( m >= 1 )
+            stage_group( 1 );
     __syncthreads();
     SobolLane sob;
     sob.V = V;
     sob.tab = lds;
     sob.set_lane( wib, lane );
     BridgeWalk<R, NF, DEPTH> bw;
-    bw.start( B, sob );
+    bw.start( B, sob, lsv ? NF : B.factors );
 
     const int udl = B.c_udl[0];
     const R K = (R)B.c_strike[0];
     const bool call = B.c_call[0] != 0;
-    const int P = B.lev_points;
-    const int last = P - 1;
 
     R S[NA], Var[NA], W[NA], gd[NA], ge[NA], dv[NA], dl[NA];
 #pragma unroll
@@ -487,18 +527,27 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
     for ( int p = 0; p < m; p++ )
     {
         const int i = p + 1;
-        bw.step( B, sob, p, nz );
+        if constexpr ( lsv )
+            if ( ( i - 1 ) % G == 0 )
+            {
+                // this group's layers are visible and the previous group's
+                // slots free: stage the next group
+                __syncthreads();
+                if ( i + G <= m )
+                    stage_group( i + G );
+            }
+        bw.step( B, sob, p, nz, lsv ? NF : B.factors );
 #pragma unroll
         for ( int a = 0; a < NA; a++ )
         {
-            if ( a >= na )
+            if ( !lsv && a >= na )
                 continue;
             const DevAsset& A = B.asset[a];
-            const int model = A.model;
+            const int model = lsv ? 2 : A.model;
             // correlated noise, as the sweep (correlated_noise_node.cpp:15-27)
             R eps = nz[a];
             const size_t crow = ( (size_t)i * kMaxAssets + a ) * kMaxAssets;
-            if ( na > 1 && ( model == 0 || B.corr_stoch ) )
+            if ( !lsv && na > 1 && ( model == 0 || B.corr_stoch ) )
             {
                 eps = R( 0 );
 #pragma unroll
@@ -543,7 +592,8 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
                 }
                 continue;
             }
-            const int vf = __builtin_amdgcn_readfirstlane( A.var_factor );
+            // LSV form: every equity stochastic, variance noise NA + a
+            const int vf = lsv ? NA + a : __builtin_amdgcn_readfirstlane( A.var_factor );
             const R zv = nz[vf];
             const R vp = Var[a];
             R vc, dvc = R( 0 );
@@ -628,6 +678,22 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
                         Le = ( 1.0 - w ) * de[k] + w * de[k + 1];
                     }
                 }
+                else if constexpr ( lsv )
+                {
+                    // as below, from the LDS ring: one float4 per node
+                    const float lev_top = __uint_as_float( __float_as_uint( (float)last ) - 1u );
+                    const float fc = __builtin_amdgcn_fmed3f( f, 0.0f, lev_top );
+                    const int k = (int)fc;
+                    const float w = fc - (float)k;
+                    const float4* lq = lring + ( a * 2 * G + ( i - 1 ) % ( 2 * G ) ) * P;
+                    const float4 n0 = lq[k], n1 = lq[k + 1];
+                    const float sl = n1.x - n0.x;
+                    L = fmaf( w, sl, n0.x );
+                    const bool inside = f > 0.0f && f < (float)last;
+                    Lx = inside ? sl * inv_step : 0.0f;
+                    Ls = fmaf( w, n1.y - n0.y, n0.y );
+                    Le = fmaf( w, n1.z - n0.z, n0.z );
+                }
                 else
                 {
                     // the sweep's clamp to [0, last) (k <= last - 1, w < 1)
@@ -688,7 +754,7 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
             else
             {
                 float det, bvol;
-                if ( A.xi > 1e-12 )
+                if ( lsv || A.xi > 1e-12 )
                 {
                     const float sv = vp + vc;
                     const float u = fmaf( sc[SC_RX], vc - vp, fmaf( sc[SC_A1], sv, sc[SC_K0] ) );

@@ -375,10 +375,10 @@ struct BridgeWalk
             stk[f][pos] = x;
     }
 
-    // endpoint first (build index 0, path_generator.cpp:166-167)
-    __device__ __forceinline__ void start( const DevBook& B, const SobolLane& sob )
+    // endpoint first (build index 0, path_generator.cpp:166-167); F the
+    // book's factor count (a compile-time NF where the caller knows it)
+    __device__ __forceinline__ void start( const DevBook& B, const SobolLane& sob, int F )
     {
-        const int F = B.factors;
         if constexpr ( paired )
         {
 #pragma unroll
@@ -411,10 +411,9 @@ struct BridgeWalk
     // bridge step p: build the pending points, then pop W(t_{p+1}) into nz
     // (normalised increment per factor, path_generator.cpp:175-196)
     template <class V>
-    __device__ __forceinline__ void step( const DevBook& B, const SobolLane& sob, int p, V& nz )
+    __device__ __forceinline__ void step( const DevBook& B, const SobolLane& sob, int p, V& nz, int F )
     {
         constexpr bool F64 = std::is_same_v<R, double>;
-        const int F = B.factors;
         const int i = p + 1;
         const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
         for ( int o = o0; o < o1; o++ )
