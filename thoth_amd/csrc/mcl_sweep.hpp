@@ -434,12 +434,40 @@ struct BridgeWalk
                 rw = as_const( B.op_wf )[3 * o + 1];
                 sd = as_const( B.op_wf )[3 * o + 2];
             }
+            // FP32: the central polynomial for every factor straight-line,
+            // the tail lanes (w >= 8) redone in one block (normal_f32's values)
+            R zf[NF];
+            uint32_t tails = 0;
+#pragma unroll
+            for ( int f = 0; f < NF; f++ )
+            {
+                zf[f] = R( 0 );
+                if ( f < F )
+                {
+                    if constexpr ( F64 )
+                        zf[f] = normal_of<R>( sob( bidx * F + f ) );
+                    else
+                    {
+                        bool t;
+                        zf[f] = normal_f32_central( sob( bidx * F + f ), t );
+                        tails |= t ? 1u << f : 0u;
+                    }
+                }
+            }
+            if constexpr ( !F64 )
+                if ( __builtin_amdgcn_ballot_w64( tails != 0 ) )
+                {
+#pragma unroll
+                    for ( int f = 0; f < NF; f++ )
+                        if ( ( tails >> f ) & 1u )
+                            zf[f] = normal_f32( sob( bidx * F + f ) );
+                }
 #pragma unroll
             for ( int f = 0; f < NF; f++ )
             {
                 if ( f < F )
                 {
-                    const R z = normal_of<R>( sob( bidx * F + f ) );
+                    const R z = zf[f];
                     const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
                     stk_set( f, pos, top[f] );
                     top[f] = nv;
