@@ -413,12 +413,13 @@ static int RingSlots( const std::vector<int>& so, int G )
 static size_t LdsBytes( const DeviceBatch& B, int G )
 {
     // the pathwise kernel reads a Sobol block table; the sweep a block table
-    // (not in the in-loop inference kernels) and a ring.  The leverage ring:
+    // (not in the in-loop inference kernels) and a ring (so does the staged
+    // pathwise form).  The leverage ring:
     // 2G layers per equity, of floats in the sweep, of float4 nodes (layer
     // and its two sensitivities) in the staged pathwise form, none in the
     // other pathwise kernels (they read it from device memory)
     size_t ring = 0;
-    if ( !B.pathwise )
+    if ( !B.pathwise || B.pw_lsv )
         for ( const std::vector<int>& so : B.step_ops )
             ring = std::max( ring, (size_t)RingSlots( so, G ) * B.nf_inst * 32 );
     const size_t table = B.pathwise && !B.pw_lsv ? B.lds_dims

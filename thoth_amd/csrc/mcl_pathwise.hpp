@@ -207,10 +207,12 @@ __global__ void __launch_bounds__( kBlock ) pathwise_kernel( const DevBook* __re
     R y = R( 0 ), dl = R( 0 ), vg = R( 0 );
 
     facv nz = (facv)( R( 0 ) );
+    const int F = B.factors;
+    auto point = [&]( int o, int f ) __attribute__( ( always_inline ) ) { return sob( B.op_b[o] * F + f ); };
     for ( int p = 0; p < m; p++ )
     {
         const int i = p + 1;
-        bw.step( B, sob, p, nz, B.factors );
+        bw.step( B, point, p, nz, F );
         cptr<float> sc = as_const( B.sc ) + (size_t)i * SC_COUNT;
         R eps = nz[0];
         R cd;
@@ -466,10 +468,46 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
     sobol_block_table( lds, V, m * B.factors, bq, nullptr );
     const int P = B.lev_points;
     const int last = P - 1;
-    // LSV: the leverage ring after the block table (16-byte aligned); layer
-    // date i of equity a in slot (a 2G + (i - 1) mod 2G), node k at + k
+    // LSV form, after the block table (16-byte aligned): the sweep's Sobol
+    // ring (op o, factor f: 32 words in slot o mod nslots, the lane's point
+    // lo[gray(l) & 15] ^ hi[gray(l) >> 4] with the block part folded into hi),
+    // then the leverage ring (layer date i of equity a in slot
+    // a 2G + (i - 1) mod 2G, node k at + k)
     const int G = B.lev_group;
-    [[maybe_unused]] float4* lring = reinterpret_cast<float4*>( lds + ( ( m * B.factors + 3 ) & ~3 ) );
+    const int F = lsv ? NF : B.factors;
+    const int nslots = lsv ? B.sob_slots : 0;
+    const int tab_words = ( m * B.factors + 3 ) & ~3;
+    [[maybe_unused]] uint32_t* ring = lds + tab_words;
+    [[maybe_unused]] float4* lring = reinterpret_cast<float4*>( lds + tab_words + nslots * NF * 32 );
+    const uint32_t gln = gray32( (uint32_t)threadIdx.x );
+    [[maybe_unused]] const uint32_t* rlo = ring + ( gln & 15u );
+    [[maybe_unused]] const uint32_t* rhi = ring + 16 + ( gln >> 4 );
+    auto so = [&]( int i ) __attribute__( ( always_inline ) ) { return B.step_ops[min( i, m )]; };
+    [[maybe_unused]] auto stage_ops = [&]( int o0, int o1 ) __attribute__( ( always_inline ) )
+    {
+        // four adjacent lanes per (op, factor), each owning 8 of its 32 words
+        typedef uint32_t u32x4 __attribute__( ( ext_vector_type( 4 ) ) );
+        const uint32_t* nib = V + kSobolNibbles;
+        const int n = ( o1 - o0 ) * NF * 4;
+        for ( int k = threadIdx.x; k < n; k += kBlock )
+        {
+            const int q = k & 3;
+            const int pair = k >> 2;
+            const int o = o0 + pair / NF;
+            const int f = pair - ( pair / NF ) * NF;
+            const int d = B.op_b[o] * F + f;
+            const u32x4* src = reinterpret_cast<const u32x4*>( nib + (size_t)d * 32 + 8 * q );
+            u32x4 x0 = src[0], x1 = src[1];
+            if ( q >= 2 )
+            {
+                x0 ^= lds[d];
+                x1 ^= lds[d];
+            }
+            u32x4* dst = reinterpret_cast<u32x4*>( ring + ( ( o & ( nslots - 1 ) ) * NF + f ) * 32 + 8 * q );
+            dst[0] = x0;
+            dst[1] = x1;
+        }
+    };
     [[maybe_unused]] auto stage_group = [&]( int first ) __attribute__( ( always_inline ) )
     {
         // dates first .. first + G - 1 are contiguous in every layer array
@@ -492,16 +530,32 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
                 lring[( a * 2 * G + q0 ) * P + e] = make_float4( v[a][0], v[a][1], v[a][2], 0.0f );
         }
     };
+    // the block table is complete before the ops are staged from it
+    __syncthreads();
     if constexpr ( lsv )
+    {
+        // the ops of bridge step 0 and of the first group; its layers
+        stage_ops( 0, so( G + 1 ) );
         if ( m >= 1 )
             stage_group( 1 );
+    }
     __syncthreads();
     SobolLane sob;
     sob.V = V;
     sob.tab = lds;
     sob.set_lane( wib, lane );
     BridgeWalk<R, NF, DEPTH> bw;
-    bw.start( B, sob, lsv ? NF : B.factors );
+    bw.start( B, sob, F );
+    auto point = [&]( int o, int f ) __attribute__( ( always_inline ) ) -> uint32_t
+    {
+        if constexpr ( lsv )
+        {
+            const int base = ( ( o & ( nslots - 1 ) ) * NF + f ) * 32;
+            return rlo[base] ^ rhi[base];
+        }
+        else
+            return sob( B.op_b[o] * F + f );
+    };
 
     const int udl = B.c_udl[0];
     const R K = (R)B.c_strike[0];
@@ -523,20 +577,22 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
     }
     R y = R( 0 ), vg = R( 0 );
 
+    // bridge step p (the increments of step p + 1) runs at the end of step p
     facv nz = (facv)( R( 0 ) );
-    for ( int p = 0; p < m; p++ )
+    if ( m >= 1 )
+        bw.step( B, point, 0, nz, F );
+    for ( int i = 1; i <= m; i++ )
     {
-        const int i = p + 1;
         if constexpr ( lsv )
             if ( ( i - 1 ) % G == 0 )
             {
-                // this group's layers are visible and the previous group's
-                // slots free: stage the next group
+                // this group's layers and ops are visible and the previous
+                // group's slots free: stage the next group
                 __syncthreads();
                 if ( i + G <= m )
                     stage_group( i + G );
+                stage_ops( so( i + G ), so( i + 2 * G ) );
             }
-        bw.step( B, sob, p, nz, lsv ? NF : B.factors );
 #pragma unroll
         for ( int a = 0; a < NA; a++ )
         {
@@ -855,6 +911,8 @@ __global__ void __launch_bounds__( kBlock ) pathwise_basket_kernel( const DevBoo
                 dl[a] = a == js ? chain * gdj : R( 0 );
             vg = chain * gej * R( 0.01 );  // per vol point
         }
+        if ( i < m )
+            bw.step( B, point, i, nz, F );
     }
     // ---- block sums of the estimators (FP64)
     double e[3 + NA];

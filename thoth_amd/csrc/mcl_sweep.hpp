@@ -409,16 +409,16 @@ struct BridgeWalk
     }
 
     // bridge step p: build the pending points, then pop W(t_{p+1}) into nz
-    // (normalised increment per factor, path_generator.cpp:175-196)
-    template <class V>
-    __device__ __forceinline__ void step( const DevBook& B, const SobolLane& sob, int p, V& nz, int F )
+    // (normalised increment per factor, path_generator.cpp:175-196);
+    // point(o, f) is the Sobol point of op o, factor f (dimension op_b[o] F + f)
+    template <class V, class Pt>
+    __device__ __forceinline__ void step( const DevBook& B, const Pt& point, int p, V& nz, int F )
     {
         constexpr bool F64 = std::is_same_v<R, double>;
         const int i = p + 1;
         const int o0 = B.step_ops[p], o1 = B.step_ops[p + 1];
         for ( int o = o0; o < o1; o++ )
         {
-            const int bidx = B.op_b[o];
             const bool left = as_const( B.op_left )[o] != 0;
             const int pos = __builtin_amdgcn_readfirstlane( sp - 1 );
             R lw, rw, sd;
@@ -434,44 +434,51 @@ struct BridgeWalk
                 rw = as_const( B.op_wf )[3 * o + 1];
                 sd = as_const( B.op_wf )[3 * o + 2];
             }
-            // FP32: the central polynomial for every factor straight-line,
-            // the tail lanes (w >= 8) redone in one block (normal_f32's values)
-            R zf[NF];
-            uint32_t tails = 0;
-#pragma unroll
-            for ( int f = 0; f < NF; f++ )
+            // FP32 with four or more factors: the central polynomial for
+            // every factor straight-line, the tail lanes (w >= 8) redone in
+            // one block (normal_f32's values); otherwise each normal where it
+            // is used (with two factors, of which the book may use one, the
+            // staging and mask bookkeeping cost more than the branch)
+            constexpr bool central = !F64 && NF >= 4;
+            auto build = [&]( int f, R z ) __attribute__( ( always_inline ) )
             {
-                zf[f] = R( 0 );
-                if ( f < F )
+                const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
+                stk_set( f, pos, top[f] );
+                top[f] = nv;
+            };
+            if constexpr ( central )
+            {
+                R zf[NF];
+                uint32_t tails = 0;
+#pragma unroll
+                for ( int f = 0; f < NF; f++ )
                 {
-                    if constexpr ( F64 )
-                        zf[f] = normal_of<R>( sob( bidx * F + f ) );
-                    else
+                    zf[f] = R( 0 );
+                    if ( f < F )
                     {
                         bool t;
-                        zf[f] = normal_f32_central( sob( bidx * F + f ), t );
+                        zf[f] = normal_f32_central( point( o, f ), t );
                         tails |= t ? 1u << f : 0u;
                     }
                 }
-            }
-            if constexpr ( !F64 )
                 if ( __builtin_amdgcn_ballot_w64( tails != 0 ) )
                 {
 #pragma unroll
                     for ( int f = 0; f < NF; f++ )
                         if ( ( tails >> f ) & 1u )
-                            zf[f] = normal_f32( sob( bidx * F + f ) );
+                            zf[f] = normal_f32( point( o, f ) );
                 }
 #pragma unroll
-            for ( int f = 0; f < NF; f++ )
+                for ( int f = 0; f < NF; f++ )
+                    if ( f < F )
+                        build( f, zf[f] );
+            }
+            else
             {
-                if ( f < F )
-                {
-                    const R z = zf[f];
-                    const R nv = left ? lw * prev[f] + rw * top[f] + sd * z : rw * top[f] + sd * z;
-                    stk_set( f, pos, top[f] );
-                    top[f] = nv;
-                }
+#pragma unroll
+                for ( int f = 0; f < NF; f++ )
+                    if ( f < F )
+                        build( f, normal_of<R>( point( o, f ) ) );
             }
             sp++;
         }
